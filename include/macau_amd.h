@@ -1,0 +1,125 @@
+/*
+ * macau_amd.h -- C ABI of the MI355X-native Gibbs sampler for BPMF / Macau.
+ *
+ * This is the drop-in boundary for the reference's sampling step.  The
+ * reference binds its C++ core from Cython (python/macau/macau.pxd:63-84);
+ * every entry point below replaces one of those bindings or one hot-path
+ * function of the C++ core, cited next to it (file:line in jaak-s/macau).
+ *
+ * Conventions
+ *   - plain C types only; arrays are host pointers, copied in and out (the
+ *     library never retains a caller pointer after the call returns);
+ *   - matrices are column-major like Eigen::MatrixXd (a K x N factor matrix
+ *     stores latent vector n as K contiguous doubles);
+ *   - functions returning int give 0 on success and -1 on error; the message
+ *     of the last error on the calling thread is macau_last_error();
+ *     reference exceptions keep their text (e.g. "Cholesky Decomposition
+ *     failed!", latentprior.cpp:375; "Less 2 priors are not supported.",
+ *     macau.cpp:162);
+ *   - one host thread per model; a model owns one HIP stream on device 0.
+ */
+#ifndef MACAU_AMD_H
+#define MACAU_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct macau_model macau_model;
+
+/* sampler kinds (macau_set_sampler) */
+#define MACAU_SAMPLER_PLAIN_FP32 0 /* one work-group per row, FP32 Gram       */
+#define MACAU_SAMPLER_BATCHED    1 /* nnz-bucketed, split-BF16 MFMA Gram      */
+
+const char* macau_last_error(void);
+const char* macau_version(void);
+
+/* ---- factories: make_macau_fixed / make_macau_adaptive / make_macau_probit
+ *      (macau.h:83-85, macau.cpp:294-315; macau.pxd:82-84) ---- */
+macau_model* macau_make_fixed(int nmodes, int num_latent, double precision);
+macau_model* macau_make_adaptive(int nmodes, int num_latent, double sn_init, double sn_max);
+/* probit noise is not provided by this library: returns NULL with an error */
+macau_model* macau_make_probit(int nmodes, int num_latent);
+void macau_free(macau_model* m); /* replaces `del macau` (macau.pyx:449) */
+
+/* ---- priors: BPMFPrior(num_latent) (latentprior.h:53-79, macau.pxd:43-47)
+ *      and MacauPrior<SparseDoubleFeat> (latentprior.h:82-125,
+ *      macau.pyx:186-192; side information as COO, rows = data mode) ---- */
+int macau_add_bpmf_prior(macau_model* m);
+int macau_add_macau_prior(macau_model* m, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
+                          const double* vals /* NULL = binary */, double lambda_beta, double tol);
+
+/* ---- Macau API (macau.h:35-53; macau.pxd:61-76) ---- */
+int macau_set_samples(macau_model* m, int burnin, int nsamples);
+/* idx: nnz x nmodes int32, column-major (toMatrix, sparsetensor.cpp:323-331) */
+int macau_set_relation_data(macau_model* m, const int* idx, int nmodes, const double* values, int nnz,
+                            const int* dims);
+int macau_set_relation_data_test(macau_model* m, const int* idx, int nmodes, const double* values, int nnz,
+                                 const int* dims);
+int macau_set_verbose(macau_model* m, int verbose);
+int macau_set_save_model(macau_model* m, int save);
+int macau_set_save_prefix(macau_model* m, const char* prefix);
+int macau_run(macau_model* m); /* MacauX::run (macau.cpp:180-235) */
+double macau_get_rmse_test(macau_model* m);
+long long macau_get_test_nonzeros(macau_model* m);
+int macau_get_predictions(macau_model* m, double* out); /* ntest */
+int macau_get_stds(macau_model* m, double* out);        /* ntest (macau.cpp:261-275) */
+int macau_get_test_data(macau_model* m, double* out);   /* ntest x (nmodes+1), col-major */
+
+/* ---- extensions (no reference counterpart) ---- */
+int macau_set_seed(macau_model* m, unsigned long long seed);   /* counter-based RNG key */
+int macau_set_sampler(macau_model* m, int kind);               /* MACAU_SAMPLER_* */
+int macau_set_graph(macau_model* m, int on);                   /* replay iterations as a HIP graph */
+int macau_init(macau_model* m);                                /* MacauX::init (macau.cpp:158-173) */
+int macau_iterate(macau_model* m, int n);                      /* n Gibbs iterations, asynchronous */
+int macau_synchronize(macau_model* m);                         /* wait; raise sampler failures */
+int macau_get_iteration(macau_model* m);
+double macau_get_rmse_onesample(macau_model* m);
+double macau_get_rmse_avg(macau_model* m);
+double macau_get_noise_precision(macau_model* m);
+double macau_get_mean_value(macau_model* m);
+int macau_get_dims(macau_model* m, int* dims);
+/* current samples / hyperparameters of one mode, FP64 */
+int macau_get_latents(macau_model* m, int mode, double* out);              /* K x N_mode */
+int macau_get_hyper(macau_model* m, int mode, double* mu, double* Lambda); /* K, K x K */
+int macau_get_link(macau_model* m, int mode, double* beta);                /* K x nfeat */
+double macau_get_link_lambda(macau_model* m, int mode);
+int macau_get_last_cg_iters(macau_model* m, int mode);
+
+/* ---- stateless hot-path entry points ---- */
+
+/* One half-sweep: ILatentPrior::sample_latents(U, mat, mean, samples, alpha, K)
+ * (latentprior.h:19-20) for BPMFPrior (latentprior.cpp:69-77) and
+ * MacauPrior (mu_rows != NULL, latentprior.cpp:218-227), and the tensor form
+ * sample_latent_tensor (latentprior.cpp:118-161) when nother > 1.
+ *   rows: nrows + 1 offsets into idx (nnz x nother, row-major) and vals;
+ *   others[t]: K x other_n[t] column-major factors;
+ *   draws: normal k of row n is stream (c0 = n, iter, tag = (1 << 24) | mode). */
+int macau_sample_latents(int num_latent, int nother, int nrows, const long long* row_ptr, const int* idx,
+                         const double* vals, double mean, const double* const* others, const int* other_n,
+                         double alpha, const double* mu, const double* Lambda, const double* mu_rows,
+                         unsigned long long seed, int iter, int mode, int sampler, double* U_out);
+
+/* CondNormalWishart (mvnormal.cpp:314-336): U is K x N column-major. */
+int macau_cond_normal_wishart(int num_latent, int N, const double* U, const double* mu0, double kappa,
+                              const double* T, double nu, unsigned long long seed, int iter, int mode,
+                              double* mu_out, double* Lambda_out);
+
+/* solve_blockcg (linop.h:228-343; Python blockcg, macau.pyx:67-91): solves
+ * X (F^T F + reg I) = B for nrhs <= 40 right-hand sides; B and X are
+ * nrhs x nfeat column-major; F is COO nrow x ncol (vals NULL = binary).
+ * *iters receives the iteration count. */
+int macau_solve_blockcg(int nrhs, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
+                        const double* vals, double reg, const double* B, double tol, double* X, int* iters);
+
+/* RNG probes: Philox words, FP32 and FP64 normals of stream (c0, iter, tag)
+ * for j = 0..n-1, and FP64 gamma draws with shapes[j] at c0 = j. */
+int macau_rng_probe(unsigned long long seed, unsigned c0, unsigned iter, unsigned tag, int n, unsigned* bits,
+                    float* z32, double* z64);
+int macau_gamma_probe(unsigned long long seed, const double* shapes, int n, unsigned iter, unsigned tag,
+                      double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

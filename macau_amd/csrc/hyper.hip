@@ -1,0 +1,475 @@
+// hyper.hip -- FP64 hyperprior draw, adaptive noise, evaluation, RNG probes.
+//
+// Normal-Wishart conjugate update (CondNormalWishart, mvnormal.cpp:314-336):
+//   mu_c = (kappa mu0 + sum U) / (kappa + N),  kappa_c = kappa + N,
+//   nu_c = nu + N,  X = T + U U^T + kappa mu0 mu0^T - kappa_c mu_c mu_c^T,
+//   T_c = X^{-1};  Lambda ~ Wishart(T_c, nu_c) by Bartlett
+//   (mvnormal.cpp:217-266);  mu ~ N(mu_c, (kappa_c Lambda)^{-1})
+//   (MvNormal_prec, mvnormal.cpp:169-184).
+// The sufficient statistics are reduced over columns by many work-groups in
+// FP64; the K x K draw then runs in one work-group.
+#include "common.h"
+#include "kernels.h"
+
+namespace mgpu {
+
+constexpr int kStatThreads = 256;
+
+int hyper_stats_blocks(int N) {
+  int b = (N + 511) / 512;
+  if (b < 1) b = 1;
+  if (b > 256) b = 256;
+  return b;
+}
+
+// partial[blk] = (sum_n x_n (K), sum_n x_n x_n^T (K x K, col-major))
+template <typename T, int KP>
+__global__ void __launch_bounds__(kStatThreads)
+    k_hyper_stats(const T* __restrict__ U, const float* __restrict__ sub, int ld, int N, int K,
+                  double* __restrict__ partial) {
+  constexpr int ENT = (KP * KP + kStatThreads - 1) / kStatThreads;
+  constexpr int TC = 32;
+  __shared__ double tile[TC][KP + 1];
+  const int t = threadIdx.x;
+  const int chunk = (N + gridDim.x - 1) / gridDim.x;
+  const int c0 = blockIdx.x * chunk;
+  const int c1 = min(N, c0 + chunk);
+  const int KK = K * K;
+  int ii[ENT], jj[ENT];
+  double acc[ENT];
+#pragma unroll
+  for (int s = 0; s < ENT; s++) {
+    const int q = t + s * kStatThreads;
+    ii[s] = q < KK ? q % K : 0;
+    jj[s] = q < KK ? q / K : 0;
+    acc[s] = 0.0;
+  }
+  double sum = 0.0;
+  for (int base = c0; base < c1; base += TC) {
+    const int nc = min(TC, c1 - base);
+    for (int q = t; q < nc * K; q += kStatThreads) {
+      const int c = q / K, i = q % K;
+      double x = (double)U[(size_t)(base + c) * ld + i];
+      if (sub) x -= (double)sub[(size_t)(base + c) * ld + i];
+      tile[c][i] = x;
+    }
+    __syncthreads();
+    for (int c = 0; c < nc; c++) {
+#pragma unroll
+      for (int s = 0; s < ENT; s++) acc[s] += tile[c][ii[s]] * tile[c][jj[s]];
+      if (t < K) sum += tile[c][t];
+    }
+    __syncthreads();
+  }
+  double* out = partial + (size_t)blockIdx.x * (K + KK);
+  if (t < K) out[t] = sum;
+#pragma unroll
+  for (int s = 0; s < ENT; s++) {
+    const int q = t + s * kStatThreads;
+    if (q < KK) out[K + q] = acc[s];
+  }
+}
+
+template <typename T>
+static void launch_stats(const T* U, const float* sub, int ld, int N, int K, double* partial, int nblk,
+                         hipStream_t s) {
+  const int KP = pad_latent(K);
+  switch (KP) {
+    case 8: hipLaunchKernelGGL((k_hyper_stats<T, 8>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
+    case 16: hipLaunchKernelGGL((k_hyper_stats<T, 16>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
+    case 32: hipLaunchKernelGGL((k_hyper_stats<T, 32>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
+    case 64: hipLaunchKernelGGL((k_hyper_stats<T, 64>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
+    default: hipLaunchKernelGGL((k_hyper_stats<T, 128>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
+  }
+}
+
+void launch_hyper_stats_f32(const float* U, const float* sub, int ld, int N, int K, double* partial, int nblk,
+                            hipStream_t s) {
+  launch_stats<float>(U, sub, ld, N, K, partial, nblk, s);
+}
+void launch_hyper_stats_f64(const double* U, int ld, int N, int K, double* partial, int nblk, hipStream_t s) {
+  launch_stats<double>(U, nullptr, ld, N, K, partial, nblk, s);
+}
+
+// ------------------------------------------------------------------
+// Block-cooperative dense FP64 helpers (col-major, ld = K).
+// ------------------------------------------------------------------
+constexpr int kNWThreads = 256;
+
+// A = L L^T in place (lower), upper zeroed.  Returns false if not SPD.
+__device__ bool blk_chol(double* A, int K, int* sflag) {
+  const int t = threadIdx.x;
+  for (int k = 0; k < K; k++) {
+    if (t == 0) {
+      const double d = A[k + k * K];
+      if (!(d > 0.0)) *sflag = 1;
+      A[k + k * K] = sqrt(d > 0.0 ? d : 1.0);
+    }
+    __syncthreads();
+    const double d = A[k + k * K];
+    for (int i = k + 1 + t; i < K; i += kNWThreads) A[i + k * K] /= d;
+    __syncthreads();
+    const int m = K - k - 1;
+    for (int q = t; q < m * m; q += kNWThreads) {
+      const int i = k + 1 + q % m, j = k + 1 + q / m;
+      if (j <= i) A[i + j * K] -= A[i + k * K] * A[j + k * K];
+    }
+    __syncthreads();
+  }
+  for (int q = t; q < K * K; q += kNWThreads) {
+    const int i = q % K, j = q / K;
+    if (i < j) A[q] = 0.0;
+  }
+  __syncthreads();
+  return *sflag == 0;
+}
+
+// W = L^{-1} (lower): thread j solves L w = e_j
+__device__ void blk_tri_inv(const double* L, double* W, int K) {
+  const int j = threadIdx.x;
+  if (j < K) {
+    for (int i = 0; i < j; i++) W[i + j * K] = 0.0;
+    for (int i = j; i < K; i++) {
+      double s = (i == j) ? 1.0 : 0.0;
+      for (int k = j; k < i; k++) s -= L[i + k * K] * W[k + j * K];
+      W[i + j * K] = s / L[i + i * K];
+    }
+  }
+  __syncthreads();
+}
+
+// C = op(A) * op(B), op = transpose if flag set
+__device__ void blk_gemm(const double* A, bool ta, const double* B, bool tb, double* C, int K) {
+  for (int q = threadIdx.x; q < K * K; q += kNWThreads) {
+    const int i = q % K, j = q / K;
+    double s = 0.0;
+    for (int k = 0; k < K; k++) {
+      const double a = ta ? A[k + i * K] : A[i + k * K];
+      const double b = tb ? B[j + k * K] : B[k + j * K];
+      s += a * b;
+    }
+    C[q] = s;
+  }
+  __syncthreads();
+}
+
+// x <- L^{-T} x (x in shared memory), column-oriented back substitution
+__device__ void blk_trsv_upper_t(const double* L, double* x, int K) {
+  const int t = threadIdx.x;
+  for (int j = K - 1; j >= 0; j--) {
+    if (t == 0) x[j] = x[j] / L[j + j * K];
+    __syncthreads();
+    const double xj = x[j];
+    for (int i = t; i < j; i += kNWThreads) x[i] -= L[j + i * K] * xj;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWArgs a) {
+  extern __shared__ double dyn[];
+  __shared__ int sflag;
+  const int K = a.K, KK = K * K, t = threadIdx.x;
+  // matrices live in LDS when they fit, else in the global workspace
+  double* base = a.ws ? a.ws : dyn;
+  double* A = base;
+  double* B = A + KK;
+  double* D = B + KK;
+  double* v_mu = D + KK;
+  double* v_z = v_mu + K;
+  double* v_nu = v_z + K;
+  if (t == 0) sflag = 0;
+  const int it = *a.iter;
+  const double N = a.N;
+  const double kappa_c = a.kappa + N, nu_c = a.nu + N;
+  // 1. reduce the partial statistics (fixed order)
+  for (int q = t; q < K + KK; q += kNWThreads) {
+    double s = 0.0;
+    for (int b = 0; b < a.nblk; b++) s += a.partial[(size_t)b * (K + KK) + q];
+    if (q < K) v_nu[q] = s;
+    else A[q - K] = s;  // NS
+  }
+  // Macau: T = WI + lambda_beta * beta beta^T (latentprior.cpp:258-260)
+  const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
+  for (int q = t; q < KK; q += kNWThreads) {
+    double tq = a.WI[q];
+    if (a.BBt_partial) {
+      double s = 0.0;
+      for (int b = 0; b < a.bbt_nblk; b++) s += a.BBt_partial[(size_t)b * (K + KK) + K + q];
+      tq += lb * s;
+    }
+    B[q] = tq;
+  }
+  __syncthreads();
+  // 2. mu_c and X (mvnormal.cpp:322-325)
+  for (int k = t; k < K; k += kNWThreads) v_mu[k] = (a.kappa * a.mu0[k] + v_nu[k]) / (a.kappa + N);
+  __syncthreads();
+  for (int q = t; q < KK; q += kNWThreads) {
+    const int i = q % K, j = q / K;
+    A[q] = B[q] + A[q] + a.kappa * a.mu0[i] * a.mu0[j] - kappa_c * v_mu[i] * v_mu[j];
+  }
+  __syncthreads();
+  // 3. T_c = X^{-1} through the Cholesky factor (mvnormal.cpp:326)
+  bool ok = blk_chol(A, K, &sflag);
+  blk_tri_inv(A, B, K);                 // B = L_X^{-1}
+  blk_gemm(B, true, B, false, A, K);    // A = T_c = L_X^{-T} L_X^{-1}
+  // 4. Wishart(T_c, nu_c): L = chol(T_c), Lambda = L (C^T C) L^T
+  ok = blk_chol(A, K, &sflag) && ok;
+  for (int q = t; q < KK; q += kNWThreads) {
+    const int i = q % K, j = q / K;
+    double c = 0.0;
+    if (i == j) {
+      const double g = gamma_f64(a.seed, 0.5 * (nu_c - i), (uint32_t)i, (uint32_t)it,
+                                 make_tag(TAG_WISHART_CHI, a.mode));
+      c = sqrt(2.0 * g);
+    } else if (j > i) {
+      c = normal_f64(a.seed, (uint32_t)i, (uint32_t)j, (uint32_t)it, make_tag(TAG_WISHART_Z, a.mode));
+    }
+    B[q] = c;
+  }
+  __syncthreads();
+  blk_gemm(B, true, B, false, D, K);    // D = AU = C^T C
+  blk_gemm(A, false, D, false, B, K);   // B = L AU
+  blk_gemm(B, false, A, true, D, K);    // D = Lambda = L AU L^T
+  // 5. mu = mu_c + chol(kappa_c Lambda)^{-T} z
+  for (int q = t; q < KK; q += kNWThreads) B[q] = D[q] * kappa_c;
+  for (int k = t; k < K; k += kNWThreads)
+    v_z[k] = normal_f64(a.seed, 0u, (uint32_t)k, (uint32_t)it, make_tag(TAG_NW_MU, a.mode));
+  __syncthreads();
+  ok = blk_chol(B, K, &sflag) && ok;
+  blk_trsv_upper_t(B, v_z, K);
+  for (int k = t; k < K; k += kNWThreads) {
+    const double m = v_mu[k] + v_z[k];
+    a.mu[k] = m;
+    v_mu[k] = m;
+  }
+  __syncthreads();
+  // 6. outputs: FP64 state and the FP32 sampler parameters
+  const int KP = a.KP;
+  for (int q = t; q < KK; q += kNWThreads) a.Lambda[q] = D[q];
+  for (int q = t; q < KP * KP; q += kNWThreads) {
+    const int i = q % KP, j = q / KP;
+    float v = (i == j) ? 1.f : 0.f;
+    if (i < K && j < K) v = (float)D[i + j * K];
+    a.Lambda_f[q] = v;
+  }
+  for (int i = t; i < KP; i += kNWThreads) {
+    double s = 0.0;
+    if (i < K)
+      for (int j = 0; j < K; j++) s += D[i + j * K] * v_mu[j];
+    a.Lmu_f[i] = (float)s;
+  }
+  if (a.Lchol) {
+    __syncthreads();
+    for (int q = t; q < KK; q += kNWThreads) A[q] = D[q];
+    __syncthreads();
+    ok = blk_chol(A, K, &sflag) && ok;
+    for (int q = t; q < KK; q += kNWThreads) a.Lchol[q] = A[q];
+  }
+  if (t == 0 && !ok && a.fail) atomicMax(a.fail, 1 << 30);
+}
+
+void launch_nw_draw(const NWArgs& a, hipStream_t s) {
+  const size_t bytes = (size_t)(3 * a.K * a.K + 3 * a.K) * sizeof(double);
+  if (a.ws) {
+    hipLaunchKernelGGL(k_nw_draw, dim3(1), dim3(kNWThreads), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_nw_draw, dim3(1), dim3(kNWThreads), bytes, s, a);
+  }
+}
+
+// ------------------------------------------------------------------
+// Adaptive noise: SSE over the training cells (noisemodels.cpp:55-124)
+// ------------------------------------------------------------------
+constexpr int kNoiseThreads = 256;
+
+int noise_sse_blocks(int N0) {
+  int b = (N0 + 3) / 4;
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return b;
+}
+
+template <int KP>
+__global__ void __launch_bounds__(kNoiseThreads) k_noise_sse(NoiseArgs a) {
+  __shared__ float us[kNoiseThreads / 64][KP];
+  __shared__ double red[kNoiseThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = gridDim.x * (kNoiseThreads / 64);
+  const int no = a.nmodes - 1;
+  double acc = 0.0;
+  for (int row = blockIdx.x * (kNoiseThreads / 64) + w; row < a.rows0.N; row += nw) {
+    for (int i = lane; i < KP; i += 64) us[w][i] = a.U0[(size_t)row * KP + i];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int beg = a.rows0.row_ptr[row], end = a.rows0.row_ptr[row + 1];
+    for (int e = beg + lane; e < end; e += 64) {
+      float dot = 0.f;
+      const int* ix = a.rows0.idx + (size_t)e * no;
+      for (int d = 0; d < a.K; d++) {
+        float v = us[w][d];
+        for (int m = 0; m < no; m++) v *= a.others.f[m][(size_t)ix[m] * KP + d];
+        dot += v;
+      }
+      const double r = (double)dot - (double)a.rows0.r[e];
+      acc += r * r;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) red[w] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < kNoiseThreads / 64; q++) s += red[q];
+    a.partial[blockIdx.x] = s;
+  }
+}
+
+void launch_noise_sse(const NoiseArgs& a, hipStream_t s) {
+  switch (a.KP) {
+    case 8: hipLaunchKernelGGL((k_noise_sse<8>), dim3(a.nblk), dim3(kNoiseThreads), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_noise_sse<16>), dim3(a.nblk), dim3(kNoiseThreads), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_noise_sse<32>), dim3(a.nblk), dim3(kNoiseThreads), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_noise_sse<64>), dim3(a.nblk), dim3(kNoiseThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_noise_sse<128>), dim3(a.nblk), dim3(kNoiseThreads), 0, s, a); break;
+  }
+}
+
+// alpha ~ Gamma(a0 + nnz/2, 1/(b0 + SSE/2)), clamped to alpha_max
+__global__ void k_noise_draw(NoiseDrawArgs a) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < a.nblk; b += 256) s += a.partial[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double sse = red[0];
+    const double aN = 0.5 + a.nnz / 2.0;
+    const double bN = a.b0 + sse / 2.0;
+    double al = gamma_f64(a.seed, aN, 0u, (uint32_t)*a.iter, make_tag(TAG_NOISE, 0)) / bN;
+    if (al > a.alpha_max) al = a.alpha_max;
+    *a.alpha = al;
+    if (a.sse_out) *a.sse_out = sse;
+  }
+}
+
+void launch_noise_draw(const NoiseDrawArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_noise_draw, dim3(1), dim3(256), 0, s, a);
+}
+
+// ------------------------------------------------------------------
+// Evaluation: eval_rmse / eval_rmse_tensor (bpmfutils.h:86-114,
+// bpmfutils.cpp:10-73) with Welford running mean and variance.
+// ------------------------------------------------------------------
+constexpr int kEvalThreads = 256;
+int eval_blocks(int ntest) {
+  int b = (ntest + kEvalThreads * 8 - 1) / (kEvalThreads * 8);
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return b;
+}
+
+__global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
+  __shared__ double red[2][kEvalThreads];
+  const int it = *a.iter;
+  const int n = it < a.burnin ? 0 : it - a.burnin;
+  double se = 0.0, se_avg = 0.0;
+  for (int q = blockIdx.x * kEvalThreads + threadIdx.x; q < a.ntest; q += gridDim.x * kEvalThreads) {
+    const int* ix = a.idx + (size_t)q * a.nmodes;
+    float dot = 0.f;
+    for (int d = 0; d < a.K; d++) {
+      float v = a.f[0][(size_t)ix[0] * a.KP + d];
+      for (int m = 1; m < a.nmodes; m++) v *= a.f[m][(size_t)ix[m] * a.KP + d];
+      dot += v;
+    }
+    const double p = a.mean + (double)dot;
+    double pa;
+    if (n == 0) {
+      pa = p;
+      a.pred_var[q] = 0.0;
+    } else {
+      const double delta = p - a.pred[q];
+      pa = a.pred[q] + delta / (n + 1);
+      a.pred_var[q] += delta * (p - pa);
+    }
+    a.pred[q] = pa;
+    const double y = a.vals[q];
+    se += (y - p) * (y - p);
+    se_avg += (y - pa) * (y - pa);
+  }
+  red[0][threadIdx.x] = se;
+  red[1][threadIdx.x] = se_avg;
+  __syncthreads();
+  for (int off = kEvalThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + off];
+      red[1][threadIdx.x] += red[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.partial[2 * blockIdx.x] = red[0][0];
+    a.partial[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+__global__ void k_eval_finish(EvalArgs a) {
+  if (threadIdx.x == 0 && a.ntest == 0) {
+    a.out[0] = NAN;  // sqrt(0 / 0) (bpmfutils.h:110-112)
+    a.out[1] = NAN;
+  } else if (threadIdx.x == 0) {
+    double se = 0.0, sa = 0.0;
+    for (int b = 0; b < a.nblk; b++) {
+      se += a.partial[2 * b];
+      sa += a.partial[2 * b + 1];
+    }
+    a.out[0] = sqrt(se / (double)a.ntest);
+    a.out[1] = sqrt(sa / (double)a.ntest);
+  }
+}
+
+void launch_eval(const EvalArgs& a, hipStream_t s) {
+  if (a.ntest > 0) hipLaunchKernelGGL(k_eval, dim3(a.nblk), dim3(kEvalThreads), 0, s, a);
+  hipLaunchKernelGGL(k_eval_finish, dim3(1), dim3(64), 0, s, a);
+}
+
+__global__ void k_advance_iter(int* iter) {
+  if (threadIdx.x == 0) *iter += 1;
+}
+void launch_advance_iter(int* iter, hipStream_t s) { hipLaunchKernelGGL(k_advance_iter, dim3(1), dim3(64), 0, s, iter); }
+
+// ------------------------------------------------------------------
+// RNG probes used by the parity tests
+// ------------------------------------------------------------------
+__global__ void k_rng_probe(uint64_t seed, uint32_t c0, uint32_t it, uint32_t tag, int n, uint32_t* bits,
+                            float* z32, double* z64) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  if ((j & 3) == 0) {
+    uint32_t w[4];
+    philox_at(seed, c0, (uint32_t)(j >> 2), it, tag, w);
+    for (int q = 0; q < 4 && j + q < n; q++) bits[j + q] = w[q];
+  }
+  z32[j] = normal_f32(seed, c0, (uint32_t)j, it, tag);
+  z64[j] = normal_f64(seed, c0, (uint32_t)j, it, tag);
+}
+void launch_rng_probe(uint64_t seed, uint32_t c0, uint32_t it, uint32_t tag, int n, uint32_t* bits, float* z32,
+                      double* z64, hipStream_t s) {
+  hipLaunchKernelGGL(k_rng_probe, dim3((n + 255) / 256), dim3(256), 0, s, seed, c0, it, tag, n, bits, z32, z64);
+}
+
+__global__ void k_gamma_probe(uint64_t seed, const double* shapes, int n, uint32_t it, uint32_t tag, double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = gamma_f64(seed, shapes[j], (uint32_t)j, it, tag);
+}
+void launch_gamma_probe(uint64_t seed, const double* shapes, int n, uint32_t it, uint32_t tag, double* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_gamma_probe, dim3((n + 255) / 256), dim3(256), 0, s, seed, shapes, n, it, tag, out);
+}
+
+}  // namespace mgpu

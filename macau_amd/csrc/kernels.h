@@ -1,0 +1,195 @@
+// kernels.h -- host-side launch wrappers for the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mgpu {
+
+constexpr int kMaxOther = 4;  // up to 5-mode tensors
+
+// Observed cells of one mode, grouped by that mode's index (the reference's
+// SparseMode / Eigen CSC column view, sparsetensor.h:76-91).
+struct RowsDev {
+  int N = 0;                  // rows of this mode
+  int nother = 1;             // other modes
+  const int* row_ptr = nullptr;  // N + 1
+  const int* idx = nullptr;      // nnz * nother, row-major, view order
+  const float* r = nullptr;      // nnz: y - mean, rounded to FP32
+};
+
+struct FactorSet {
+  const float* f[kMaxOther] = {nullptr, nullptr, nullptr, nullptr};  // KP x N_m
+};
+
+// Work list of the batched sampler (rows sorted by nnz, heavy rows split).
+struct BatchPlan {
+  int n_light = 0;             // light rows (sorted by nnz, descending)
+  const int* light = nullptr;  // row ids
+  int n_seg = 0;               // segments of heavy rows
+  const int* seg_row = nullptr;    // heavy-row slot of each segment
+  const int* seg_beg = nullptr;    // first nnz
+  const int* seg_end = nullptr;    // one past last nnz
+  int n_heavy = 0;
+  const int* heavy = nullptr;      // row id of each heavy slot
+  const int* heavy_seg0 = nullptr; // first segment of each heavy slot (n_heavy + 1)
+  float* partial = nullptr;        // per-segment Gram (KP*KP) and rhs (KP)
+};
+
+struct SampleArgs {
+  RowsDev rows;
+  FactorSet others;
+  const double* alpha = nullptr;   // device scalar: noise precision
+  const float* Lambda_f = nullptr; // KP x KP, pad diagonal = 1
+  const float* Lmu_f = nullptr;    // KP: Lambda mu (shared by all rows)
+  const float* Lmu_rows = nullptr; // KP x N: Lambda (mu + Uhat_n), or null
+  float* U = nullptr;              // KP x N, written
+  int K = 0, KP = 0;
+  uint64_t seed = 0;
+  const int* iter = nullptr;       // device iteration counter
+  int mode = 0;
+  int* fail = nullptr;             // device flag: row + 1 of a failed Cholesky
+  // algebraic train SSE of this half-sweep (batched sampler only), or null
+  double* sse_partial = nullptr;
+  const float* r2 = nullptr;       // per-row sum of squared residuals
+};
+
+void launch_sample_plain(const SampleArgs& a, hipStream_t s);
+void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
+int batched_supported(int KP, int nother);
+
+// --- hyperprior (FP64) ---
+// partial sums of x = U[:,n] - sub[:,n] (sub may be null): sum (K) and
+// scatter (K x K) per block.  Returns the number of blocks used.
+int hyper_stats_blocks(int N);
+void launch_hyper_stats_f32(const float* U, const float* sub, int ld, int N, int K, double* partial, int nblk,
+                            hipStream_t s);
+void launch_hyper_stats_f64(const double* U, int ld, int N, int K, double* partial, int nblk, hipStream_t s);
+
+struct NWArgs {
+  int K = 0, KP = 0;
+  const double* partial = nullptr;  // nblk x (K + K*K)
+  int nblk = 0;
+  double N = 0;                     // number of columns
+  double kappa = 2.0;
+  const double* mu0 = nullptr;      // K
+  const double* WI = nullptr;       // K x K base scale
+  const double* BBt_partial = nullptr;  // Macau: partial stats of beta (or null)
+  int bbt_nblk = 0;
+  const double* lambda_beta = nullptr;  // device scalar (Macau)
+  double nu = 0;                    // degrees of freedom
+  uint64_t seed = 0;
+  const int* iter = nullptr;
+  int mode = 0;
+  double* ws = nullptr;             // workspace, 6*K*K + 4*K doubles
+  double* mu = nullptr;             // out K
+  double* Lambda = nullptr;         // out K x K
+  double* Lchol = nullptr;          // out chol(Lambda), K x K (or null)
+  float* Lambda_f = nullptr;        // out KP x KP
+  float* Lmu_f = nullptr;           // out KP
+  int* fail = nullptr;
+};
+void launch_nw_draw(const NWArgs& a, hipStream_t s);
+
+// --- noise and evaluation ---
+struct NoiseArgs {
+  RowsDev rows0;      // mode-0 rows
+  const float* U0 = nullptr;
+  FactorSet others;
+  int K = 0, KP = 0, nmodes = 2;
+  double* partial = nullptr;  // per-block SSE
+  int nblk = 0;
+};
+int noise_sse_blocks(int N0);
+void launch_noise_sse(const NoiseArgs& a, hipStream_t s);
+struct NoiseDrawArgs {
+  const double* partial = nullptr;
+  int nblk = 0;
+  double nnz = 0, var_total = 1, b0 = 0.5, alpha_max = 0;
+  uint64_t seed = 0;
+  const int* iter = nullptr;
+  double* alpha = nullptr;
+  double* sse_out = nullptr;
+};
+void launch_noise_draw(const NoiseDrawArgs& a, hipStream_t s);
+
+struct EvalArgs {
+  int ntest = 0, nmodes = 2, K = 0, KP = 0;
+  const int* idx = nullptr;      // ntest x nmodes
+  const double* vals = nullptr;  // ntest
+  const float* f[kMaxOther + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double mean = 0;
+  double* pred = nullptr;
+  double* pred_var = nullptr;
+  double* partial = nullptr;     // per-block (se, se_avg)
+  int nblk = 0;
+  const int* iter = nullptr;
+  int burnin = 0;
+  double* out = nullptr;         // (rmse1, rmse_avg)
+};
+int eval_blocks(int ntest);
+void launch_eval(const EvalArgs& a, hipStream_t s);
+
+void launch_advance_iter(int* iter, hipStream_t s);
+
+// --- RNG probes (parity tests) ---
+void launch_rng_probe(uint64_t seed, uint32_t c0, uint32_t it, uint32_t tag, int n, uint32_t* bits, float* z32,
+                      double* z64, hipStream_t s);
+void launch_gamma_probe(uint64_t seed, const double* shapes, int n, uint32_t it, uint32_t tag, double* out,
+                        hipStream_t s);
+
+// --- sparse linear algebra for the link matrix (FP64) ---
+struct CsrDev {
+  int nrow = 0, ncol = 0;
+  const int* row_ptr = nullptr;
+  const int* cols = nullptr;
+  const double* vals = nullptr;  // null = binary
+};
+// out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr;
+// reg is a device scalar, read only when add is non-null
+void launch_spmm(const CsrDev& A, const double* in, int nr, double* out, const double* reg, const double* add,
+                 hipStream_t s);
+// copy rows [r0, r0 + nr) of a (K x n, ld K) matrix to (nr x n, ld nr), or back
+void launch_rows_copy(const double* src, int lds, double* dst, int ldd, int r0s, int r0d, int nr, int n, hipStream_t s);
+// sum of squares of n values (partial per block, FP64)
+void launch_sumsq_f32(const float* x, size_t n, double* partial, int nblk, hipStream_t s);
+void launch_sumsq_f64(const double* x, size_t n, double* partial, int nblk, hipStream_t s);
+// partial lower dots: per block nr*nr of sum_f a[:,f] b[:,f]^T
+int dots_blocks(int n);
+void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s);
+
+struct CgState {
+  int nr = 0, n = 0;
+  double* X = nullptr;
+  double* R = nullptr;
+  double* P = nullptr;
+  double* KPm = nullptr;
+  double* inner = nullptr;
+  double* partial = nullptr;
+  int nblk = 0;
+  double* small = nullptr;  // RtR, RtR2, PtKP, A, Psi, norms, flags
+};
+void launch_cg_init(const CgState& c, const double* B, hipStream_t s);
+void launch_cg_step_a(const CgState& c, hipStream_t s);   // A from PtKP partials
+void launch_cg_update_xr(const CgState& c, hipStream_t s);
+void launch_cg_step_b(const CgState& c, double tolsq, hipStream_t s);  // convergence + Psi
+void launch_cg_update_p(const CgState& c, hipStream_t s);
+void launch_cg_finish(const CgState& c, hipStream_t s);
+
+struct BetaArgs {
+  int K = 0, KP = 0, N = 0, nfeat = 0;
+  const float* U = nullptr;
+  const double* mu = nullptr;
+  const double* Lchol = nullptr;
+  const double* lambda_beta = nullptr;
+  uint64_t seed = 0;
+  const int* iter = nullptr;
+  int mode = 0;
+};
+void launch_beta_tmp(const BetaArgs& a, double* tmp, hipStream_t s);
+void launch_beta_noise_add(const BetaArgs& a, double* Fty, hipStream_t s);
+void launch_uhat_finish(const CsrDev& F, const double* beta, int K, int KP, const double* mu, const double* Lambda,
+                        float* Uhat, float* Lmu_rows, hipStream_t s);
+void launch_lambda_beta(int K, double nfeat, const double* bbt_partial, int nblk, const double* Lambda,
+                        double* lambda_beta, uint64_t seed, const int* iter, int mode, hipStream_t s);
+
+}  // namespace mgpu

@@ -1,0 +1,901 @@
+// model.cpp -- Gibbs driver, data layer, priors and noise models.
+//
+// Control flow follows MacauX::run (macau.cpp:180-235): per iteration,
+// sample every mode's latents, then update every prior, then the noise
+// precision, then evaluate.  All state stays on the GPU; one iteration is a
+// fixed sequence of kernel launches on one stream, captured once into a HIP
+// graph and replayed when no host round trip is needed (no link-matrix CG).
+#include "model.h"
+
+#include <signal.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <numeric>
+#include <sstream>
+
+namespace mgpu {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+  }
+}
+
+void DevBuf::alloc(size_t bytes) {
+  release();
+  if (bytes == 0) bytes = 16;
+  HIPCHK(hipMalloc(&p_, bytes));
+  n_ = bytes;
+}
+void DevBuf::release() {
+  if (p_) (void)hipFree(p_);
+  p_ = nullptr;
+  n_ = 0;
+}
+
+template <typename T>
+static void upload_scalar(DevBuf& b, T v, hipStream_t s) {
+  if (!b.as<void>()) b.alloc(sizeof(T));
+  HIPCHK(hipMemcpyAsync(b.as<void>(), &v, sizeof(T), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+template <typename T>
+static T download_scalar(const DevBuf& b, hipStream_t s) {
+  T v;
+  HIPCHK(hipMemcpyAsync(&v, b.as<void>(), sizeof(T), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return v;
+}
+
+static double tick() {
+  return std::chrono::duration<double>(std::chrono::high_resolution_clock::now().time_since_epoch()).count();
+}
+
+static std::string to_string_with_precision(double v, int n) {  // bpmfutils.h:175-180
+  std::ostringstream out;
+  out << std::setprecision(n) << v;
+  return out.str();
+}
+
+// writeToCSVfile (bpmfutils.h:169-173): 6 significant digits, rows by "\n"
+static void write_csv(const std::string& fn, const std::vector<double>& colmajor, int rows, long long cols) {
+  std::ofstream f(fn.c_str());
+  f << std::setprecision(6);
+  for (int i = 0; i < rows; i++) {
+    for (long long j = 0; j < cols; j++) {
+      f << colmajor[(size_t)j * rows + i];
+      if (j + 1 < cols) f << ",";
+    }
+    if (i + 1 < rows) f << "\n";
+  }
+}
+
+// ------------------------------------------------------------------
+// data layer (sparsetensor.cpp:11-60, 78-117; bpmfutils.h:57-82)
+// ------------------------------------------------------------------
+struct Cell {
+  long long key;
+  int pos;
+};
+
+// Eigen setFromTriplets: duplicates summed; returns cells sorted by
+// (outer, inner) with outer = column (CSC).
+static void sum_duplicates(const int* idx, int nnz, const double* vals, const int* dims, std::vector<int>& r,
+                           std::vector<int>& c, std::vector<double>& v) {
+  std::vector<Cell> cells(nnz);
+  for (int e = 0; e < nnz; e++) {
+    const int row = idx[e], col = idx[nnz + e];
+    if (row < 0 || row >= dims[0] || col < 0 || col >= dims[1])
+      throw std::runtime_error("MatrixData: index out of range");
+    cells[e] = {(long long)col * dims[0] + row, e};
+  }
+  std::stable_sort(cells.begin(), cells.end(), [](const Cell& a, const Cell& b) { return a.key < b.key; });
+  r.clear();
+  c.clear();
+  v.clear();
+  for (int e = 0; e < nnz; e++) {
+    if (e > 0 && cells[e].key == cells[e - 1].key) {
+      v.back() += vals[cells[e].pos];
+      continue;
+    }
+    r.push_back((int)(cells[e].key % dims[0]));
+    c.push_back((int)(cells[e].key / dims[0]));
+    v.push_back(vals[cells[e].pos]);
+  }
+}
+
+// group cells by key (stable), keeping the other indices in view order
+static HostRows group_rows(const std::vector<int>& key, int nkeys, const std::vector<int>& other, int nother,
+                           const std::vector<double>& vals) {
+  HostRows h;
+  const size_t n = key.size();
+  h.row_ptr.assign(nkeys + 1, 0);
+  for (size_t e = 0; e < n; e++) h.row_ptr[key[e] + 1]++;
+  for (int i = 0; i < nkeys; i++) h.row_ptr[i + 1] += h.row_ptr[i];
+  std::vector<int> pos(h.row_ptr.begin(), h.row_ptr.end() - 1);
+  h.idx.resize(n * nother);
+  h.vals.resize(n);
+  for (size_t e = 0; e < n; e++) {
+    const int d = pos[key[e]]++;
+    for (int t = 0; t < nother; t++) h.idx[(size_t)d * nother + t] = other[e * nother + t];
+    h.vals[d] = vals[e];
+  }
+  return h;
+}
+
+void Model::setRelationData(const int* idx, int nm, const double* vals, int nnz_in, const int* d) {
+  if (nm != nmodes) throw std::runtime_error("setRelationData: number of modes differs from the model");
+  dims.assign(d, d + nm);
+  rows.clear();
+  if (!is_tensor) {
+    if (nm != 2) throw std::runtime_error("MatrixData: tensor training input not supported.");
+    std::vector<int> r, c;
+    std::vector<double> v;
+    sum_duplicates(idx, nnz_in, vals, d, r, c, v);
+    nnz = (long long)v.size();
+    double s = 0.0;
+    for (double x : v) s += x;
+    mean_value = s / (double)nnz;  // Y.sum() / Y.nonZeros() (sparsetensor.cpp:202)
+    // mode 0: rows of Y, column indices ascending (Yt columns)
+    std::vector<size_t> order(r.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return r[a] < r[b]; });
+    std::vector<int> k0(r.size()), o0(r.size());
+    std::vector<double> v0(r.size());
+    for (size_t q = 0; q < order.size(); q++) {
+      k0[q] = r[order[q]];
+      o0[q] = c[order[q]];
+      v0[q] = v[order[q]];
+    }
+    rows.push_back(group_rows(k0, d[0], o0, 1, v0));
+    // mode 1: columns of Y, row indices ascending (Y columns)
+    rows.push_back(group_rows(c, d[1], r, 1, v));
+  } else {
+    for (int e = 0; e < nnz_in; e++)
+      for (int m = 0; m < nm; m++)
+        if (idx[(size_t)m * nnz_in + e] < 0 || idx[(size_t)m * nnz_in + e] >= d[m])
+          throw std::runtime_error("SparseMode: mode value larger than mode_size");
+    nnz = nnz_in;
+    double s = 0.0;
+    for (int e = 0; e < nnz_in; e++) s += vals[e];
+    mean_value = s / (double)nnz_in;  // vals.mean() (sparsetensor.cpp:247)
+    std::vector<double> v(vals, vals + nnz_in);
+    for (int m = 0; m < nm; m++) {
+      std::vector<int> key(nnz_in), other((size_t)nnz_in * (nm - 1));
+      for (int e = 0; e < nnz_in; e++) {
+        key[e] = idx[(size_t)m * nnz_in + e];
+        int t = 0;
+        for (int q = 0; q < nm; q++)
+          if (q != m) other[(size_t)e * (nm - 1) + t++] = idx[(size_t)q * nnz_in + e];
+      }
+      rows.push_back(group_rows(key, d[m], other, nm - 1, v));
+    }
+  }
+}
+
+void Model::setRelationDataTest(const int* idx, int nm, const double* vals, int nnz_in, const int* d) {
+  if (nm != nmodes) throw std::runtime_error("setRelationDataTest: number of modes differs from the model");
+  has_test = true;
+  test_idx.clear();
+  test_vals.clear();
+  if (!is_tensor) {
+    std::vector<int> r, c;
+    std::vector<double> v;
+    sum_duplicates(idx, nnz_in, vals, d, r, c, v);  // Ytest in CSC order
+    for (size_t q = 0; q < v.size(); q++) {
+      test_idx.push_back(r[q]);
+      test_idx.push_back(c[q]);
+    }
+    test_vals = v;
+  } else {
+    for (int m = 0; m < nm; m++)
+      if (!dims.empty() && d[m] != dims[m])
+        throw std::runtime_error("setTest(): train and test Tensor sizes are not equal.");
+    for (int e = 0; e < nnz_in; e++)
+      for (int m = 0; m < nm; m++)
+        if (idx[(size_t)m * nnz_in + e] < 0 || idx[(size_t)m * nnz_in + e] >= d[m])
+          throw std::runtime_error("SparseMode: mode value larger than mode_size");
+    std::vector<int> order(nnz_in);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return idx[a] < idx[b]; });
+    for (int q : order) {
+      for (int m = 0; m < nm; m++) test_idx.push_back(idx[(size_t)m * nnz_in + q]);
+      test_vals.push_back(vals[q]);
+    }
+  }
+  ntest = (long long)test_vals.size();
+}
+
+// ------------------------------------------------------------------
+// Model
+// ------------------------------------------------------------------
+Model::Model(int nm, int K, std::unique_ptr<INoiseModel> nz)
+    : nmodes(nm), num_latent(K), KP(pad_latent(K)), noise(std::move(nz)) {
+  is_tensor = nm != 2;
+  if (K < 1 || K > 128) throw std::runtime_error("num_latent must be in [1, 128]");
+  if (nm < 2 || nm > kMaxOther + 1) throw std::runtime_error("number of modes must be in [2, 5]");
+}
+
+Model::~Model() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+FactorSet Model::others_of(int mode) const {
+  FactorSet f;
+  int t = 0;
+  for (int q = 0; q < nmodes; q++)
+    if (q != mode) f.f[t++] = d_U[q].as<float>();
+  return f;
+}
+
+void Model::init() {
+  if ((int)priors.size() < 2) throw std::runtime_error("Less 2 priors are not supported.");
+  if ((int)priors.size() != nmodes)
+    throw std::runtime_error("number of priors must equal the number of data modes");
+  if (rows.empty()) throw std::runtime_error("training data not set");
+  if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+  if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  d_rowptr.clear(); d_idx.clear(); d_r.clear(); d_U.clear(); rdev.clear();
+  for (int m = 0; m < nmodes; m++) {
+    const HostRows& h = rows[m];
+    std::vector<float> r(h.vals.size());
+    for (size_t e = 0; e < r.size(); e++) r[e] = (float)(h.vals[e] - mean_value);
+    d_rowptr.emplace_back();
+    d_idx.emplace_back();
+    d_r.emplace_back();
+    upload(d_rowptr.back(), h.row_ptr, stream);
+    upload(d_idx.back(), h.idx, stream);
+    upload(d_r.back(), r, stream);
+    d_U.emplace_back((size_t)KP * dims[m] * sizeof(float));
+    HIPCHK(hipMemsetAsync(d_U.back().as<void>(), 0, (size_t)KP * dims[m] * sizeof(float), stream));
+    RowsDev rd;
+    rd.N = dims[m];
+    rd.nother = nmodes - 1;
+    rd.row_ptr = d_rowptr.back().as<int>();
+    rd.idx = d_idx.back().as<int>();
+    rd.r = d_r.back().as<float>();
+    rdev.push_back(rd);
+    HIPCHK(hipStreamSynchronize(stream));  // host vectors r go out of scope
+  }
+  upload_scalar<int>(d_iter, 0, stream);
+  upload_scalar<int>(d_fail, 0, stream);
+  d_eval_out.alloc(2 * sizeof(double));
+  d_sse.alloc(sizeof(double));
+  d_alpha.alloc(sizeof(double));
+  // test cells
+  upload(d_test_idx, test_idx, stream);
+  upload(d_test_vals, test_vals, stream);
+  d_pred.alloc(std::max<long long>(ntest, 1) * sizeof(double));
+  d_pred_var.alloc(std::max<long long>(ntest, 1) * sizeof(double));
+  HIPCHK(hipMemsetAsync(d_pred.as<void>(), 0, d_pred.bytes(), stream));
+  HIPCHK(hipMemsetAsync(d_pred_var.as<void>(), 0, d_pred_var.bytes(), stream));
+  eval_nblk = eval_blocks((int)ntest);
+  d_eval_partial.alloc((size_t)2 * eval_nblk * sizeof(double));
+  noise_nblk = noise_sse_blocks(dims[0]);
+  d_noise_partial.alloc((size_t)noise_nblk * sizeof(double));
+  d_norm_partial.alloc(256 * sizeof(double));
+  for (int m = 0; m < nmodes; m++) priors[m]->init(*this, m);
+  noise->init(*this);
+  build_plans();
+  HIPCHK(hipStreamSynchronize(stream));
+  host_iter = 0;
+  initialized = true;
+}
+
+void Model::set_iteration(int it) {
+  upload_scalar<int>(d_iter, it, stream);
+  host_iter = it;
+}
+
+BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<DevBuf>& B, hipStream_t s) {
+  const int N = (int)row_ptr.size() - 1;
+  // rows sorted by nnz (descending); rows above the segment size are split
+  const int seg = 1024;
+  std::vector<int> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (row_ptr[a + 1] - row_ptr[a]) > (row_ptr[b + 1] - row_ptr[b]);
+  });
+  std::vector<int> light, heavy, hseg0, sr, sb, se;
+  for (int q = 0; q < N; q++) {
+    const int r = order[q];
+    const int nz = row_ptr[r + 1] - row_ptr[r];
+    if (nz > seg) {
+      hseg0.push_back((int)sb.size());
+      for (int b = row_ptr[r]; b < row_ptr[r + 1]; b += seg) {
+        sr.push_back((int)heavy.size());
+        sb.push_back(b);
+        se.push_back(std::min(b + seg, row_ptr[r + 1]));
+      }
+      heavy.push_back(r);
+    } else {
+      light.push_back(r);
+    }
+  }
+  hseg0.push_back((int)sb.size());
+  B.clear();
+  B.resize(7);
+  upload(B[0], light, s);
+  upload(B[1], sr, s);
+  upload(B[2], sb, s);
+  upload(B[3], se, s);
+  upload(B[4], heavy, s);
+  upload(B[5], hseg0, s);
+  B[6].alloc(std::max<size_t>(sb.size(), 1) * (size_t)(KP * KP + KP) * sizeof(float));
+  BatchPlan p;
+  p.n_light = (int)light.size();
+  p.light = B[0].as<int>();
+  p.n_seg = (int)sb.size();
+  p.seg_row = B[1].as<int>();
+  p.seg_beg = B[2].as<int>();
+  p.seg_end = B[3].as<int>();
+  p.n_heavy = (int)heavy.size();
+  p.heavy = B[4].as<int>();
+  p.heavy_seg0 = B[5].as<int>();
+  p.partial = B[6].as<float>();
+  HIPCHK(hipStreamSynchronize(s));
+  return p;
+}
+
+void Model::build_plans() {
+  plans.assign(nmodes, BatchPlan());
+  plan_bufs.clear();
+  plan_bufs.resize(nmodes);
+  if (sampler != SAMPLER_BATCHED) return;
+  for (int m = 0; m < nmodes; m++) {
+    if (!batched_supported(KP, nmodes - 1))
+      throw std::runtime_error("batched sampler: unsupported latent size / mode count");
+    plans[m] = build_batch_plan(rows[m].row_ptr, KP, plan_bufs[m], stream);
+  }
+}
+
+SampleArgs Model::sample_args(int mode, const ILatentPrior& p) {
+  SampleArgs a;
+  a.rows = rdev[mode];
+  a.others = others_of(mode);
+  a.alpha = d_alpha.as<double>();
+  a.Lambda_f = p.Lambda_f.as<float>();
+  a.Lmu_f = p.Lmu_f.as<float>();
+  a.Lmu_rows = p.prior_mean_rows();
+  a.U = d_U[mode].as<float>();
+  a.K = num_latent;
+  a.KP = KP;
+  a.seed = seed;
+  a.iter = d_iter.as<int>();
+  a.mode = mode;
+  a.fail = d_fail.as<int>();
+  return a;
+}
+
+void Model::step_launch() {
+  for (int m = 0; m < nmodes; m++) priors[m]->sample_latents(*this, m);
+  for (int m = 0; m < nmodes; m++) priors[m]->update_prior(*this, m);
+  noise->update(*this);
+  noise->evalModel(*this);
+  launch_advance_iter(d_iter.as<int>(), stream);
+}
+
+void Model::iterate(int n) {
+  if (!initialized) throw std::runtime_error("model not initialized");
+  bool graphable = use_graph;
+  for (auto& p : priors)
+    if (dynamic_cast<MacauPrior*>(p.get())) graphable = false;
+  for (int i = 0; i < n; i++) {
+    if (graphable) {
+      if (!graph_exec) {
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        step_launch();
+        HIPCHK(hipStreamEndCapture(stream, &graph));
+        HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(graph_exec, stream));
+    } else {
+      step_launch();
+    }
+    host_iter++;
+  }
+}
+
+void Model::check_fail() {
+  const int f = download_scalar<int>(d_fail, stream);
+  if (f) {
+    upload_scalar<int>(d_fail, 0, stream);
+    throw std::runtime_error("Cholesky Decomposition failed!");
+  }
+}
+
+double Model::host_eval_out(int which) {
+  double v[2];
+  HIPCHK(hipMemcpyAsync(v, d_eval_out.as<void>(), sizeof(v), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return v[which];
+}
+
+double Model::factor_norm(int mode) {
+  launch_sumsq_f32(d_U[mode].as<float>(), (size_t)KP * dims[mode], d_norm_partial.as<double>(), 256, stream);
+  std::vector<double> h(256);
+  HIPCHK(hipMemcpyAsync(h.data(), d_norm_partial.as<void>(), 256 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  double s = 0.0;
+  for (double x : h) s += x;
+  return std::sqrt(s);
+}
+
+double Model::noise_precision() { return download_scalar<double>(d_alpha, stream); }
+
+void Model::download_latents(int mode, std::vector<double>& out) {
+  std::vector<float> h((size_t)KP * dims[mode]);
+  HIPCHK(hipMemcpyAsync(h.data(), d_U[mode].as<void>(), h.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  out.resize((size_t)num_latent * dims[mode]);
+  for (int n = 0; n < dims[mode]; n++)
+    for (int k = 0; k < num_latent; k++) out[(size_t)n * num_latent + k] = h[(size_t)n * KP + k];
+}
+
+static volatile sig_atomic_t keep_running = 1;
+static void int_handler(int) {
+  keep_running = 0;
+  printf("[Received Ctrl-C. Stopping after finishing the current iteration.]\n");
+}
+
+void Model::run() {
+  init();
+  if (verbose) {
+    printf("%s\n", noise->getInitStatus().c_str());
+    printf("Sampling\n");
+  }
+  if (save_model) saveGlobalParams();
+  keep_running = 1;
+  signal(SIGINT, int_handler);
+  const double start = tick();
+  for (int i = 0; i < burnin + nsamples; i++) {
+    if (!keep_running) {
+      keep_running = 1;
+      break;
+    }
+    if (verbose && i == burnin) printf(" ====== Burn-in complete, averaging samples ====== \n");
+    const double starti = tick();
+    iterate(1);
+    const bool need_host = verbose || (save_model && i >= burnin);
+    if (need_host) {
+      sync();
+      check_fail();
+      noise->rmse_test_onesample = host_eval_out(0);
+      noise->rmse_test = host_eval_out(1);
+      noise->sync_host(*this);
+    }
+    if (save_model && i >= burnin) saveModel(i - burnin + 1);
+    if (verbose) printStatus(i, tick() - starti);
+  }
+  (void)start;
+  sync();
+  check_fail();
+  noise->rmse_test_onesample = host_eval_out(0);
+  noise->rmse_test = host_eval_out(1);
+  rmse_test = noise->getEvalMetric();
+}
+
+void Model::printStatus(int i, double elapsedi) {  // macau.cpp:237-259
+  printf("Iter %3d: %s  ", i, noise->getEvalString().c_str());
+  printf("U:[");
+  for (int p = 0; p < nmodes; p++) {
+    printf("%1.2e", factor_norm(p));
+    if (p + 1 < nmodes) printf(", ");
+  }
+  printf("]");
+  printf("\tSide:[");
+  for (int p = 0; p < nmodes; p++) {
+    printf("%1.2e", priors[p]->getLinkNorm(*this));
+    if (p + 1 < nmodes) printf(", ");
+  }
+  printf("]");
+  printf("%s [took %0.1fs]\n", noise->getStatus().c_str(), elapsedi);
+  fflush(stdout);
+}
+
+std::vector<double> Model::getPredictions() {
+  std::vector<double> p(ntest);
+  if (ntest) {
+    HIPCHK(hipMemcpyAsync(p.data(), d_pred.as<void>(), ntest * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  return p;
+}
+
+std::vector<double> Model::getStds() {  // macau.cpp:261-275
+  std::vector<double> s(ntest);
+  if (nsamples <= 1) {
+    std::fill(s.begin(), s.end(), NAN);
+    return s;
+  }
+  if (ntest) {
+    HIPCHK(hipMemcpyAsync(s.data(), d_pred_var.as<void>(), ntest * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  const double inorm = 1.0 / (nsamples - 1);
+  for (auto& x : s) x = std::sqrt(x * inorm);
+  return s;
+}
+
+std::vector<double> Model::getTestData() {  // sparsetensor.cpp:63-76, 178-191
+  std::vector<double> out((size_t)ntest * (nmodes + 1));
+  for (long long t = 0; t < ntest; t++) {
+    for (int m = 0; m < nmodes; m++) out[(size_t)m * ntest + t] = test_idx[(size_t)t * nmodes + m];
+    out[(size_t)nmodes * ntest + t] = test_vals[t];
+  }
+  return out;
+}
+
+void Model::saveModel(int isample) {  // macau.cpp:277-285
+  const std::string fprefix = save_prefix + "-sample" + std::to_string(isample) + "-";
+  for (int i = 0; i < nmodes; i++) {
+    std::vector<double> U;
+    download_latents(i, U);
+    write_csv(fprefix + "U" + std::to_string(i + 1) + "-latents.csv", U, num_latent, dims[i]);
+    priors[i]->saveModel(*this, i, fprefix + "U" + std::to_string(i + 1));
+  }
+}
+
+void Model::saveGlobalParams() {  // macau.cpp:287-292
+  write_csv(save_prefix + "-meanvalue.csv", std::vector<double>{mean_value}, 1, 1);
+}
+
+// ------------------------------------------------------------------
+// priors
+// ------------------------------------------------------------------
+void ILatentPrior::init(Model& m, int mode) {
+  const int K = m.num_latent, KP = m.KP;
+  // BPMFPrior::init (latentprior.cpp:84-99): mu = 0, Lambda = 10 I, WI = I, mu0 = 0
+  std::vector<double> muh(K, 0.0), Lh((size_t)K * K, 0.0), WIh((size_t)K * K, 0.0);
+  std::vector<float> Lf((size_t)KP * KP, 0.f), Lmuf(KP, 0.f);
+  for (int i = 0; i < K; i++) {
+    Lh[(size_t)i * K + i] = 10.0;
+    WIh[(size_t)i * K + i] = 1.0;
+  }
+  for (int i = 0; i < KP; i++) Lf[(size_t)i * KP + i] = i < K ? 10.f : 1.f;
+  upload(mu, muh, m.stream);
+  upload(mu0, muh, m.stream);
+  upload(Lambda, Lh, m.stream);
+  upload(WI, WIh, m.stream);
+  upload(Lambda_f, Lf, m.stream);
+  upload(Lmu_f, Lmuf, m.stream);
+  stats_nblk = hyper_stats_blocks(m.dims[mode]);
+  stats.alloc((size_t)stats_nblk * (K + K * K) * sizeof(double));
+  if (K > 32) nw_ws.alloc((size_t)(3 * K * K + 3 * K) * sizeof(double));
+  HIPCHK(hipStreamSynchronize(m.stream));
+}
+
+void ILatentPrior::sample_latents(Model& m, int mode) {
+  const SampleArgs a = m.sample_args(mode, *this);
+  if (m.sampler == SAMPLER_BATCHED) launch_sample_batched(a, m.plans[mode], m.stream);
+  else launch_sample_plain(a, m.stream);
+}
+
+void ILatentPrior::saveModel(Model& m, int, const std::string& prefix) {  // latentprior.cpp:307-309
+  std::vector<double> h(m.num_latent);
+  HIPCHK(hipMemcpyAsync(h.data(), mu.as<void>(), h.size() * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+  HIPCHK(hipStreamSynchronize(m.stream));
+  write_csv(prefix + "-latentmean.csv", h, m.num_latent, 1);
+}
+
+static NWArgs nw_args(Model& m, ILatentPrior& p, int mode, double nu) {
+  NWArgs a;
+  a.K = m.num_latent;
+  a.KP = m.KP;
+  a.partial = p.stats.as<double>();
+  a.nblk = p.stats_nblk;
+  a.N = m.dims[mode];
+  a.kappa = 2.0;  // b0 (latentprior.cpp:97)
+  a.mu0 = p.mu0.as<double>();
+  a.WI = p.WI.as<double>();
+  a.nu = nu;
+  a.seed = m.seed;
+  a.iter = m.d_iter.as<int>();
+  a.mode = mode;
+  a.ws = p.nw_ws.as<double>();
+  a.mu = p.mu.as<double>();
+  a.Lambda = p.Lambda.as<double>();
+  a.Lambda_f = p.Lambda_f.as<float>();
+  a.Lmu_f = p.Lmu_f.as<float>();
+  a.fail = m.d_fail.as<int>();
+  return a;
+}
+
+void BPMFPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:79-81
+  launch_hyper_stats_f32(m.d_U[mode].as<float>(), nullptr, m.KP, m.dims[mode], m.num_latent, stats.as<double>(),
+                         stats_nblk, m.stream);
+  launch_nw_draw(nw_args(m, *this, mode, (double)m.num_latent), m.stream);  // df = K
+}
+
+// ---- MacauPrior ----
+static void build_csr(int nrow, const std::vector<int>& r, const std::vector<int>& c, const std::vector<double>& v,
+                      std::vector<int>& rp, std::vector<int>& cols, std::vector<double>& vals) {
+  // new_csr (libfastsparse): grouped by row, input order kept within a row
+  rp.assign(nrow + 1, 0);
+  for (int x : r) rp[x + 1]++;
+  for (int i = 0; i < nrow; i++) rp[i + 1] += rp[i];
+  std::vector<int> pos(rp.begin(), rp.end() - 1);
+  cols.resize(r.size());
+  vals.resize(r.size());
+  for (size_t e = 0; e < r.size(); e++) {
+    const int d = pos[r[e]]++;
+    cols[d] = c[e];
+    vals[d] = v.empty() ? 1.0 : v[e];
+  }
+}
+
+MacauPrior::MacauPrior(int nr, int nc, const std::vector<int>& rows, const std::vector<int>& cols,
+                       const std::vector<double>& vals, bool bin, double lb, double t)
+    : nrow(nr), nfeat(nc), lambda_beta0(lb), tol(t), binary(bin) {
+  for (size_t e = 0; e < rows.size(); e++)
+    if (rows[e] < 0 || rows[e] >= nr || cols[e] < 0 || cols[e] >= nc)
+      throw std::runtime_error("side information index out of range");
+  build_csr(nr, rows, cols, vals, F_rowptr, F_cols, F_vals);
+  build_csr(nc, cols, rows, vals, Ft_rowptr, Ft_cols, Ft_vals);
+}
+
+CsrDev MacauPrior::F() const {
+  CsrDev c;
+  c.nrow = nrow;
+  c.ncol = nfeat;
+  c.row_ptr = dF_rowptr.as<int>();
+  c.cols = dF_cols.as<int>();
+  c.vals = binary ? nullptr : dF_vals.as<double>();
+  return c;
+}
+CsrDev MacauPrior::Ft() const {
+  CsrDev c;
+  c.nrow = nfeat;
+  c.ncol = nrow;
+  c.row_ptr = dFt_rowptr.as<int>();
+  c.cols = dFt_cols.as<int>();
+  c.vals = binary ? nullptr : dFt_vals.as<double>();
+  return c;
+}
+
+void MacauPrior::init(Model& m, int mode) {
+  ILatentPrior::init(m, mode);
+  if (nrow != m.dims[mode])
+    throw std::runtime_error("side information rows must equal the size of its data mode");
+  const int K = m.num_latent, KP = m.KP, N = nrow;
+  upload(dF_rowptr, F_rowptr, m.stream);
+  upload(dF_cols, F_cols, m.stream);
+  upload(dF_vals, F_vals, m.stream);
+  upload(dFt_rowptr, Ft_rowptr, m.stream);
+  upload(dFt_cols, Ft_cols, m.stream);
+  upload(dFt_vals, Ft_vals, m.stream);
+  const size_t kf = (size_t)K * nfeat;
+  beta.alloc(kf * sizeof(double));
+  HIPCHK(hipMemsetAsync(beta.as<void>(), 0, beta.bytes(), m.stream));
+  Uhat.alloc((size_t)KP * N * sizeof(float));
+  HIPCHK(hipMemsetAsync(Uhat.as<void>(), 0, Uhat.bytes(), m.stream));
+  Lmu_rows.alloc((size_t)KP * N * sizeof(float));
+  HIPCHK(hipMemsetAsync(Lmu_rows.as<void>(), 0, Lmu_rows.bytes(), m.stream));  // Lambda (mu + 0), mu = 0
+  upload_scalar<double>(lambda_beta_d, lambda_beta0, m.stream);
+  Lchol.alloc((size_t)K * K * sizeof(double));
+  tmp.alloc((size_t)K * N * sizeof(double));
+  Fty.alloc(kf * sizeof(double));
+  bbt_nblk = hyper_stats_blocks(nfeat);
+  bbt.alloc((size_t)bbt_nblk * (K + K * K) * sizeof(double));
+  partial_sumsq.alloc(256 * sizeof(double));
+  // block CG buffers sized for the largest block (<= 40 right-hand sides)
+  cg.alloc(std::min(K, 40), nfeat, N);
+  HIPCHK(hipStreamSynchronize(m.stream));
+}
+
+void CgBuffers::alloc(int nr, int nfeat, int nsamples) {
+  nrmax = nr;
+  X.alloc((size_t)nr * nfeat * sizeof(double));
+  R.alloc((size_t)nr * nfeat * sizeof(double));
+  P.alloc((size_t)nr * nfeat * sizeof(double));
+  KP.alloc((size_t)nr * nfeat * sizeof(double));
+  B.alloc((size_t)nr * nfeat * sizeof(double));
+  inner.alloc((size_t)nr * std::max(nsamples, 1) * sizeof(double));
+  partial.alloc((size_t)dots_blocks(nfeat) * nr * nr * sizeof(double));
+  small.alloc((size_t)(5 * nr * nr + nr + 4) * sizeof(double));
+}
+
+int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
+                   double tol, double* X, CgBuffers& b, hipStream_t s) {
+  if (nr > b.nrmax || nr > 40) throw std::runtime_error("BlockCG only available for up to 40 RHSs.");
+  const int nfeat = F.ncol;
+  CgState c;
+  c.nr = nr;
+  c.n = nfeat;
+  c.X = (nr == ld) ? X : b.X.as<double>();
+  c.R = b.R.as<double>();
+  c.P = b.P.as<double>();
+  c.KPm = b.KP.as<double>();
+  c.inner = b.inner.as<double>();
+  c.partial = b.partial.as<double>();
+  c.nblk = dots_blocks(nfeat);
+  c.small = b.small.as<double>();
+  const double* Bb = B;
+  if (nr != ld) {
+    launch_rows_copy(B, ld, b.B.as<double>(), nr, r0, 0, nr, nfeat, s);
+    Bb = b.B.as<double>();
+  }
+  launch_cg_init(c, Bb, s);
+  const double tolsq = tol * tol;
+  double flags[3] = {0, 0, 0};
+  const double* dflags = c.small + 5 * nr * nr + nr;
+  // 8 iterations per host round trip; kernels of iterations after
+  // convergence return immediately (device-side done flag)
+  for (int it = 0; it < 100000; it += 8) {
+    for (int q = 0; q < 8; q++) {
+      launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, s);
+      launch_spmm(Ft, c.inner, nr, c.KPm, reg, c.P, s);
+      launch_cg_step_a(c, s);
+      launch_cg_update_xr(c, s);
+      launch_cg_step_b(c, tolsq, s);
+      launch_cg_update_p(c, s);
+    }
+    HIPCHK(hipMemcpyAsync(flags, dflags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flags[0] != 0.0) break;
+  }
+  if (flags[2] != 0.0) throw std::runtime_error("block CG: Cholesky of a small system failed");
+  launch_cg_finish(c, s);
+  if (nr != ld) launch_rows_copy(c.X, nr, X, ld, 0, r0, nr, nfeat, s);
+  return (int)flags[1];
+}
+
+void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
+  const int K = m.num_latent, KP = m.KP, N = nrow;
+  hipStream_t s = m.stream;
+  // residual statistics of U - Uhat, and beta beta^T of the current beta
+  launch_hyper_stats_f32(m.d_U[mode].as<float>(), Uhat.as<float>(), KP, N, K, stats.as<double>(), stats_nblk, s);
+  launch_hyper_stats_f64(beta.as<double>(), K, nfeat, K, bbt.as<double>(), bbt_nblk, s);
+  NWArgs a = nw_args(m, *this, mode, (double)K + (double)nfeat);  // df + beta.cols()
+  a.BBt_partial = bbt.as<double>();
+  a.bbt_nblk = bbt_nblk;
+  a.lambda_beta = lambda_beta_d.as<double>();
+  a.Lchol = Lchol.as<double>();
+  launch_nw_draw(a, s);
+  // sample_beta (latentprior.cpp:272-293)
+  BetaArgs b;
+  b.K = K;
+  b.KP = KP;
+  b.N = N;
+  b.nfeat = nfeat;
+  b.U = m.d_U[mode].as<float>();
+  b.mu = mu.as<double>();
+  b.Lchol = Lchol.as<double>();
+  b.lambda_beta = lambda_beta_d.as<double>();
+  b.seed = m.seed;
+  b.iter = m.d_iter.as<int>();
+  b.mode = mode;
+  launch_beta_tmp(b, tmp.as<double>(), s);
+  launch_spmm(Ft(), tmp.as<double>(), K, Fty.as<double>(), nullptr, nullptr, s);
+  launch_beta_noise_add(b, Fty.as<double>(), s);
+  // solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (linop.h:206-226)
+  const int blocksize = 32, excess = 8;
+  if (K <= excess + blocksize) {
+    last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, 0, K, tol,
+                                   beta.as<double>(), cg, s);
+  } else {
+    for (int i = 0; i < K; i += blocksize) {
+      int nr = blocksize;
+      if (i + blocksize + excess >= K) nr = K - i;
+      last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, i, nr, tol,
+                                     beta.as<double>(), cg, s);
+    }
+  }
+  // compute_uhat and the per-row prior term for the next half-sweep
+  launch_uhat_finish(F(), beta.as<double>(), K, KP, mu.as<double>(), Lambda.as<double>(), Uhat.as<float>(),
+                     Lmu_rows.as<float>(), s);
+  // lambda_beta = sample_lambda_beta(beta, Lambda, nu0, mu0)
+  launch_hyper_stats_f64(beta.as<double>(), K, nfeat, K, bbt.as<double>(), bbt_nblk, s);
+  launch_lambda_beta(K, (double)nfeat, bbt.as<double>(), bbt_nblk, Lambda.as<double>(), lambda_beta_d.as<double>(),
+                     m.seed, m.d_iter.as<int>(), mode, s);
+}
+
+double MacauPrior::getLinkNorm(Model& m) {
+  launch_sumsq_f64(beta.as<double>(), (size_t)m.num_latent * nfeat, partial_sumsq.as<double>(), 256, m.stream);
+  std::vector<double> h(256);
+  HIPCHK(hipMemcpyAsync(h.data(), partial_sumsq.as<void>(), 256 * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+  HIPCHK(hipStreamSynchronize(m.stream));
+  double s = 0.0;
+  for (double x : h) s += x;
+  return std::sqrt(s);
+}
+
+double MacauPrior::getLinkLambda(Model& m) { return download_scalar<double>(lambda_beta_d, m.stream); }
+
+void MacauPrior::saveModel(Model& m, int mode, const std::string& prefix) {  // latentprior.cpp:311-315
+  ILatentPrior::saveModel(m, mode, prefix);
+  std::vector<double> h((size_t)m.num_latent * nfeat);
+  HIPCHK(hipMemcpyAsync(h.data(), beta.as<void>(), h.size() * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+  HIPCHK(hipStreamSynchronize(m.stream));
+  write_csv(prefix + "-link.csv", h, m.num_latent, nfeat);
+}
+
+// ------------------------------------------------------------------
+// noise models
+// ------------------------------------------------------------------
+std::string INoiseModel::getEvalString() {  // noisemodels.h:46
+  return std::string("RMSE: ") + to_string_with_precision(rmse_test, 5) + " (1samp: " +
+         to_string_with_precision(rmse_test_onesample, 5) + ")";
+}
+
+void INoiseModel::evalModel(Model& m) {
+  EvalArgs a;
+  a.ntest = (int)m.ntest;
+  a.nmodes = m.nmodes;
+  a.K = m.num_latent;
+  a.KP = m.KP;
+  a.idx = m.d_test_idx.as<int>();
+  a.vals = m.d_test_vals.as<double>();
+  for (int q = 0; q < m.nmodes; q++) a.f[q] = m.d_U[q].as<float>();
+  a.mean = m.mean_value;
+  a.pred = m.d_pred.as<double>();
+  a.pred_var = m.d_pred_var.as<double>();
+  a.partial = m.d_eval_partial.as<double>();
+  a.nblk = m.eval_nblk;
+  a.iter = m.d_iter.as<int>();
+  a.burnin = m.burnin;
+  a.out = m.d_eval_out.as<double>();
+  launch_eval(a, m.stream);
+}
+
+void FixedGaussianNoise::init(Model& m) { upload_scalar<double>(m.d_alpha, alpha, m.stream); }
+
+std::string FixedGaussianNoise::getInitStatus() {
+  return std::string("Noise precision: ") + std::to_string(alpha) + " (fixed)";
+}
+
+void AdaptiveGaussianNoise::init(Model& m) {  // noisemodels.cpp:14-53
+  double se = 0.0;
+  for (double v : m.rows[0].vals) se += (v - m.mean_value) * (v - m.mean_value);
+  var_total = se / (double)m.nnz;
+  if (var_total <= 0.0 || std::isnan(var_total)) var_total = 1.0;
+  alpha = (sn_init + 1.0) / var_total;
+  alpha_max = (sn_max + 1.0) / var_total;
+  upload_scalar<double>(m.d_alpha, alpha, m.stream);
+}
+
+void AdaptiveGaussianNoise::update(Model& m) {  // noisemodels.cpp:55-124
+  NoiseArgs a;
+  a.rows0 = m.rdev[0];
+  a.U0 = m.d_U[0].as<float>();
+  a.others = m.others_of(0);
+  a.K = m.num_latent;
+  a.KP = m.KP;
+  a.nmodes = m.nmodes;
+  a.partial = m.d_noise_partial.as<double>();
+  a.nblk = m.noise_nblk;
+  launch_noise_sse(a, m.stream);
+  NoiseDrawArgs d;
+  d.partial = m.d_noise_partial.as<double>();
+  d.nblk = m.noise_nblk;
+  d.nnz = (double)m.nnz;
+  d.var_total = var_total;
+  d.b0 = m.is_tensor ? 0.5 : 0.5 * var_total;
+  d.alpha_max = alpha_max;
+  d.seed = m.seed;
+  d.iter = m.d_iter.as<int>();
+  d.alpha = m.d_alpha.as<double>();
+  d.sse_out = m.d_sse.as<double>();
+  launch_noise_draw(d, m.stream);
+}
+
+std::string AdaptiveGaussianNoise::getInitStatus() {
+  return std::string("Noise precision: adaptive (with max precision of ") + std::to_string(alpha_max) + ")";
+}
+
+std::string AdaptiveGaussianNoise::getStatus() { return std::string("Prec:") + to_string_with_precision(alpha, 2); }
+
+void AdaptiveGaussianNoise::sync_host(Model& m) { alpha = m.noise_precision(); }
+
+}  // namespace mgpu

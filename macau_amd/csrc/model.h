@@ -1,0 +1,243 @@
+// model.h -- host C++ mirror of the reference's Macau / ILatentPrior /
+// INoiseModel interfaces (macau.h:11-81, latentprior.h:17-125,
+// noisemodels.h:13-86), backed by device-resident state on one MI355X.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace mgpu {
+
+void hip_check(hipError_t e, const char* what);
+#define HIPCHK(x) ::mgpu::hip_check((x), #x)
+
+// Owning device allocation.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void alloc(size_t bytes);
+  void release();
+  template <typename T> T* as() const { return static_cast<T*>(p_); }
+  size_t bytes() const { return n_; }
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+enum SamplerKind { SAMPLER_PLAIN = 0, SAMPLER_BATCHED = 1 };
+
+// Sparse rows of one mode on the host (SparseMode, sparsetensor.h:76-91).
+struct HostRows {
+  std::vector<int> row_ptr;  // N + 1
+  std::vector<int> idx;      // nnz * nother
+  std::vector<double> vals;  // raw values
+};
+
+class Model;
+
+// Buffers of one block-CG solve (solve_blockcg, linop.h:228-343).
+struct CgBuffers {
+  DevBuf X, R, P, KP, B, inner, partial, small;
+  int nrmax = 0;
+  void alloc(int nrmax, int nfeat, int nsamples);
+};
+// Solves X[r0:r0+nr, :] (F^T F + reg I) = B[r0:r0+nr, :] with B and X of
+// leading dimension ld (= total right-hand sides); reg is a device scalar.
+// Returns the iteration count of the reference's loop.
+int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
+                   double tol, double* X, CgBuffers& b, hipStream_t s);
+
+// Work list of the batched sampler for one mode (rows sorted by nnz,
+// heavy rows split into segments).
+BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<DevBuf>& bufs, hipStream_t s);
+
+// latentprior.h:17-49
+class ILatentPrior {
+ public:
+  virtual ~ILatentPrior() = default;
+  virtual void init(Model& m, int mode);
+  virtual void sample_latents(Model& m, int mode);
+  virtual void update_prior(Model& m, int mode) = 0;
+  virtual double getLinkNorm(Model&) { return NAN; }
+  virtual double getLinkLambda(Model&) { return NAN; }
+  virtual void saveModel(Model& m, int mode, const std::string& prefix);
+  virtual const float* prior_mean_rows() const { return nullptr; }
+
+  // device parameters shared with the sampler (FP64 state + FP32 copies)
+  DevBuf mu, Lambda, Lambda_f, Lmu_f, mu0, WI, stats, nw_ws;
+  int stats_nblk = 0;
+};
+
+// BPMFPrior (latentprior.h:53-79, latentprior.cpp:69-99)
+class BPMFPrior : public ILatentPrior {
+ public:
+  void update_prior(Model& m, int mode) override;
+};
+
+// MacauPrior<SparseDoubleFeat> (latentprior.h:82-125, latentprior.cpp:180-331)
+class MacauPrior : public ILatentPrior {
+ public:
+  MacauPrior(int nrow, int ncol, const std::vector<int>& rows, const std::vector<int>& cols,
+             const std::vector<double>& vals, bool binary, double lambda_beta, double tol);
+  void init(Model& m, int mode) override;
+  void update_prior(Model& m, int mode) override;
+  double getLinkNorm(Model& m) override;
+  double getLinkLambda(Model& m) override;
+  void saveModel(Model& m, int mode, const std::string& prefix) override;
+  const float* prior_mean_rows() const override { return Lmu_rows.as<float>(); }
+
+  int nrow, nfeat;
+  double lambda_beta0, tol;
+  bool binary;
+  std::vector<int> F_rowptr, F_cols, Ft_rowptr, Ft_cols;
+  std::vector<double> F_vals, Ft_vals;
+  DevBuf dF_rowptr, dF_cols, dF_vals, dFt_rowptr, dFt_cols, dFt_vals;
+  DevBuf beta, Uhat, Lmu_rows, lambda_beta_d, Lchol, tmp, Fty, bbt, partial_sumsq;
+  CgBuffers cg;
+  int bbt_nblk = 0;
+  int last_cg_iters = 0;
+  CsrDev F() const;
+  CsrDev Ft() const;
+};
+
+// noisemodels.h:13-29
+class INoiseModel {
+ public:
+  virtual ~INoiseModel() = default;
+  virtual void init(Model& m) = 0;
+  virtual void update(Model&) {}
+  virtual void evalModel(Model& m);
+  virtual double getEvalMetric() { return rmse_test; }
+  virtual std::string getEvalString();
+  virtual std::string getInitStatus() = 0;
+  virtual std::string getStatus() = 0;
+  virtual void sync_host(Model&) {}
+  double rmse_test = NAN, rmse_test_onesample = NAN;
+};
+
+// FixedGaussianNoise (noisemodels.h:32-50)
+class FixedGaussianNoise : public INoiseModel {
+ public:
+  explicit FixedGaussianNoise(double a) : alpha(a) {}
+  void init(Model& m) override;
+  std::string getInitStatus() override;
+  std::string getStatus() override { return ""; }
+  double alpha;
+};
+
+// AdaptiveGaussianNoise (noisemodels.h:53-86, noisemodels.cpp:14-124)
+class AdaptiveGaussianNoise : public INoiseModel {
+ public:
+  AdaptiveGaussianNoise(double sinit, double smax) : sn_init(sinit), sn_max(smax) {}
+  void init(Model& m) override;
+  void update(Model& m) override;
+  std::string getInitStatus() override;
+  std::string getStatus() override;
+  void sync_host(Model& m) override;
+  double sn_init, sn_max;
+  double alpha = NAN, alpha_max = NAN, var_total = NAN;
+};
+
+// MacauX<DType, NType> (macau.h:56-81, macau.cpp)
+class Model {
+ public:
+  Model(int nmodes, int num_latent, std::unique_ptr<INoiseModel> noise);
+  ~Model();
+
+  // macau.h:35-53
+  void addPrior(std::unique_ptr<ILatentPrior> p) { priors.push_back(std::move(p)); }
+  void setSamples(int b, int n) { burnin = b; nsamples = n; }
+  void setRelationData(const int* idx, int nmodes, const double* vals, int nnz, const int* dims);
+  void setRelationDataTest(const int* idx, int nmodes, const double* vals, int nnz, const int* dims);
+  void init();
+  void run();
+  void printStatus(int i, double elapsedi);
+  double getRmseTest() const { return rmse_test; }
+  std::vector<double> getPredictions();
+  std::vector<double> getStds();
+  std::vector<double> getTestData();  // ntest x (nmodes + 1), column-major
+  void saveModel(int isample);
+  void saveGlobalParams();
+
+  // extensions
+  void iterate(int n);                 // n Gibbs iterations after init()
+  void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+  void check_fail();
+  void download_latents(int mode, std::vector<double>& out);  // K x N col-major
+  double noise_precision();
+  void set_iteration(int it);
+  void capture_graph(bool on) { use_graph = on; }
+
+  // -- configuration --
+  int nmodes, num_latent, KP;
+  int burnin = 50, nsamples = 100;
+  bool verbose = true, save_model = false;
+  std::string save_prefix = "model";
+  uint64_t seed = 0;
+  int sampler = SAMPLER_PLAIN;
+  bool use_graph = true;
+  bool is_tensor;
+  std::vector<std::unique_ptr<ILatentPrior>> priors;
+  std::unique_ptr<INoiseModel> noise;
+  double rmse_test = 0.0;
+
+  // -- host data (sparsetensor.cpp) --
+  std::vector<int> dims;
+  std::vector<HostRows> rows;  // per mode
+  double mean_value = 0.0;
+  long long nnz = 0;
+  bool has_test = false;
+  std::vector<int> test_idx;  // ntest x nmodes
+  std::vector<double> test_vals;
+  long long ntest = 0;
+
+  // -- device state --
+  hipStream_t stream = nullptr;
+  std::vector<DevBuf> d_rowptr, d_idx, d_r, d_U;
+  std::vector<RowsDev> rdev;
+  std::vector<BatchPlan> plans;
+  std::vector<std::vector<DevBuf>> plan_bufs;
+  DevBuf d_iter, d_alpha, d_fail, d_eval_out, d_test_idx, d_test_vals, d_pred, d_pred_var, d_eval_partial,
+      d_noise_partial, d_norm_partial, d_sse;
+  int eval_nblk = 0, noise_nblk = 0;
+  bool initialized = false;
+  int host_iter = 0;  // iterations launched so far
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  FactorSet others_of(int mode) const;
+  SampleArgs sample_args(int mode, const ILatentPrior& p);
+  void step_launch();  // enqueue one Gibbs iteration
+  double factor_norm(int mode);
+  double host_eval_out(int which);
+
+ private:
+  void build_plans();
+};
+
+// Setup-time copy of a host vector into a new allocation; synchronous, so
+// the host vector may be a temporary.
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+}  // namespace mgpu

@@ -1,0 +1,168 @@
+// sampler_plain.hip -- plain FP32 per-row latent sampler.
+//
+// Computes, for every row n of one mode (sample_latent_blas,
+// latentprior.cpp:361-385; sample_latent_tensor, latentprior.cpp:118-161):
+//   P = Lambda + alpha * sum_e c_e c_e^T,   b = alpha * sum_e c_e r_e + Lambda mu_n
+//   P = L L^T,   u_n = L^{-T} (L^{-1} b + z_n)
+// where c_e is the (Hadamard product of the) other modes' latent columns at
+// nonzero e and r_e = y_e - mean.
+//
+// Layout: one work-group handles G = 64/KP rows (KP <= 64) or one row
+// (KP = 128); a group of KP lanes owns one row and lane i owns row i of P
+// in registers.  The Gram is accumulated in FP32 on the vector ALUs from
+// nonzeros staged through LDS.  The Cholesky is right-looking with the
+// current column broadcast through LDS; the forward solve L y = b rides
+// along as an extra column; the back solve reads L row-major from LDS.
+// Rows are processed in natural order with no load balancing: this is the
+// plain baseline the batched sampler is measured against.
+#include "common.h"
+#include "kernels.h"
+
+namespace mgpu {
+
+template <int NO>
+__device__ __forceinline__ float gather_comp(const FactorSet& F, const int* __restrict__ idx, int e, int i,
+                                             int KP) {
+  float x = F.f[0][(size_t)idx[(size_t)e * NO] * KP + i];
+#pragma unroll
+  for (int t = 1; t < NO; t++) x *= F.f[t][(size_t)idx[(size_t)e * NO + t] * KP + i];
+  return x;
+}
+
+template <int KP, int NO>
+__global__ void __launch_bounds__(KP < 64 ? 64 : KP) k_sample_plain(SampleArgs a) {
+  constexpr int WG = KP < 64 ? 64 : KP;
+  constexpr int G = WG / KP;
+  constexpr int CH = 16;
+  __shared__ float vbuf[G][CH][KP];
+  __shared__ float rbuf[G][CH];
+  __shared__ float colb[2][G][KP + 1];
+  __shared__ float Lrow[G][KP][KP + 1];
+  __shared__ float xs[2][G];
+
+  const int tid = threadIdx.x, g = tid / KP, i = tid % KP;
+  const int row = blockIdx.x * G + g;
+  const bool active = row < a.rows.N;
+  int beg = 0, end = 0;
+  if (active) {
+    beg = a.rows.row_ptr[row];
+    end = a.rows.row_ptr[row + 1];
+  }
+  const int nnz = end - beg;
+  const int it = *a.iter;
+
+  // ---- Gram and right-hand side ----
+  float acc[KP];
+#pragma unroll
+  for (int j = 0; j < KP; j++) acc[j] = 0.f;
+  float bi = 0.f;
+  for (int base = 0; __syncthreads_or(base < nnz); base += CH) {
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      float x = 0.f, rr = 0.f;
+      if (base + c < nnz) {
+        const int e = beg + base + c;
+        x = gather_comp<NO>(a.others, a.rows.idx, e, i, KP);
+        rr = a.rows.r[e];
+      }
+      vbuf[g][c][i] = x;
+      if (i == 0) rbuf[g][c] = rr;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const float vi = vbuf[g][c][i];
+      const float rc = rbuf[g][c];
+#pragma unroll
+      for (int j = 0; j < KP; j++) acc[j] += vi * vbuf[g][c][j];
+      bi += vi * rc;
+    }
+    __syncthreads();
+  }
+
+  // ---- P = Lambda + alpha G,  b = alpha c + Lambda mu ----
+  const float alpha = (float)(*a.alpha);
+#pragma unroll
+  for (int j = 0; j < KP; j++) acc[j] = a.Lambda_f[i * KP + j] + alpha * acc[j];
+  float b = alpha * bi;
+  if (a.Lmu_rows) {
+    if (active) b += a.Lmu_rows[(size_t)row * KP + i];
+  } else {
+    b += a.Lmu_f[i];
+  }
+
+  // ---- Cholesky with fused forward solve ----
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    float* cb = colb[k & 1][g];
+    if (i >= k) cb[i] = acc[k];
+    if (i == k) cb[KP] = b;
+    __syncthreads();
+    float dkk = cb[k];
+    if (!(dkk > 0.f)) {
+      if (i == k && active) atomicMax(a.fail, row + 1);
+      dkk = 1.f;
+    }
+    const float sq = sqrtf(dkk);
+    const float bk = cb[KP];
+    if (i > k) {
+      const float s = acc[k] / dkk;
+#pragma unroll
+      for (int j = k + 1; j < KP; j++) acc[j] -= s * cb[j];
+      b -= s * bk;
+      acc[k] = acc[k] / sq;
+    } else if (i == k) {
+      acc[k] = sq;
+      b = bk / sq;
+    }
+  }
+
+  // ---- add z, back solve L^T x = y ----
+  if (i < a.K) b += normal_f32(a.seed, (uint32_t)row, (uint32_t)i, (uint32_t)it, make_tag(TAG_LATENT, a.mode));
+#pragma unroll
+  for (int j = 0; j < KP; j++) Lrow[g][i][j] = (j <= i) ? acc[j] : 0.f;
+  __syncthreads();
+#pragma unroll
+  for (int j = KP - 1; j >= 0; j--) {
+    const float lji = Lrow[g][j][i];  // L[j][i]
+    if (i == j) xs[j & 1][g] = b / lji;
+    __syncthreads();
+    const float xj = xs[j & 1][g];
+    if (i < j) b -= lji * xj;
+    if (i == j) b = xj;
+  }
+  if (active) a.U[(size_t)row * KP + i] = b;
+}
+
+template <int KP, int NO>
+static void launch_plain_t(const SampleArgs& a, hipStream_t s) {
+  constexpr int WG = KP < 64 ? 64 : KP;
+  constexpr int G = WG / KP;
+  const int grid = (a.rows.N + G - 1) / G;
+  if (grid == 0) return;
+  hipLaunchKernelGGL((k_sample_plain<KP, NO>), dim3(grid), dim3(WG), 0, s, a);
+}
+
+template <int KP>
+static void launch_plain_k(const SampleArgs& a, hipStream_t s) {
+  switch (a.rows.nother) {
+    case 1: return launch_plain_t<KP, 1>(a, s);
+    case 2: return launch_plain_t<KP, 2>(a, s);
+    case 3: return launch_plain_t<KP, 3>(a, s);
+    case 4: return launch_plain_t<KP, 4>(a, s);
+    default: break;
+  }
+}
+
+void launch_sample_plain(const SampleArgs& a, hipStream_t s) {
+  switch (a.KP) {
+    case 8: return launch_plain_k<8>(a, s);
+    case 16: return launch_plain_k<16>(a, s);
+    case 32: return launch_plain_k<32>(a, s);
+    case 64: return launch_plain_k<64>(a, s);
+    case 128: return launch_plain_k<128>(a, s);
+    default: break;
+  }
+}
+
+}  // namespace mgpu

@@ -1,0 +1,390 @@
+"""FP64 CPU oracle for the jaak-s/macau Gibbs sampling step.
+
+TEST INFRASTRUCTURE ONLY.  tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, as the checker
+and the reported CPU baseline.  The product (``macau_amd``) never imports it.
+
+The numeric core is ``oracle.c`` (built into ``liboracle.so`` by
+``oracle/Makefile``).  This module adds the data layer restated in numpy:
+
+* ``matrix_rows`` -- ``MatrixData::setTrain`` (sparsetensor.cpp:198-205)
+  through ``sparseFromIJV`` (bpmfutils.h:57-65): Eigen ``setFromTriplets``
+  sums duplicates and sorts inner indices; mode 0 iterates ``Yt`` columns
+  (rows of Y), mode 1 iterates ``Y`` columns (latentprior.cpp:30-46).
+* ``sparse_mode`` -- ``SparseMode::init`` (sparsetensor.cpp:11-60): stable
+  grouping by one mode, the other modes' indices kept in order.
+* ``matrix_test`` / ``tensor_test`` -- the test-cell order of
+  ``getTestData`` (sparsetensor.cpp:63-76 CSC order; 178-191 mode-0 order).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+TAG_LATENT, TAG_WISHART_Z, TAG_WISHART_CHI, TAG_NW_MU = 1, 2, 3, 4
+TAG_NOISE, TAG_BETA_U, TAG_BETA_F, TAG_LAMBDA_BETA = 5, 6, 7, 8
+
+
+def tag(stream, mode=0):
+    return (stream << 24) | mode
+
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_dbl = ctypes.c_double
+
+
+class OrcRows(ctypes.Structure):
+    _fields_ = [("nrows", _i64), ("nother", ctypes.c_int), ("row_ptr", _p), ("idx", _p), ("vals", _p)]
+
+
+class OrcCsr(ctypes.Structure):
+    _fields_ = [("nrow", _i64), ("ncol", _i64), ("row_ptr", _p), ("cols", _p), ("vals", _p)]
+
+
+class OrcRunCfg(ctypes.Structure):
+    _fields_ = [
+        ("nmodes", ctypes.c_int), ("K", ctypes.c_int), ("is_tensor", ctypes.c_int),
+        ("dims", _i64 * 8), ("train", OrcRows * 8), ("mean", _dbl), ("nnz", _i64),
+        ("ntest", _i64), ("test_idx", _p), ("test_vals", _p),
+        ("noise_adaptive", ctypes.c_int), ("precision", _dbl), ("sn_init", _dbl), ("sn_max", _dbl),
+        ("macau", ctypes.c_int * 8), ("F", OrcCsr * 8), ("Ft", OrcCsr * 8),
+        ("lambda_beta0", _dbl), ("tol", _dbl), ("burnin", ctypes.c_int), ("nsamples", ctypes.c_int),
+        ("seed", _u64), ("max_iters", ctypes.c_int),
+        ("pred", _p), ("pred_var", _p), ("U_out", _p * 8), ("rmse", _dbl), ("rmse1", _dbl),
+        ("alpha", _dbl), ("alpha_trace", _p), ("rmse_trace", _p), ("lambda_beta_out", _p),
+        ("cg_iters_last", ctypes.c_int),
+    ]
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        L.orc_philox4x32_10.argtypes = [_p, _p, _p]
+        L.orc_normal.restype = _dbl
+        L.orc_normal.argtypes = [_u64, _u32, _u32, _u32, _u32]
+        L.orc_normals.argtypes = [_u64, _u32, _u32, _u32, _u32, _p]
+        L.orc_gamma.restype = _dbl
+        L.orc_gamma.argtypes = [_u64, _dbl, _u32, _u32, _u32]
+        L.orc_sample_latents.restype = _i64
+        L.orc_sample_latents.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p, _dbl, _p, _p, _p,
+                                         _u64, _u32, ctypes.c_int, _p]
+        L.orc_cond_normal_wishart.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _p, _dbl, _u64, _u32,
+                                              ctypes.c_int, _p, _p]
+        L.orc_normal_wishart_from_stats.argtypes = [ctypes.c_int, _i64, _p, _p, _p, _dbl, _p, _dbl, _u64,
+                                                    _u32, ctypes.c_int, _p, _p]
+        L.orc_train_sse.restype = _dbl
+        L.orc_train_sse.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p]
+        L.orc_noise_alpha.restype = _dbl
+        L.orc_noise_alpha.argtypes = [_dbl, _i64, _dbl, ctypes.c_int, _dbl, _u64, _u32]
+        L.orc_eval_rmse.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _dbl, _p, ctypes.c_int, _p,
+                                    _p, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
+        L.orc_csr_A_mul_Bn.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _p, _p]
+        L.orc_AtA_mul_B.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p, _p]
+        L.orc_blockcg.restype = ctypes.c_int
+        L.orc_blockcg.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p, _dbl, _p]
+        L.orc_blockcg_blocked.restype = ctypes.c_int
+        L.orc_blockcg_blocked.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p,
+                                          _dbl, ctypes.c_int, ctypes.c_int, _p]
+        L.orc_posterior_lambda_beta.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _dbl, ctypes.POINTER(_dbl),
+                                                ctypes.POINTER(_dbl)]
+        L.orc_run.restype = ctypes.c_int
+        L.orc_run.argtypes = [ctypes.POINTER(OrcRunCfg)]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ---------------------------------------------------------------- RNG
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def normals(seed, c0, n, it, tg):
+    out = np.zeros(n, dtype=np.float64)
+    lib().orc_normals(seed, c0, n, it, tg, _ptr(out))
+    return out
+
+
+def gamma(seed, shape, c0, it, tg):
+    return lib().orc_gamma(seed, shape, c0, it, tg)
+
+
+# ---------------------------------------------------------------- data layer
+class Rows:
+    """Sparse rows of one mode: row_ptr (int64), idx (nnz x nother int32), vals (float64)."""
+
+    def __init__(self, row_ptr, idx, vals):
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        self.idx = np.ascontiguousarray(idx, dtype=np.int32)
+        if self.idx.ndim == 1:
+            self.idx = self.idx.reshape(-1, 1)
+        self.vals = np.ascontiguousarray(vals, dtype=np.float64)
+
+    @property
+    def nrows(self):
+        return len(self.row_ptr) - 1
+
+    def c(self):
+        return OrcRows(self.nrows, self.idx.shape[1], _ptr(self.row_ptr), _ptr(self.idx), _ptr(self.vals))
+
+
+def sum_duplicates(rows, cols, vals, nrows, ncols):
+    """setFromTriplets: duplicates summed; returns (row, col, val) in CSC order."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    vals = np.asarray(vals, dtype=np.float64)
+    key = cols * nrows + rows
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    if len(ks) == 0:
+        return rows[:0], cols[:0], vals[:0]
+    starts = np.concatenate([[0], np.nonzero(np.diff(ks))[0] + 1])
+    v = np.add.reduceat(vals[order], starts)
+    uk = ks[starts]
+    return (uk % nrows).astype(np.int64), (uk // nrows).astype(np.int64), v
+
+
+def _group(keys, nkeys, other, vals):
+    order = np.argsort(keys, kind="stable")
+    counts = np.bincount(keys, minlength=nkeys)
+    row_ptr = np.zeros(nkeys + 1, dtype=np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    return Rows(row_ptr, other[order], vals[order])
+
+
+def matrix_rows(rows, cols, vals, nrows, ncols):
+    """MatrixData::setTrain: returns ([mode0 Rows, mode1 Rows], mean, nnz)."""
+    r, c, v = sum_duplicates(rows, cols, vals, nrows, ncols)
+    mean = v.sum() / len(v) if len(v) else np.nan
+    # mode 0: rows of Y with column indices ascending (CSC of Yt)
+    o0 = np.lexsort((c, r))
+    m0 = _group(r[o0], nrows, c[o0].reshape(-1, 1), v[o0])
+    # mode 1: columns of Y with row indices ascending (CSC of Y)
+    m1 = _group(c, ncols, r.reshape(-1, 1), v)
+    return [m0, m1], mean, len(v)
+
+
+def sparse_mode(idx, vals, mode, mode_size):
+    """SparseMode::init: idx is nnz x nmodes; stable grouping by `mode`."""
+    idx = np.asarray(idx, dtype=np.int64)
+    others = np.delete(idx, mode, axis=1)
+    return _group(idx[:, mode], mode_size, others, np.asarray(vals, dtype=np.float64))
+
+
+def tensor_rows(idx, vals, dims):
+    idx = np.asarray(idx, dtype=np.int64)
+    vals = np.asarray(vals, dtype=np.float64)
+    return [sparse_mode(idx, vals, m, dims[m]) for m in range(len(dims))], vals.mean(), len(vals)
+
+
+def matrix_test(rows, cols, vals, nrows, ncols):
+    """getTestData order for MatrixData: CSC of Ytest, duplicates summed."""
+    r, c, v = sum_duplicates(rows, cols, vals, nrows, ncols)
+    return np.stack([r, c], axis=1).astype(np.int32), v
+
+
+def tensor_test(idx, vals, dims):
+    """getTestData order for TensorData: SparseMode on mode 0."""
+    idx = np.asarray(idx, dtype=np.int64)
+    order = np.argsort(idx[:, 0], kind="stable")
+    return np.ascontiguousarray(idx[order].astype(np.int32)), np.asarray(vals, dtype=np.float64)[order]
+
+
+class Csr:
+    def __init__(self, row_ptr, cols, vals, nrow, ncol):
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        self.cols = np.ascontiguousarray(cols, dtype=np.int32)
+        self.vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float64)
+        self.nrow, self.ncol = nrow, ncol
+
+    def c(self):
+        return OrcCsr(self.nrow, self.ncol, _ptr(self.row_ptr), _ptr(self.cols), _ptr(self.vals))
+
+
+def csr_from_coo(rows, cols, vals, nrow, ncol):
+    """new_csr (libfastsparse): rows grouped, insertion order kept within a row."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    order = np.argsort(rows, kind="stable")
+    row_ptr = np.zeros(nrow + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rows, minlength=nrow), out=row_ptr[1:])
+    v = None if vals is None else np.asarray(vals, dtype=np.float64)[order]
+    return Csr(row_ptr, cols[order], v, nrow, ncol)
+
+
+def csr_pair(rows, cols, vals, nrow, ncol):
+    """SparseDoubleFeat: F as CSR and its transpose as CSR (linop.h:39-42)."""
+    return csr_from_coo(rows, cols, vals, nrow, ncol), csr_from_coo(cols, rows, vals, ncol, nrow)
+
+
+# ---------------------------------------------------------------- numerics
+def sample_latents(K, rows, mean, others, alpha, mu, Lambda, seed, it, mode, mu_rows=None):
+    """Per-row sampler.  Factor matrices are K x N, passed as (N, K) C-order arrays."""
+    N = rows.nrows
+    U = np.zeros((N, K))
+    oth = [np.ascontiguousarray(o, dtype=np.float64) for o in others]
+    arr = (ctypes.c_void_p * len(oth))(*[_ptr(o) for o in oth])
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    Lam = np.asfortranarray(Lambda, dtype=np.float64)
+    mr = None if mu_rows is None else np.ascontiguousarray(mu_rows, dtype=np.float64)
+    rc = lib().orc_sample_latents(K, ctypes.byref(rows.c()), mean, ctypes.cast(arr, ctypes.c_void_p), alpha,
+                                  _ptr(mu), _ptr(Lam), _ptr(mr), seed, it, mode, _ptr(U))
+    if rc:
+        raise RuntimeError("Cholesky Decomposition failed! (row %d)" % (-rc - 1))
+    return U
+
+
+def cond_normal_wishart(U, mu0, kappa, T, nu, seed, it, mode):
+    """U as (N, K) C-order (i.e. K x N column-major)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N, K = U.shape
+    mu = np.zeros(K)
+    Lam = np.zeros((K, K), order="F")
+    rc = lib().orc_cond_normal_wishart(K, N, _ptr(U), _ptr(np.ascontiguousarray(mu0, dtype=np.float64)), kappa,
+                                       _ptr(np.asfortranarray(T, dtype=np.float64)), nu, seed, it, mode,
+                                       _ptr(mu), _ptr(Lam))
+    if rc:
+        raise RuntimeError("CondNormalWishart failed (%d)" % rc)
+    return mu, np.array(Lam)
+
+
+def train_sse(K, rows0, mean, factors):
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    arr = (ctypes.c_void_p * len(fs))(*[_ptr(f) for f in fs])
+    return lib().orc_train_sse(K, len(fs), ctypes.byref(rows0.c()), mean, ctypes.cast(arr, ctypes.c_void_p))
+
+
+def noise_alpha(sse, nnz, var_total, is_tensor, alpha_max, seed, it):
+    return lib().orc_noise_alpha(sse, nnz, var_total, int(is_tensor), alpha_max, seed, it)
+
+
+def eval_rmse(K, test_idx, test_vals, mean, factors, n, pred, pred_var):
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    arr = (ctypes.c_void_p * len(fs))(*[_ptr(f) for f in fs])
+    ti = np.ascontiguousarray(test_idx, dtype=np.int32)
+    tv = np.ascontiguousarray(test_vals, dtype=np.float64)
+    r1, ra = _dbl(), _dbl()
+    lib().orc_eval_rmse(K, len(fs), len(tv), _ptr(ti), _ptr(tv), mean, ctypes.cast(arr, ctypes.c_void_p), n,
+                        _ptr(pred), _ptr(pred_var), ctypes.byref(r1), ctypes.byref(ra))
+    return r1.value, ra.value
+
+
+def csr_A_mul_Bn(A, B):
+    """out[r] = sum_c A[r,c] B[c]; B is (ncol, nrhs) C-order."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    out = np.zeros((A.nrow, B.shape[1]))
+    lib().orc_csr_A_mul_Bn(B.shape[1], ctypes.byref(A.c()), _ptr(B), _ptr(out))
+    return out
+
+
+def AtA_mul_B(A, At, reg, B):
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    out = np.zeros_like(B)
+    lib().orc_AtA_mul_B(B.shape[1], ctypes.byref(A.c()), ctypes.byref(At.c()), reg, _ptr(B), _ptr(out))
+    return out
+
+
+def blockcg(A, At, reg, B, tol, blocksize=None, excess=None):
+    """Solve X (A^T A + reg I) = B; B is (nfeat, nrhs) C-order (= nrhs x nfeat col-major)."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    X = np.zeros_like(B)
+    if blocksize is None:
+        it = lib().orc_blockcg(B.shape[1], ctypes.byref(A.c()), ctypes.byref(At.c()), reg, _ptr(B), tol, _ptr(X))
+    else:
+        it = lib().orc_blockcg_blocked(B.shape[1], ctypes.byref(A.c()), ctypes.byref(At.c()), reg, _ptr(B), tol,
+                                       blocksize, excess, _ptr(X))
+    return X, it
+
+
+def posterior_lambda_beta(beta, Lambda, nu, mu):
+    """beta is (nfeat, K) C-order (K x nfeat col-major)."""
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    nfeat, K = beta.shape
+    a, b = _dbl(), _dbl()
+    lib().orc_posterior_lambda_beta(K, nfeat, _ptr(beta), _ptr(np.asfortranarray(Lambda, dtype=np.float64)),
+                                    nu, mu, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+class RunResult:
+    pass
+
+
+def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False, adaptive=False,
+        precision=1.0, sn_init=1.0, sn_max=10.0, side=None, lambda_beta=5.0, tol=1e-6, burnin=10,
+        nsamples=10, seed=0, max_iters=0, want_U=False):
+    """Full Gibbs run (MacauX::run).  side: per-mode (F, Ft) Csr pairs or None."""
+    M = len(dims)
+    cfg = OrcRunCfg()
+    cfg.nmodes, cfg.K, cfg.is_tensor = M, K, int(is_tensor)
+    keep = []
+    for m in range(M):
+        cfg.dims[m] = dims[m]
+        cfg.train[m] = train_modes[m].c()
+    cfg.mean, cfg.nnz = mean, nnz
+    ti = np.ascontiguousarray(test_idx, dtype=np.int32).reshape(-1, M)
+    tv = np.ascontiguousarray(test_vals, dtype=np.float64)
+    keep += [ti, tv]
+    cfg.ntest, cfg.test_idx, cfg.test_vals = len(tv), _ptr(ti), _ptr(tv)
+    cfg.noise_adaptive, cfg.precision, cfg.sn_init, cfg.sn_max = int(adaptive), precision, sn_init, sn_max
+    side = side or [None] * M
+    for m in range(M):
+        if side[m] is not None:
+            cfg.macau[m] = 1
+            cfg.F[m] = side[m][0].c()
+            cfg.Ft[m] = side[m][1].c()
+    cfg.lambda_beta0, cfg.tol = lambda_beta, tol
+    cfg.burnin, cfg.nsamples, cfg.seed, cfg.max_iters = burnin, nsamples, seed, max_iters
+    pred = np.zeros(len(tv))
+    pvar = np.zeros(len(tv))
+    total = burnin + nsamples if max_iters <= 0 else min(max_iters, burnin + nsamples)
+    at = np.zeros(max(total, 1))
+    rt = np.zeros(max(total, 1))
+    lb = np.zeros(8)
+    cfg.pred, cfg.pred_var, cfg.alpha_trace, cfg.rmse_trace = _ptr(pred), _ptr(pvar), _ptr(at), _ptr(rt)
+    cfg.lambda_beta_out = _ptr(lb)
+    Us = []
+    if want_U:
+        for m in range(M):
+            u = np.zeros((dims[m], K))
+            Us.append(u)
+            cfg.U_out[m] = _ptr(u)
+    rc = lib().orc_run(ctypes.byref(cfg))
+    if rc:
+        raise RuntimeError("oracle run failed (%d)" % rc)
+    res = RunResult()
+    res.rmse, res.rmse1, res.alpha = cfg.rmse, cfg.rmse1, cfg.alpha
+    res.pred, res.pred_var = pred, pvar
+    res.alpha_trace, res.rmse_trace = at[:total], rt[:total]
+    res.U = Us
+    res.lambda_beta = lb[:M]
+    res.cg_iters_last = cfg.cg_iters_last
+    return res

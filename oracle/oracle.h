@@ -1,0 +1,159 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the jaak-s/macau Gibbs sampling step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in macau_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Every function follows the reference algorithm it cites (file:line into the
+ * jaak-s/macau tree).  All randomness is consumed through counter-based
+ * Philox4x32-10 draws indexed by logical coordinates (see "Draw indexing"
+ * below), so the oracle and the GPU consume identical standard-normal and
+ * gamma draws.  The reference itself is unseeded (macau.cpp:73,
+ * mvnormal.cpp:27-43), so no reference run can pin a sampled value; the
+ * sampled quantities here are pinned by closed forms, textbook routines and
+ * the reference's own known-answer tests (see tests/).
+ *
+ * Draw indexing (shared contract with the GPU library):
+ *   key      = (seed & 0xffffffff, seed >> 32)
+ *   normal j of stream (c0, it, tag)  -> block = philox(c0, j/4, it, tag),
+ *                                        component j%4 of Box-Muller on
+ *                                        (w0,w1) -> z0,z1 and (w2,w3) -> z2,z3
+ *   gamma    of stream (c0, it, tag)  -> Marsaglia-Tsang, attempt t uses
+ *                                        philox(c0, t, it, tag)
+ *   tags: ORC_TAG_* below, (stream << 24) | mode.
+ */
+#ifndef MACAU_ORACLE_H
+#define MACAU_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_TAG_LATENT      1u  /* c0 = row,  j = latent component          */
+#define ORC_TAG_WISHART_Z   2u  /* c0 = i,    j = column (upper Bartlett)   */
+#define ORC_TAG_WISHART_CHI 3u  /* c0 = i,    gamma for the i-th diagonal   */
+#define ORC_TAG_NW_MU       4u  /* c0 = 0,    j = component of mu           */
+#define ORC_TAG_NOISE       5u  /* c0 = 0,    gamma for the noise precision */
+#define ORC_TAG_BETA_U      6u  /* c0 = col,  j = component (sample_beta)   */
+#define ORC_TAG_BETA_F      7u  /* c0 = feat, j = component (sample_beta)   */
+#define ORC_TAG_LAMBDA_BETA 8u  /* c0 = 0,    gamma for lambda_beta         */
+#define ORC_TAG(stream, mode) (((uint32_t)(stream) << 24) | (uint32_t)(mode))
+
+/* ---- RNG (mvnormal.cpp:59-166 replaced by counter-based draws) ---- */
+void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void   orc_normal_block(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t it, uint32_t tag, double z[4]);
+double orc_normal(uint64_t seed, uint32_t c0, uint32_t j, uint32_t it, uint32_t tag);
+void   orc_normals(uint64_t seed, uint32_t c0, uint32_t n, uint32_t it, uint32_t tag, double* out);
+double orc_gamma(uint64_t seed, double shape, uint32_t c0, uint32_t it, uint32_t tag);
+
+/* ---- sparse rows of one mode (sparsetensor.h:76-91, Eigen CSC columns) ---- */
+typedef struct {
+  int64_t        nrows;
+  int            nother;   /* number of other modes (1 for a matrix)        */
+  const int64_t* row_ptr;  /* nrows + 1                                      */
+  const int32_t* idx;      /* nnz * nother, row-major, view order           */
+  const double*  vals;     /* nnz, raw observed values                      */
+} orc_rows;
+
+/* CSR matrix (libfastsparse CSR, linop.h:32-51) */
+typedef struct {
+  int64_t        nrow, ncol;
+  const int64_t* row_ptr;
+  const int32_t* cols;
+  const double*  vals;
+} orc_csr;
+
+/* ---- dense helpers (Eigen LLT / triangular solves) ---- */
+int  orc_chol_lower(int n, double* A);                 /* A = L L^T in place, col-major */
+void orc_trsv_lower(int n, const double* L, double* x); /* x <- L^{-1} x  */
+void orc_trsv_upper_t(int n, const double* L, double* x); /* x <- L^{-T} x */
+
+/* ---- a6/a7/a8: per-row latent sampler ----
+ * sample_latent_blas (latentprior.cpp:361-385) and sample_latent_tensor
+ * (latentprior.cpp:118-161): for each row n
+ *   P = Lambda + alpha sum c c^T,  b = alpha sum c (y - mean) + Lambda mu_n
+ *   u = L^{-T}(L^{-1} b + z),  P = L L^T
+ * where c is the Hadamard product of the other modes' columns.
+ * mu_rows (K x nrows) replaces mu per row when non-NULL (MacauPrior,
+ * latentprior.cpp:218-227).  Returns 0, or -1-row if Cholesky fails. */
+int64_t orc_sample_latents(int K, const orc_rows* rows, double mean, const double* const* others,
+                           double alpha, const double* mu, const double* Lambda,
+                           const double* mu_rows, uint64_t seed, uint32_t it, int mode, double* U);
+
+/* ---- a9: CondNormalWishart (mvnormal.cpp:314-336) with NormalWishart
+ * (:270-287), Wishart (:243-266), WishartUnit (:217-241), MvNormal_prec
+ * (:169-184).  U is K x N column-major. */
+int orc_cond_normal_wishart(int K, int64_t N, const double* U, const double* mu0, double kappa,
+                            const double* T, double nu, uint64_t seed, uint32_t it, int mode,
+                            double* mu_out, double* Lambda_out);
+/* the same draw from given sufficient statistics sum (K) and scatter (K x K) */
+int orc_normal_wishart_from_stats(int K, int64_t N, const double* sum, const double* scatter,
+                                  const double* mu0, double kappa, const double* T, double nu,
+                                  uint64_t seed, uint32_t it, int mode, double* mu_out,
+                                  double* Lambda_out);
+
+/* ---- a12: noise (noisemodels.cpp:14-124) ---- */
+double orc_train_sse(int K, int nmodes, const orc_rows* rows0, double mean, const double* const* U);
+double orc_noise_alpha(double sse, int64_t nnz, double var_total, int is_tensor, double alpha_max,
+                       uint64_t seed, uint32_t it);
+
+/* ---- a13: eval_rmse / eval_rmse_tensor (bpmfutils.h:86-114, bpmfutils.cpp:10-73) ----
+ * test_idx is ntest x nmodes row-major in output order. */
+void orc_eval_rmse(int K, int nmodes, int64_t ntest, const int32_t* test_idx, const double* test_vals,
+                   double mean, const double* const* U, int n, double* pred, double* pred_var,
+                   double* rmse1, double* rmse_avg);
+
+/* ---- a10/a11: Macau link matrix ---- */
+/* out[:, r] = sum_c A[r, c] B[:, c]  (A_mul_Bx<N>, linop.h:374-403; libfastsparse csr_A_mul_Bn) */
+void orc_csr_A_mul_Bn(int nrhs, const orc_csr* A, const double* B, double* out);
+/* out = (A^T A + reg I) B as rows (AtA_mul_Bx<N>, linop.h:502-535) */
+void orc_AtA_mul_B(int nrhs, const orc_csr* A, const orc_csr* At, double reg, const double* B, double* out);
+/* solve_blockcg core (linop.h:228-343): X (nrhs x nfeat, col-major K x nfeat)
+ * solves X (A^T A + reg I) = B.  Returns the iteration count. */
+int orc_blockcg(int nrhs, const orc_csr* A, const orc_csr* At, double reg, const double* B, double tol, double* X);
+/* blocked driver (linop.h:206-226) */
+int orc_blockcg_blocked(int nrhs, const orc_csr* A, const orc_csr* At, double reg, const double* B,
+                        double tol, int blocksize, int excess, double* X);
+void orc_posterior_lambda_beta(int K, int64_t nfeat, const double* beta, const double* Lambda, double nu,
+                               double mu, double* shape, double* scale);
+
+/* ---- a4: full Gibbs run (macau.cpp:158-235) ---- */
+typedef struct {
+  int      nmodes, K, is_tensor;
+  int64_t  dims[8];
+  orc_rows train[8];       /* per-mode rows, raw values                      */
+  double   mean;
+  int64_t  nnz;            /* train cells (after duplicate merging)         */
+  int64_t  ntest;
+  const int32_t* test_idx; /* ntest x nmodes row-major, output order         */
+  const double*  test_vals;
+  int      noise_adaptive;
+  double   precision, sn_init, sn_max;
+  int      macau[8];       /* 1 = MacauPrior with sparse side information   */
+  orc_csr  F[8], Ft[8];
+  double   lambda_beta0, tol;
+  int      burnin, nsamples;
+  uint64_t seed;
+  int      max_iters;      /* >0: stop after this many iterations           */
+  /* outputs */
+  double*  pred;           /* ntest */
+  double*  pred_var;       /* ntest */
+  double*  U_out[8];       /* K x dims[m] each, or NULL                     */
+  double   rmse, rmse1, alpha;
+  double*  alpha_trace;    /* per-iteration noise precision, or NULL         */
+  double*  rmse_trace;     /* per-iteration posterior-mean RMSE, or NULL     */
+  double*  lambda_beta_out;/* per mode, or NULL                              */
+  int      cg_iters_last;
+} orc_run_cfg;
+int orc_run(orc_run_cfg* cfg);
+
+void orc_set_threads(int n);
+int  orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
