@@ -69,11 +69,16 @@ int macau_get_test_data(macau_model* m, double* out);   /* ntest x (nmodes+1), c
 /* ---- extensions (no reference counterpart) ---- */
 int macau_set_seed(macau_model* m, unsigned long long seed);   /* counter-based RNG key */
 int macau_set_sampler(macau_model* m, int kind);               /* MACAU_SAMPLER_* */
+int macau_sampler_supported(int kind, int num_latent, int nmodes); /* 1 if available */
 int macau_set_graph(macau_model* m, int on);                   /* replay iterations as a HIP graph */
 int macau_init(macau_model* m);                                /* MacauX::init (macau.cpp:158-173) */
 int macau_iterate(macau_model* m, int n);                      /* n Gibbs iterations, asynchronous */
 int macau_synchronize(macau_model* m);                         /* wait; raise sampler failures */
 int macau_get_iteration(macau_model* m);
+/* n iterations without graph replay, timed with HIP events on the model's
+ * stream; phase_ms[nmodes + 3] receives the mean ms of: sampling of each
+ * mode, the prior updates of all modes, the noise update, the evaluation */
+int macau_profile(macau_model* m, int n, double* phase_ms);
 double macau_get_rmse_onesample(macau_model* m);
 double macau_get_rmse_avg(macau_model* m);
 double macau_get_noise_precision(macau_model* m);

@@ -45,11 +45,13 @@ SIGNATURES = {
     "macau_get_test_data": (_i, [_p, _p]),
     "macau_set_seed": (_i, [_p, _ull]),
     "macau_set_sampler": (_i, [_p, _i]),
+    "macau_sampler_supported": (_i, [_i, _i, _i]),
     "macau_set_graph": (_i, [_p, _i]),
     "macau_init": (_i, [_p]),
     "macau_iterate": (_i, [_p, _i]),
     "macau_synchronize": (_i, [_p]),
     "macau_get_iteration": (_i, [_p]),
+    "macau_profile": (_i, [_p, _i, _p]),
     "macau_get_rmse_onesample": (_d, [_p]),
     "macau_get_rmse_avg": (_d, [_p]),
     "macau_get_noise_precision": (_d, [_p]),

@@ -160,6 +160,18 @@ int macau_synchronize(macau_model* h) {
   M(h).check_fail();
   API_END_INT
 }
+int macau_profile(macau_model* h, int n, double* phase_ms) {
+  API_BEGIN
+  auto ms = M(h).profile(n);
+  memcpy(phase_ms, ms.data(), ms.size() * sizeof(double));
+  API_END_INT
+}
+int macau_sampler_supported(int kind, int num_latent, int nmodes) {
+  if (num_latent < 1 || num_latent > 128 || nmodes < 2 || nmodes > kMaxOther + 1) return 0;
+  if (kind == SAMPLER_PLAIN) return 1;
+  if (kind == SAMPLER_BATCHED) return batched_supported(pad_latent(num_latent), nmodes - 1);
+  return 0;
+}
 int macau_get_iteration(macau_model* h) {
   API_BEGIN
   return M(h).host_iter;
@@ -391,7 +403,7 @@ int macau_cond_normal_wishart(int K, int N, const double* U, const double* mu0, 
   upload(dfail, std::vector<int>{0}, s);
   const int nblk = hyper_stats_blocks(N);
   dpart.alloc((size_t)nblk * (K + K * K) * sizeof(double));
-  if (K > 32) dws.alloc((size_t)(3 * K * K + 3 * K) * sizeof(double));
+  if (nw_workspace_bytes(K)) dws.alloc(nw_workspace_bytes(K));
   dmu.alloc(K * sizeof(double));
   dL.alloc((size_t)K * K * sizeof(double));
   dLf.alloc((size_t)KP * KP * sizeof(float));

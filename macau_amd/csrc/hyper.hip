@@ -18,7 +18,7 @@ constexpr int kStatThreads = 256;
 int hyper_stats_blocks(int N) {
   int b = (N + 511) / 512;
   if (b < 1) b = 1;
-  if (b > 256) b = 256;
+  if (b > 64) b = 64;
   return b;
 }
 
@@ -96,7 +96,8 @@ void launch_hyper_stats_f64(const double* U, int ld, int N, int K, double* parti
 // ------------------------------------------------------------------
 constexpr int kNWThreads = 256;
 
-// A = L L^T in place (lower), upper zeroed.  Returns false if not SPD.
+// A = L L^T in place (lower), right-looking, upper part zeroed.  Returns
+// false if a pivot is not positive.
 __device__ bool blk_chol(double* A, int K, int* sflag) {
   const int t = threadIdx.x;
   for (int k = 0; k < K; k++) {
@@ -106,8 +107,8 @@ __device__ bool blk_chol(double* A, int K, int* sflag) {
       A[k + k * K] = sqrt(d > 0.0 ? d : 1.0);
     }
     __syncthreads();
-    const double d = A[k + k * K];
-    for (int i = k + 1 + t; i < K; i += kNWThreads) A[i + k * K] /= d;
+    const double inv = 1.0 / A[k + k * K];
+    for (int i = k + 1 + t; i < K; i += kNWThreads) A[i + k * K] *= inv;
     __syncthreads();
     const int m = K - k - 1;
     for (int q = t; q < m * m; q += kNWThreads) {
@@ -124,56 +125,24 @@ __device__ bool blk_chol(double* A, int K, int* sflag) {
   return *sflag == 0;
 }
 
-// W = L^{-1} (lower): thread j solves L w = e_j
-__device__ void blk_tri_inv(const double* L, double* W, int K) {
-  const int j = threadIdx.x;
-  if (j < K) {
-    for (int i = 0; i < j; i++) W[i + j * K] = 0.0;
-    for (int i = j; i < K; i++) {
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int k = j; k < i; k++) s -= L[i + k * K] * W[k + j * K];
-      W[i + j * K] = s / L[i + i * K];
-    }
-  }
-  __syncthreads();
-}
-
-// C = op(A) * op(B), op = transpose if flag set
-__device__ void blk_gemm(const double* A, bool ta, const double* B, bool tb, double* C, int K) {
-  for (int q = threadIdx.x; q < K * K; q += kNWThreads) {
-    const int i = q % K, j = q / K;
-    double s = 0.0;
-    for (int k = 0; k < K; k++) {
-      const double a = ta ? A[k + i * K] : A[i + k * K];
-      const double b = tb ? B[j + k * K] : B[k + j * K];
-      s += a * b;
-    }
-    C[q] = s;
-  }
-  __syncthreads();
-}
-
-// x <- L^{-T} x (x in shared memory), column-oriented back substitution
-__device__ void blk_trsv_upper_t(const double* L, double* x, int K) {
-  const int t = threadIdx.x;
-  for (int j = K - 1; j >= 0; j--) {
-    if (t == 0) x[j] = x[j] / L[j + j * K];
-    __syncthreads();
-    const double xj = x[j];
-    for (int i = t; i < j; i += kNWThreads) x[i] -= L[j + i * K] * xj;
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWArgs a) {
-  extern __shared__ double dyn[];
+// Normal-Wishart draw of one mode (CondNormalWishart, mvnormal.cpp:314-336).
+// With X = T + S + kappa mu0 mu0^T - kappa_c mu_c mu_c^T and J the reversal
+// permutation, M = chol(J X J) gives chol(X^{-1}) = J M^{-T} J (the lower
+// Cholesky factor is unique), so the Bartlett draw
+//   Lambda = L_T (C^T C) L_T^T,  L_T = chol(T_c) = chol(X^{-1})
+// (Wishart, mvnormal.cpp:243-266) becomes Lambda = B B^T with
+// B = L_T C^T = J M^{-T} J C^T.  B is lower triangular with a positive
+// diagonal, i.e. B = chol(Lambda), hence chol(kappa_c Lambda) = sqrt(kappa_c) B
+// and mu = mu_c + B^{-T} z / sqrt(kappa_c) (MvNormal_prec, mvnormal.cpp:169-184).
+// These are the reference's quantities computed through one Cholesky and one
+// triangular solve instead of an explicit inverse, three Cholesky factors and
+// three products.
+__device__ void nw_draw(const NWArgs& a, double* base) {
   __shared__ int sflag;
   const int K = a.K, KK = K * K, t = threadIdx.x;
-  // matrices live in LDS when they fit, else in the global workspace
-  double* base = a.ws ? a.ws : dyn;
-  double* A = base;
-  double* B = A + KK;
-  double* D = B + KK;
+  double* A = base;       // X, then M
+  double* Bm = A + KK;    // C (Bartlett, upper), then Z, then B
+  double* D = Bm + KK;    // Lambda
   double* v_mu = D + KK;
   double* v_z = v_mu + K;
   double* v_nu = v_z + K;
@@ -181,71 +150,106 @@ __global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWArgs a) {
   const int it = *a.iter;
   const double N = a.N;
   const double kappa_c = a.kappa + N, nu_c = a.nu + N;
-  // 1. reduce the partial statistics (fixed order)
+  // 1. reduce the partial statistics (fixed block order, loads batched)
+  const size_t stride = (size_t)(K + KK);
   for (int q = t; q < K + KK; q += kNWThreads) {
     double s = 0.0;
-    for (int b = 0; b < a.nblk; b++) s += a.partial[(size_t)b * (K + KK) + q];
-    if (q < K) v_nu[q] = s;
-    else A[q - K] = s;  // NS
-  }
-  // Macau: T = WI + lambda_beta * beta beta^T (latentprior.cpp:258-260)
-  const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
-  for (int q = t; q < KK; q += kNWThreads) {
-    double tq = a.WI[q];
-    if (a.BBt_partial) {
-      double s = 0.0;
-      for (int b = 0; b < a.bbt_nblk; b++) s += a.BBt_partial[(size_t)b * (K + KK) + K + q];
-      tq += lb * s;
+    int b = 0;
+    for (; b + 8 <= a.nblk; b += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = a.partial[(size_t)(b + u) * stride + q];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
     }
-    B[q] = tq;
+    for (; b < a.nblk; b++) s += a.partial[(size_t)b * stride + q];
+    if (q < K) v_nu[q] = s;
+    else D[q - K] = s;  // NS
   }
   __syncthreads();
-  // 2. mu_c and X (mvnormal.cpp:322-325)
+  // 2. mu_c and X (mvnormal.cpp:322-325); Macau adds lambda_beta beta beta^T
+  //    to T (latentprior.cpp:258-260).  A holds J X J.
   for (int k = t; k < K; k += kNWThreads) v_mu[k] = (a.kappa * a.mu0[k] + v_nu[k]) / (a.kappa + N);
+  const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
   __syncthreads();
   for (int q = t; q < KK; q += kNWThreads) {
     const int i = q % K, j = q / K;
-    A[q] = B[q] + A[q] + a.kappa * a.mu0[i] * a.mu0[j] - kappa_c * v_mu[i] * v_mu[j];
+    double tq = a.WI[q];
+    if (a.BBt_partial) {
+      double s = 0.0;
+      for (int b = 0; b < a.bbt_nblk; b++) s += a.BBt_partial[(size_t)b * stride + K + q];
+      tq += lb * s;
+    }
+    const double x = tq + D[q] + a.kappa * a.mu0[i] * a.mu0[j] - kappa_c * v_mu[i] * v_mu[j];
+    A[(K - 1 - i) + (K - 1 - j) * K] = x;
   }
-  __syncthreads();
-  // 3. T_c = X^{-1} through the Cholesky factor (mvnormal.cpp:326)
-  bool ok = blk_chol(A, K, &sflag);
-  blk_tri_inv(A, B, K);                 // B = L_X^{-1}
-  blk_gemm(B, true, B, false, A, K);    // A = T_c = L_X^{-T} L_X^{-1}
-  // 4. Wishart(T_c, nu_c): L = chol(T_c), Lambda = L (C^T C) L^T
-  ok = blk_chol(A, K, &sflag) && ok;
+  // Bartlett factor C (upper): C_ii = sqrt(chi2(nu_c - i)), C_ij ~ N(0,1)
+  // (WishartUnit, mvnormal.cpp:217-241)
   for (int q = t; q < KK; q += kNWThreads) {
     const int i = q % K, j = q / K;
     double c = 0.0;
     if (i == j) {
-      const double g = gamma_f64(a.seed, 0.5 * (nu_c - i), (uint32_t)i, (uint32_t)it,
-                                 make_tag(TAG_WISHART_CHI, a.mode));
-      c = sqrt(2.0 * g);
+      c = sqrt(2.0 * gamma_f64(a.seed, 0.5 * (nu_c - i), (uint32_t)i, (uint32_t)it, make_tag(TAG_WISHART_CHI, a.mode)));
     } else if (j > i) {
       c = normal_f64(a.seed, (uint32_t)i, (uint32_t)j, (uint32_t)it, make_tag(TAG_WISHART_Z, a.mode));
     }
-    B[q] = c;
+    Bm[q] = c;
   }
-  __syncthreads();
-  blk_gemm(B, true, B, false, D, K);    // D = AU = C^T C
-  blk_gemm(A, false, D, false, B, K);   // B = L AU
-  blk_gemm(B, false, A, true, D, K);    // D = Lambda = L AU L^T
-  // 5. mu = mu_c + chol(kappa_c Lambda)^{-T} z
-  for (int q = t; q < KK; q += kNWThreads) B[q] = D[q] * kappa_c;
   for (int k = t; k < K; k += kNWThreads)
     v_z[k] = normal_f64(a.seed, 0u, (uint32_t)k, (uint32_t)it, make_tag(TAG_NW_MU, a.mode));
   __syncthreads();
-  ok = blk_chol(B, K, &sflag) && ok;
-  blk_trsv_upper_t(B, v_z, K);
+  // 3. M = chol(J X J)
+  bool ok = blk_chol(A, K, &sflag);
+  // 4. Z = M^{-T} R, R = J C^T (R[i][c] = C[c][K-1-i]); rows of Z from the
+  //    bottom; Z[i][c] = 0 for i + c > K - 1 (exact structural zeros)
+  for (int i = K - 1; i >= 0; i--) {
+    for (int c = t; c < K; c += kNWThreads) {
+      if (i + c <= K - 1) {
+        double s = Bm[c + (K - 1 - i) * K];  // C[c][K-1-i]
+        for (int k = i + 1; k + c <= K - 1; k++) s -= A[k + i * K] * D[k + c * K];
+        D[i + c * K] = s / A[i + i * K];
+      } else {
+        D[i + c * K] = 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  // B = J Z (lower triangular)
+  for (int q = t; q < KK; q += kNWThreads) {
+    const int i = q % K, c = q / K;
+    Bm[q] = D[(K - 1 - i) + c * K];
+  }
+  __syncthreads();
+  // 5. Lambda = B B^T
+  for (int q = t; q < KK; q += kNWThreads) {
+    const int i = q % K, j = q / K;
+    const int m = i < j ? i : j;
+    double s = 0.0;
+    for (int k = 0; k <= m; k++) s += Bm[i + k * K] * Bm[j + k * K];
+    D[q] = s;
+  }
+  __syncthreads();
+  // 6. mu = mu_c + B^{-T} z / sqrt(kappa_c)  (B^T upper, column-oriented)
+  for (int j = K - 1; j >= 0; j--) {
+    if (t == 0) v_z[j] = v_z[j] / Bm[j + j * K];
+    __syncthreads();
+    const double xj = v_z[j];
+    for (int i = t; i < j; i += kNWThreads) v_z[i] -= Bm[j + i * K] * xj;
+    __syncthreads();
+  }
+  const double isk = 1.0 / sqrt(kappa_c);
   for (int k = t; k < K; k += kNWThreads) {
-    const double m = v_mu[k] + v_z[k];
+    const double m = v_mu[k] + v_z[k] * isk;
     a.mu[k] = m;
     v_mu[k] = m;
   }
   __syncthreads();
-  // 6. outputs: FP64 state and the FP32 sampler parameters
+  // 7. outputs: FP64 state and the FP32 sampler parameters
   const int KP = a.KP;
-  for (int q = t; q < KK; q += kNWThreads) a.Lambda[q] = D[q];
+  for (int q = t; q < KK; q += kNWThreads) {
+    a.Lambda[q] = D[q];
+    if (a.Lchol) a.Lchol[q] = Bm[q];
+  }
   for (int q = t; q < KP * KP; q += kNWThreads) {
     const int i = q % KP, j = q / KP;
     float v = (i == j) ? 1.f : 0.f;
@@ -258,23 +262,30 @@ __global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWArgs a) {
       for (int j = 0; j < K; j++) s += D[i + j * K] * v_mu[j];
     a.Lmu_f[i] = (float)s;
   }
-  if (a.Lchol) {
-    __syncthreads();
-    for (int q = t; q < KK; q += kNWThreads) A[q] = D[q];
-    __syncthreads();
-    ok = blk_chol(A, K, &sflag) && ok;
-    for (int q = t; q < KK; q += kNWThreads) a.Lchol[q] = A[q];
-  }
   if (t == 0 && !ok && a.fail) atomicMax(a.fail, 1 << 30);
 }
 
+// one work-group per mode of the batch
+__global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWBatch b) {
+  extern __shared__ double dyn[];
+  const NWArgs& a = b.m[blockIdx.x];
+  nw_draw(a, a.ws ? a.ws : dyn);
+}
+
+static size_t nw_lds_bytes(int K) { return (size_t)(3 * K * K + 3 * K) * sizeof(double); }
+size_t nw_workspace_bytes(int K) { return nw_lds_bytes(K) <= 48 * 1024 ? 0 : nw_lds_bytes(K); }
+
+void launch_nw_draw_batch(const NWBatch& b, hipStream_t s) {
+  if (b.n <= 0) return;
+  const bool lds = b.m[0].ws == nullptr;
+  hipLaunchKernelGGL(k_nw_draw, dim3(b.n), dim3(kNWThreads), lds ? nw_lds_bytes(b.m[0].K) : 0, s, b);
+}
+
 void launch_nw_draw(const NWArgs& a, hipStream_t s) {
-  const size_t bytes = (size_t)(3 * a.K * a.K + 3 * a.K) * sizeof(double);
-  if (a.ws) {
-    hipLaunchKernelGGL(k_nw_draw, dim3(1), dim3(kNWThreads), 0, s, a);
-  } else {
-    hipLaunchKernelGGL(k_nw_draw, dim3(1), dim3(kNWThreads), bytes, s, a);
-  }
+  NWBatch b;
+  b.m[0] = a;
+  b.n = 1;
+  launch_nw_draw_batch(b, s);
 }
 
 // ------------------------------------------------------------------
@@ -365,47 +376,93 @@ void launch_noise_draw(const NoiseDrawArgs& a, hipStream_t s) {
 // ------------------------------------------------------------------
 // Evaluation: eval_rmse / eval_rmse_tensor (bpmfutils.h:86-114,
 // bpmfutils.cpp:10-73) with Welford running mean and variance.
+// A group of KP/4 lanes handles one test cell: each lane multiplies one
+// float4 of the factor columns, the group sums by shuffles, and the group's
+// first lane updates the FP64 running mean/variance and squared errors.
 // ------------------------------------------------------------------
 constexpr int kEvalThreads = 256;
 int eval_blocks(int ntest) {
-  int b = (ntest + kEvalThreads * 8 - 1) / (kEvalThreads * 8);
+  int b = (ntest + kEvalThreads * 2 - 1) / (kEvalThreads * 2);
   if (b < 1) b = 1;
-  if (b > 1024) b = 1024;
+  if (b > 2048) b = 2048;
   return b;
 }
 
+template <int KP, int NM>
 __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
-  __shared__ double red[2][kEvalThreads];
+  constexpr int G = KP / 4;  // lanes per test cell
+  __shared__ double red[2][kEvalThreads / 64];
   const int it = *a.iter;
   const int n = it < a.burnin ? 0 : it - a.burnin;
+  const int sub = threadIdx.x % G;
+  const int cells_per_block = kEvalThreads / G;
   double se = 0.0, se_avg = 0.0;
-  for (int q = blockIdx.x * kEvalThreads + threadIdx.x; q < a.ntest; q += gridDim.x * kEvalThreads) {
-    const int* ix = a.idx + (size_t)q * a.nmodes;
+  for (int q0 = blockIdx.x * cells_per_block; q0 < a.ntest; q0 += gridDim.x * cells_per_block) {
+    const int q = q0 + threadIdx.x / G;
     float dot = 0.f;
-    for (int d = 0; d < a.K; d++) {
-      float v = a.f[0][(size_t)ix[0] * a.KP + d];
-      for (int m = 1; m < a.nmodes; m++) v *= a.f[m][(size_t)ix[m] * a.KP + d];
-      dot += v;
+    if (q < a.ntest) {
+      const int* ix = a.idx + (size_t)q * NM;
+      float4 v = reinterpret_cast<const float4*>(a.f[0] + (size_t)ix[0] * KP)[sub];
+#pragma unroll
+      for (int m = 1; m < NM; m++) {
+        const float4 w = reinterpret_cast<const float4*>(a.f[m] + (size_t)ix[m] * KP)[sub];
+        v.x *= w.x;
+        v.y *= w.y;
+        v.z *= w.z;
+        v.w *= w.w;
+      }
+      dot = (v.x + v.y) + (v.z + v.w);
     }
-    const double p = a.mean + (double)dot;
-    double pa;
-    if (n == 0) {
-      pa = p;
-      a.pred_var[q] = 0.0;
-    } else {
-      const double delta = p - a.pred[q];
-      pa = a.pred[q] + delta / (n + 1);
-      a.pred_var[q] += delta * (p - pa);
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) dot += __shfl_xor(dot, off, G);
+    if (sub == 0 && q < a.ntest) {
+      const double p = a.mean + (double)dot;
+      double pa;
+      if (n == 0) {
+        pa = p;
+        a.pred_var[q] = 0.0;
+      } else {
+        const double delta = p - a.pred[q];
+        pa = a.pred[q] + delta / (n + 1);
+        a.pred_var[q] += delta * (p - pa);
+      }
+      a.pred[q] = pa;
+      const double y = a.vals[q];
+      se += (y - p) * (y - p);
+      se_avg += (y - pa) * (y - pa);
     }
-    a.pred[q] = pa;
-    const double y = a.vals[q];
-    se += (y - p) * (y - p);
-    se_avg += (y - pa) * (y - pa);
   }
-  red[0][threadIdx.x] = se;
-  red[1][threadIdx.x] = se_avg;
+  for (int off = 32; off > 0; off >>= 1) {
+    se += __shfl_down(se, off, 64);
+    se_avg += __shfl_down(se_avg, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = se;
+    red[1][threadIdx.x >> 6] = se_avg;
+  }
   __syncthreads();
-  for (int off = kEvalThreads / 2; off > 0; off >>= 1) {
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < kEvalThreads / 64; w++) {
+      s0 += red[0][w];
+      s1 += red[1][w];
+    }
+    a.partial[2 * blockIdx.x] = s0;
+    a.partial[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_eval_finish(EvalArgs a) {
+  __shared__ double red[2][256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int b = threadIdx.x; b < a.nblk; b += 256) {
+    s0 += a.partial[2 * b];
+    s1 += a.partial[2 * b + 1];
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off) {
       red[0][threadIdx.x] += red[0][threadIdx.x + off];
       red[1][threadIdx.x] += red[1][threadIdx.x + off];
@@ -413,29 +470,33 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    a.partial[2 * blockIdx.x] = red[0][0];
-    a.partial[2 * blockIdx.x + 1] = red[1][0];
+    // an empty test set gives sqrt(0 / 0) = NaN (bpmfutils.h:110-112)
+    a.out[0] = a.ntest == 0 ? NAN : sqrt(red[0][0] / (double)a.ntest);
+    a.out[1] = a.ntest == 0 ? NAN : sqrt(red[1][0] / (double)a.ntest);
   }
 }
 
-__global__ void k_eval_finish(EvalArgs a) {
-  if (threadIdx.x == 0 && a.ntest == 0) {
-    a.out[0] = NAN;  // sqrt(0 / 0) (bpmfutils.h:110-112)
-    a.out[1] = NAN;
-  } else if (threadIdx.x == 0) {
-    double se = 0.0, sa = 0.0;
-    for (int b = 0; b < a.nblk; b++) {
-      se += a.partial[2 * b];
-      sa += a.partial[2 * b + 1];
-    }
-    a.out[0] = sqrt(se / (double)a.ntest);
-    a.out[1] = sqrt(sa / (double)a.ntest);
+template <int KP>
+static void launch_eval_k(const EvalArgs& a, hipStream_t s) {
+  switch (a.nmodes) {
+    case 2: hipLaunchKernelGGL((k_eval<KP, 2>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_eval<KP, 3>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_eval<KP, 4>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_eval<KP, 5>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
   }
 }
 
 void launch_eval(const EvalArgs& a, hipStream_t s) {
-  if (a.ntest > 0) hipLaunchKernelGGL(k_eval, dim3(a.nblk), dim3(kEvalThreads), 0, s, a);
-  hipLaunchKernelGGL(k_eval_finish, dim3(1), dim3(64), 0, s, a);
+  if (a.ntest > 0) {
+    switch (a.KP) {
+      case 8: launch_eval_k<8>(a, s); break;
+      case 16: launch_eval_k<16>(a, s); break;
+      case 32: launch_eval_k<32>(a, s); break;
+      case 64: launch_eval_k<64>(a, s); break;
+      default: launch_eval_k<128>(a, s); break;
+    }
+  }
+  hipLaunchKernelGGL(k_eval_finish, dim3(1), dim3(256), 0, s, a);
 }
 
 __global__ void k_advance_iter(int* iter) {

@@ -56,6 +56,10 @@ struct SampleArgs {
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
 void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
 int batched_supported(int KP, int nother);
+// grids of the batched finishing kernels; their SSE partials are laid out
+// [light grid | heavy grid] when SampleArgs::sse_partial is set
+int batched_light_grid(const BatchPlan& p);
+int batched_heavy_grid(const BatchPlan& p);
 
 // --- hyperprior (FP64) ---
 // partial sums of x = U[:,n] - sub[:,n] (sub may be null): sum (K) and
@@ -80,7 +84,7 @@ struct NWArgs {
   uint64_t seed = 0;
   const int* iter = nullptr;
   int mode = 0;
-  double* ws = nullptr;             // workspace, 6*K*K + 4*K doubles
+  double* ws = nullptr;             // workspace (nw_workspace_bytes), or null for LDS
   double* mu = nullptr;             // out K
   double* Lambda = nullptr;         // out K x K
   double* Lchol = nullptr;          // out chol(Lambda), K x K (or null)
@@ -89,6 +93,14 @@ struct NWArgs {
   int* fail = nullptr;
 };
 void launch_nw_draw(const NWArgs& a, hipStream_t s);
+// draws of several modes (same K) in one launch, one work-group per mode
+struct NWBatch {
+  NWArgs m[kMaxOther + 1];
+  int n = 0;
+};
+void launch_nw_draw_batch(const NWBatch& b, hipStream_t s);
+// global workspace the draw needs for latent size K (0: fits in LDS)
+size_t nw_workspace_bytes(int K);
 
 // --- noise and evaluation ---
 struct NoiseArgs {

@@ -288,6 +288,25 @@ void Model::init() {
   for (int m = 0; m < nmodes; m++) priors[m]->init(*this, m);
   noise->init(*this);
   build_plans();
+  // the batched sampler computes the train SSE of the adaptive noise inside
+  // the last half-sweep (u^T G u - 2 u^T c + sum r^2 per row)
+  fused_sse = sampler == SAMPLER_BATCHED && dynamic_cast<AdaptiveGaussianNoise*>(noise.get()) != nullptr;
+  if (fused_sse) {
+    const int last = nmodes - 1;
+    const HostRows& h = rows[last];
+    std::vector<float> r2(dims[last]);
+    for (int n = 0; n < dims[last]; n++) {
+      double s = 0.0;
+      for (int e = h.row_ptr[n]; e < h.row_ptr[n + 1]; e++) {
+        const double r = (double)(float)(h.vals[e] - mean_value);
+        s += r * r;
+      }
+      r2[n] = (float)s;
+    }
+    upload(d_r2_last, r2, stream);
+    sse_partial_n = batched_light_grid(plans[last]) + batched_heavy_grid(plans[last]);
+    d_sse_partial.alloc((size_t)std::max(sse_partial_n, 1) * sizeof(double));
+  }
   HIPCHK(hipStreamSynchronize(stream));
   host_iter = 0;
   initialized = true;
@@ -332,7 +351,8 @@ BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<
   upload(B[3], se, s);
   upload(B[4], heavy, s);
   upload(B[5], hseg0, s);
-  B[6].alloc(std::max<size_t>(sb.size(), 1) * (size_t)(KP * KP + KP) * sizeof(float));
+  // per segment: the KP x KP Gram fragment and one RHS partial per lane (64)
+  B[6].alloc(std::max<size_t>(sb.size(), 1) * (size_t)(KP * KP + 64) * sizeof(float));
   BatchPlan p;
   p.n_light = (int)light.size();
   p.light = B[0].as<int>();
@@ -375,12 +395,25 @@ SampleArgs Model::sample_args(int mode, const ILatentPrior& p) {
   a.iter = d_iter.as<int>();
   a.mode = mode;
   a.fail = d_fail.as<int>();
+  if (fused_sse && mode == nmodes - 1) {
+    a.sse_partial = d_sse_partial.as<double>();
+    a.r2 = d_r2_last.as<float>();
+  }
   return a;
+}
+
+void Model::update_priors() {
+  // priors are independent of each other (each reads only its own mode's
+  // samples); BPMF draws of all modes share one launch
+  NWBatch b;
+  for (int m = 0; m < nmodes; m++)
+    if (!priors[m]->batch_update(*this, m, b)) priors[m]->update_prior(*this, m);
+  launch_nw_draw_batch(b, stream);
 }
 
 void Model::step_launch() {
   for (int m = 0; m < nmodes; m++) priors[m]->sample_latents(*this, m);
-  for (int m = 0; m < nmodes; m++) priors[m]->update_prior(*this, m);
+  update_priors();
   noise->update(*this);
   noise->evalModel(*this);
   launch_advance_iter(d_iter.as<int>(), stream);
@@ -405,6 +438,42 @@ void Model::iterate(int n) {
     }
     host_iter++;
   }
+}
+
+std::vector<double> Model::profile(int n) {
+  if (!initialized) throw std::runtime_error("model not initialized");
+  const int nph = nmodes + 3;
+  std::vector<hipEvent_t> ev((size_t)n * (nph + 1));
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (int i = 0; i < n; i++) {
+    hipEvent_t* E = &ev[(size_t)i * (nph + 1)];
+    int q = 0;
+    HIPCHK(hipEventRecord(E[q++], stream));
+    for (int m = 0; m < nmodes; m++) {
+      priors[m]->sample_latents(*this, m);
+      HIPCHK(hipEventRecord(E[q++], stream));
+    }
+    update_priors();
+    HIPCHK(hipEventRecord(E[q++], stream));
+    noise->update(*this);
+    HIPCHK(hipEventRecord(E[q++], stream));
+    noise->evalModel(*this);
+    launch_advance_iter(d_iter.as<int>(), stream);
+    HIPCHK(hipEventRecord(E[q++], stream));
+    host_iter++;
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  std::vector<double> ms(nph, 0.0);
+  for (int i = 0; i < n; i++) {
+    hipEvent_t* E = &ev[(size_t)i * (nph + 1)];
+    for (int p = 0; p < nph; p++) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, E[p], E[p + 1]));
+      ms[p] += t / n;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return ms;
 }
 
 void Model::check_fail() {
@@ -572,7 +641,7 @@ void ILatentPrior::init(Model& m, int mode) {
   upload(Lmu_f, Lmuf, m.stream);
   stats_nblk = hyper_stats_blocks(m.dims[mode]);
   stats.alloc((size_t)stats_nblk * (K + K * K) * sizeof(double));
-  if (K > 32) nw_ws.alloc((size_t)(3 * K * K + 3 * K) * sizeof(double));
+  if (nw_workspace_bytes(K)) nw_ws.alloc(nw_workspace_bytes(K));
   HIPCHK(hipStreamSynchronize(m.stream));
 }
 
@@ -613,9 +682,16 @@ static NWArgs nw_args(Model& m, ILatentPrior& p, int mode, double nu) {
 }
 
 void BPMFPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:79-81
+  NWBatch b;
+  batch_update(m, mode, b);
+  launch_nw_draw_batch(b, m.stream);
+}
+
+bool BPMFPrior::batch_update(Model& m, int mode, NWBatch& b) {
   launch_hyper_stats_f32(m.d_U[mode].as<float>(), nullptr, m.KP, m.dims[mode], m.num_latent, stats.as<double>(),
                          stats_nblk, m.stream);
-  launch_nw_draw(nw_args(m, *this, mode, (double)m.num_latent), m.stream);  // df = K
+  b.m[b.n++] = nw_args(m, *this, mode, (double)m.num_latent);  // df = K
+  return true;
 }
 
 // ---- MacauPrior ----
@@ -866,6 +942,21 @@ void AdaptiveGaussianNoise::init(Model& m) {  // noisemodels.cpp:14-53
 }
 
 void AdaptiveGaussianNoise::update(Model& m) {  // noisemodels.cpp:55-124
+  NoiseDrawArgs d;
+  d.nnz = (double)m.nnz;
+  d.var_total = var_total;
+  d.b0 = m.is_tensor ? 0.5 : 0.5 * var_total;
+  d.alpha_max = alpha_max;
+  d.seed = m.seed;
+  d.iter = m.d_iter.as<int>();
+  d.alpha = m.d_alpha.as<double>();
+  d.sse_out = m.d_sse.as<double>();
+  if (m.fused_sse) {  // SSE already reduced per wave by the last half-sweep
+    d.partial = m.d_sse_partial.as<double>();
+    d.nblk = m.sse_partial_n;
+    launch_noise_draw(d, m.stream);
+    return;
+  }
   NoiseArgs a;
   a.rows0 = m.rdev[0];
   a.U0 = m.d_U[0].as<float>();
@@ -876,17 +967,8 @@ void AdaptiveGaussianNoise::update(Model& m) {  // noisemodels.cpp:55-124
   a.partial = m.d_noise_partial.as<double>();
   a.nblk = m.noise_nblk;
   launch_noise_sse(a, m.stream);
-  NoiseDrawArgs d;
   d.partial = m.d_noise_partial.as<double>();
   d.nblk = m.noise_nblk;
-  d.nnz = (double)m.nnz;
-  d.var_total = var_total;
-  d.b0 = m.is_tensor ? 0.5 : 0.5 * var_total;
-  d.alpha_max = alpha_max;
-  d.seed = m.seed;
-  d.iter = m.d_iter.as<int>();
-  d.alpha = m.d_alpha.as<double>();
-  d.sse_out = m.d_sse.as<double>();
   launch_noise_draw(d, m.stream);
 }
 

@@ -74,6 +74,9 @@ class ILatentPrior {
   virtual void init(Model& m, int mode);
   virtual void sample_latents(Model& m, int mode);
   virtual void update_prior(Model& m, int mode) = 0;
+  // enqueue the statistics and append the Normal-Wishart draw to a batch
+  // launched for several modes at once; false: use update_prior
+  virtual bool batch_update(Model&, int, NWBatch&) { return false; }
   virtual double getLinkNorm(Model&) { return NAN; }
   virtual double getLinkLambda(Model&) { return NAN; }
   virtual void saveModel(Model& m, int mode, const std::string& prefix);
@@ -88,6 +91,7 @@ class ILatentPrior {
 class BPMFPrior : public ILatentPrior {
  public:
   void update_prior(Model& m, int mode) override;
+  bool batch_update(Model& m, int mode, NWBatch& b) override;
 };
 
 // MacauPrior<SparseDoubleFeat> (latentprior.h:82-125, latentprior.cpp:180-331)
@@ -183,6 +187,10 @@ class Model {
   double noise_precision();
   void set_iteration(int it);
   void capture_graph(bool on) { use_graph = on; }
+  // n iterations launched without a graph, with HIP events on the model's
+  // stream around each phase; returns mean ms per phase:
+  // [sample mode 0..M-1, prior updates (all modes), noise, eval]
+  std::vector<double> profile(int n);
 
   // -- configuration --
   int nmodes, num_latent, KP;
@@ -214,7 +222,9 @@ class Model {
   std::vector<BatchPlan> plans;
   std::vector<std::vector<DevBuf>> plan_bufs;
   DevBuf d_iter, d_alpha, d_fail, d_eval_out, d_test_idx, d_test_vals, d_pred, d_pred_var, d_eval_partial,
-      d_noise_partial, d_norm_partial, d_sse;
+      d_noise_partial, d_norm_partial, d_sse, d_r2_last, d_sse_partial;
+  int sse_partial_n = 0;       // fused SSE partials of the batched sampler (last mode)
+  bool fused_sse = false;
   int eval_nblk = 0, noise_nblk = 0;
   bool initialized = false;
   int host_iter = 0;  // iterations launched so far
@@ -224,6 +234,7 @@ class Model {
   FactorSet others_of(int mode) const;
   SampleArgs sample_args(int mode, const ILatentPrior& p);
   void step_launch();  // enqueue one Gibbs iteration
+  void update_priors();
   double factor_norm(int mode);
   double host_eval_out(int which);
 

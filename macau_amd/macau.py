@@ -9,8 +9,8 @@ runs on the GPU through the C ABI of ``libmacau_amd.so``.
 Differences from the reference, all additive:
   * ``seed=`` keyword: the reference seeds from the wall clock
     (macau.cpp:160); ``seed=None`` keeps that behaviour.
-  * ``sampler=`` keyword: ``"batched"`` (default, split-BF16 MFMA Gram,
-    nnz-bucketed) or ``"plain"`` (FP32 one work-group per row).
+  * ``sampler=`` keyword: ``"auto"`` (default), ``"batched"`` (split-BF16
+    MFMA Gram, nnz-bucketed) or ``"plain"`` (FP32, one work-group per row).
   * sparse side information with <= 10000 features is solved with the block
     CG instead of the dense F'F Cholesky branch (latentprior.cpp:280-289);
     both solve the same system (F'F + lambda I) beta' = Ft_y.
@@ -30,6 +30,14 @@ from . import _lib
 from ._lib import check, ptr
 
 _SAMPLERS = {"plain": _lib.SAMPLER_PLAIN_FP32, "batched": _lib.SAMPLER_BATCHED}
+
+
+def _pick_sampler(sampler, num_latent, nmodes):
+    """'auto' takes the batched kernels where they exist for this latent size."""
+    if sampler == "auto":
+        ok = _lib.lib().macau_sampler_supported(_lib.SAMPLER_BATCHED, num_latent, nmodes)
+        return "batched" if ok else "plain"
+    return sampler
 
 
 class MacauResult(object):
@@ -270,7 +278,7 @@ def macau(Y,
           save_prefix=None,
           verbose=True,
           seed=None,
-          sampler="batched"):
+          sampler="auto"):
     """
     Matrix and tensor factorization with side information.
       Y          training data to factorize (sparse matrix or DataFrame)
@@ -294,7 +302,8 @@ def macau(Y,
       save_prefix  prefix for model files or None if not saving the model
       verbose      whether to print output for each Gibbs iteration
       seed         key of the counter-based random streams (None: clock)
-      sampler      "batched" (split-BF16 MFMA, default) or "plain" (FP32)
+      sampler      "auto" (default: "batched" where available for num_latent),
+                   "batched" (split-BF16 MFMA) or "plain" (FP32)
     """
     data = Data(Y, Ytest)
 
@@ -307,8 +316,8 @@ def macau(Y,
                          (len(side), len(data.shape)))
     if univariate:
         raise NotImplementedError("the univariate sampler (MacauOnePrior) is not provided by macau_amd")
-    if sampler not in _SAMPLERS:
-        raise ValueError("sampler must be one of %s" % sorted(_SAMPLERS))
+    if sampler != "auto" and sampler not in _SAMPLERS:
+        raise ValueError("sampler must be one of %s" % sorted(list(_SAMPLERS) + ["auto"]))
 
     D = int(np.int32(num_latent))
     nmodes = len(data.shape)
@@ -323,7 +332,7 @@ def macau(Y,
     if seed is None:
         seed = int(time.time() * 1e6) & 0xFFFFFFFFFFFFFFFF
     check(L.macau_set_seed(model.h, int(seed)))
-    check(L.macau_set_sampler(model.h, _SAMPLERS[sampler]))
+    check(L.macau_set_sampler(model.h, _SAMPLERS[_pick_sampler(sampler, D, nmodes)]))
 
     if save_prefix is None:
         check(L.macau_set_save_model(model.h, 0))

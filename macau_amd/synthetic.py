@@ -1,0 +1,109 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Each generator returns numpy arrays only (no GPU work):
+  train (rows, cols, vals), test (rows, cols, vals), shape, and a dict of the
+  run settings of that configuration.
+
+* ``c1`` -- 5,000 x 2,000 BPMF, 5% of cells observed uniformly at random,
+  rank-10 truth with N(0,1) factors plus N(0, 0.5^2) noise, 20% held out;
+  K = 10, fixed precision 5, burnin 20, nsamples 50.
+* ``c2`` -- 138,000 x 27,000 recommender-scale BPMF with 20,000,000 observed
+  cells.  Row and column degrees follow a rank-frequency power law
+  d_r = clip(C r^-1.1, 20, cap) with C solved so the degrees sum to the
+  target (row cap = min(30,000, number of columns)); cells are drawn by
+  degree-weighted column sampling without duplicates.  Values come from a
+  rank-32 truth with N(0,1) factors, standardised over the observed cells to
+  mean 3.5 and sd 1.0, rounded to 0.5 steps and clipped to [0.5, 5]; 10% held
+  out; K = 32, adaptive noise precision, burnin 50, nsamples 100.
+"""
+import numpy as np
+
+
+def zipf_degrees(n, total, exponent, lo, hi):
+    """d_r = clip(C r^-exponent, lo, hi), r = 1..n, with sum(d) ~= total."""
+    r = np.arange(1, n + 1, dtype=np.float64)
+    base = r ** (-exponent)
+
+    def tot(C):
+        return np.clip(C * base, lo, hi).sum()
+
+    a, b = 0.0, float(total) * 10
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if tot(m) < total:
+            a = m
+        else:
+            b = m
+    d = np.clip(np.round(b * base), lo, hi).astype(np.int64)
+    return d
+
+
+def _split(rng, r, c, v, frac):
+    perm = rng.permutation(len(v))
+    nte = int(round(len(v) * frac))
+    te, tr = perm[:nte], perm[nte:]
+    return (r[tr], c[tr], v[tr]), (r[te], c[te], v[te])
+
+
+def c1(seed=0):
+    rng = np.random.default_rng(seed)
+    nrow, ncol, K = 5000, 2000, 10
+    ncell = int(0.05 * nrow * ncol)
+    cells = rng.choice(nrow * ncol, ncell, replace=False)
+    r, c = (cells // ncol).astype(np.int32), (cells % ncol).astype(np.int32)
+    U, V = rng.standard_normal((nrow, K)), rng.standard_normal((ncol, K))
+    y = np.einsum("ij,ij->i", U[r], V[c]) + 0.5 * rng.standard_normal(ncell)
+    train, test = _split(rng, r, c, y, 0.2)
+    settings = dict(num_latent=10, precision=5.0, burnin=20, nsamples=50)
+    return train, test, (nrow, ncol), settings
+
+
+def _degree_sampled_cells(rng, deg_r, weight_c, ncol):
+    """Unique (row, col) cells with row degrees ~deg_r and columns drawn with
+    probability proportional to weight_c."""
+    nrow = len(deg_r)
+    p = weight_c / weight_c.sum()
+    cdf = np.cumsum(p)
+    cdf[-1] = 1.0
+    need = deg_r.copy()
+    keys = np.zeros(0, dtype=np.int64)
+    for _ in range(8):
+        if need.sum() == 0:
+            break
+        rows = np.repeat(np.arange(nrow, dtype=np.int64), need)
+        cols = np.searchsorted(cdf, rng.random(len(rows)))
+        keys = np.unique(np.concatenate([keys, rows * ncol + cols]))
+        have = np.bincount((keys // ncol).astype(np.int64), minlength=nrow)
+        need = np.maximum(deg_r - have, 0)
+        # saturated rows take the remaining columns uniformly
+        need = np.minimum(need, ncol - have)
+    return keys
+
+
+def c2(seed=0, nrow=138000, ncol=27000, nnz=20_000_000, K=32):
+    rng = np.random.default_rng(seed)
+    deg_r = zipf_degrees(nrow, nnz, 1.1, 20, min(30000, ncol))
+    deg_c = zipf_degrees(ncol, nnz, 1.1, 20, min(30000, nrow))
+    deg_r = deg_r[rng.permutation(nrow)]
+    w_c = deg_c[rng.permutation(ncol)].astype(np.float64)
+    keys = _degree_sampled_cells(rng, deg_r, w_c, ncol)
+    if len(keys) > nnz:
+        keys = np.sort(rng.choice(keys, nnz, replace=False))
+    elif len(keys) < nnz:
+        # top up with uniformly random new cells
+        extra = np.unique(rng.integers(0, nrow * ncol, 2 * (nnz - len(keys)), dtype=np.int64))
+        extra = np.setdiff1d(extra, keys)[: nnz - len(keys)]
+        keys = np.sort(np.concatenate([keys, extra]))
+    r = (keys // ncol).astype(np.int32)
+    c = (keys % ncol).astype(np.int32)
+    U = rng.standard_normal((nrow, K)).astype(np.float32)
+    V = rng.standard_normal((ncol, K)).astype(np.float32)
+    y = np.empty(len(r), dtype=np.float64)
+    step = 1 << 20
+    for s in range(0, len(r), step):
+        y[s:s + step] = np.einsum("ij,ij->i", U[r[s:s + step]], V[c[s:s + step]])
+    y = (y - y.mean()) / y.std() + 3.5
+    y = np.clip(np.round(y * 2.0) / 2.0, 0.5, 5.0)
+    train, test = _split(rng, r, c, y, 0.1)
+    settings = dict(num_latent=K, precision="adaptive", burnin=50, nsamples=100)
+    return train, test, (nrow, ncol), settings

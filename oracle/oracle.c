@@ -321,11 +321,13 @@ int orc_cond_normal_wishart(int K, int64_t N, const double* U, const double* mu0
 /* ------------------------------------------------------------------ */
 double orc_train_sse(int K, int nmodes, const orc_rows* rows0, double mean, const double* const* U) {
   /* AdaptiveGaussianNoise::update (noisemodels.cpp:55-81, 83-124) */
-  double sumsq = 0.0;
+  /* per-row sums combined in row order: independent of the thread count */
   const int no = rows0->nother;
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : sumsq)
+  double* per_row = (double*)calloc((size_t)rows0->nrows + 1, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 4)
   for (int64_t n = 0; n < rows0->nrows; n++) {
     const double* u = U[0] + (size_t)n * K;
+    double s = 0.0;
     for (int64_t e = rows0->row_ptr[n]; e < rows0->row_ptr[n + 1]; e++) {
       double yhat = mean;
       for (int d = 0; d < K; d++) {
@@ -334,10 +336,13 @@ double orc_train_sse(int K, int nmodes, const orc_rows* rows0, double mean, cons
         yhat += t;
       }
       double r = yhat - rows0->vals[e];
-      sumsq += r * r;
+      s += r * r;
     }
+    per_row[n] = s;
   }
-  (void)nmodes;
+  double sumsq = 0.0;
+  for (int64_t n = 0; n < rows0->nrows; n++) sumsq += per_row[n];
+  free(per_row);
   return sumsq;
 }
 
@@ -355,8 +360,10 @@ double orc_noise_alpha(double sse, int64_t nnz, double var_total, int is_tensor,
 void orc_eval_rmse(int K, int nmodes, int64_t ntest, const int32_t* test_idx, const double* test_vals,
                    double mean, const double* const* U, int n, double* pred, double* pred_var,
                    double* rmse1, double* rmse_avg) {
-  double se = 0.0, se_avg = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : se, se_avg)
+  /* per-cell errors summed in cell order: independent of the thread count */
+  double* e1 = (double*)malloc(sizeof(double) * ((size_t)ntest + 1));
+  double* ea = (double*)malloc(sizeof(double) * ((size_t)ntest + 1));
+#pragma omp parallel for schedule(static)
   for (int64_t t = 0; t < ntest; t++) {
     const int32_t* ix = test_idx + t * nmodes;
     double p = mean;
@@ -373,10 +380,17 @@ void orc_eval_rmse(int K, int nmodes, int64_t ntest, const int32_t* test_idx, co
       pa = pred[t] + delta / (n + 1);
       pred_var[t] += delta * (p - pa);
     }
-    se += (test_vals[t] - p) * (test_vals[t] - p);
-    se_avg += (test_vals[t] - pa) * (test_vals[t] - pa);
+    e1[t] = (test_vals[t] - p) * (test_vals[t] - p);
+    ea[t] = (test_vals[t] - pa) * (test_vals[t] - pa);
     pred[t] = pa;
   }
+  double se = 0.0, se_avg = 0.0;
+  for (int64_t t = 0; t < ntest; t++) {
+    se += e1[t];
+    se_avg += ea[t];
+  }
+  free(e1);
+  free(ea);
   /* matrix: sqrt(se/N) is NaN for an empty test set (bpmfutils.h:110-112);
    * tensor: explicit NaN (bpmfutils.cpp:39-43) -- identical result */
   *rmse1 = sqrt(se / (double)ntest);
@@ -412,21 +426,27 @@ void orc_AtA_mul_B(int nrhs, const orc_csr* A, const orc_csr* At, double reg, co
 /* lower triangle of A B^T for nrhs x nfeat operands stored K x nfeat
  * (A_mul_Bt_omp_sym / A_mul_At_omp, linop.cpp:271-372), then symmetrised */
 static void rows_dot(int nr, int64_t nfeat, const double* A, const double* B, double* out) {
-  for (int q = 0; q < nr * nr; q++) out[q] = 0.0;
-#pragma omp parallel
-  {
-    double* loc = (double*)calloc((size_t)nr * nr, sizeof(double));
-#pragma omp for schedule(static)
-    for (int64_t f = 0; f < nfeat; f++) {
+  /* fixed 4096-feature chunks combined in chunk order: independent of the
+   * thread count */
+  const int64_t CH = 4096;
+  const int64_t nch = (nfeat + CH - 1) / CH;
+  const size_t NN = (size_t)nr * nr;
+  double* part = (double*)calloc((size_t)(nch > 0 ? nch : 1) * NN, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t c = 0; c < nch; c++) {
+    double* loc = part + (size_t)c * NN;
+    const int64_t f1 = (c + 1) * CH < nfeat ? (c + 1) * CH : nfeat;
+    for (int64_t f = c * CH; f < f1; f++) {
       const double* a = A + (size_t)f * nr;
       const double* b = B + (size_t)f * nr;
       for (int j = 0; j < nr; j++)
         for (int i = j; i < nr; i++) loc[i + j * nr] += a[i] * b[j];
     }
-#pragma omp critical
-    for (int q = 0; q < nr * nr; q++) out[q] += loc[q];
-    free(loc);
   }
+  for (size_t q = 0; q < NN; q++) out[q] = 0.0;
+  for (int64_t c = 0; c < nch; c++)
+    for (size_t q = 0; q < NN; q++) out[q] += part[(size_t)c * NN + q];
+  free(part);
   for (int j = 0; j < nr; j++)
     for (int i = 0; i < j; i++) out[i + j * nr] = out[j + i * nr];
 }
