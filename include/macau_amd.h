@@ -15,7 +15,7 @@
  *     of the last error on the calling thread is macau_last_error();
  *     reference exceptions keep their text (e.g. "Cholesky Decomposition
  *     failed!", latentprior.cpp:375; "Less 2 priors are not supported.",
- *     macau.cpp:162);
+ *     macau.cpp:75);
  *   - one host thread per model; a model owns one HIP stream on device 0.
  */
 #ifndef MACAU_AMD_H
@@ -35,23 +35,23 @@ const char* macau_last_error(void);
 const char* macau_version(void);
 
 /* ---- factories: make_macau_fixed / make_macau_adaptive / make_macau_probit
- *      (macau.h:83-85, macau.cpp:294-315; macau.pxd:82-84) ---- */
+ *      (macau.h:83-85, macau.cpp:207-228; macau.pxd:82-84) ---- */
 macau_model* macau_make_fixed(int nmodes, int num_latent, double precision);
 macau_model* macau_make_adaptive(int nmodes, int num_latent, double sn_init, double sn_max);
 /* probit noise is not provided by this library: returns NULL with an error */
 macau_model* macau_make_probit(int nmodes, int num_latent);
 void macau_free(macau_model* m); /* replaces `del macau` (macau.pyx:449) */
 
-/* ---- priors: BPMFPrior(num_latent) (latentprior.h:53-79, macau.pxd:43-47)
+/* ---- priors: BPMFPrior(num_latent) (latentprior.h:53-79, macau.pxd:45-49)
  *      and MacauPrior<SparseDoubleFeat> (latentprior.h:82-125,
- *      macau.pyx:186-192; side information as COO, rows = data mode) ---- */
+ *      macau.pyx:186-192, make_prior macau.pyx:154-192; side information as COO, rows = data mode) ---- */
 int macau_add_bpmf_prior(macau_model* m);
 int macau_add_macau_prior(macau_model* m, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
                           const double* vals /* NULL = binary */, double lambda_beta, double tol);
 
-/* ---- Macau API (macau.h:35-53; macau.pxd:61-76) ---- */
+/* ---- Macau API (macau.h:35-53; macau.pxd:63-84) ---- */
 int macau_set_samples(macau_model* m, int burnin, int nsamples);
-/* idx: nnz x nmodes int32, column-major (toMatrix, sparsetensor.cpp:323-331) */
+/* idx: nnz x nmodes int32, column-major (toMatrix, sparsetensor.cpp:203-211) */
 int macau_set_relation_data(macau_model* m, const int* idx, int nmodes, const double* values, int nnz,
                             const int* dims);
 int macau_set_relation_data_test(macau_model* m, const int* idx, int nmodes, const double* values, int nnz,
@@ -59,11 +59,11 @@ int macau_set_relation_data_test(macau_model* m, const int* idx, int nmodes, con
 int macau_set_verbose(macau_model* m, int verbose);
 int macau_set_save_model(macau_model* m, int save);
 int macau_set_save_prefix(macau_model* m, const char* prefix);
-int macau_run(macau_model* m); /* MacauX::run (macau.cpp:180-235) */
+int macau_run(macau_model* m); /* MacauX::run (macau.cpp:94-148) */
 double macau_get_rmse_test(macau_model* m);
 long long macau_get_test_nonzeros(macau_model* m);
 int macau_get_predictions(macau_model* m, double* out); /* ntest */
-int macau_get_stds(macau_model* m, double* out);        /* ntest (macau.cpp:261-275) */
+int macau_get_stds(macau_model* m, double* out);        /* ntest (macau.cpp:175-188) */
 int macau_get_test_data(macau_model* m, double* out);   /* ntest x (nmodes+1), col-major */
 
 /* ---- extensions (no reference counterpart) ---- */
@@ -71,7 +71,7 @@ int macau_set_seed(macau_model* m, unsigned long long seed);   /* counter-based 
 int macau_set_sampler(macau_model* m, int kind);               /* MACAU_SAMPLER_* */
 int macau_sampler_supported(int kind, int num_latent, int nmodes); /* 1 if available */
 int macau_set_graph(macau_model* m, int on);                   /* replay iterations as a HIP graph */
-int macau_init(macau_model* m);                                /* MacauX::init (macau.cpp:158-173) */
+int macau_init(macau_model* m);                                /* MacauX::init (macau.cpp:72-86) */
 int macau_iterate(macau_model* m, int n);                      /* n Gibbs iterations, asynchronous */
 int macau_synchronize(macau_model* m);                         /* wait; raise sampler failures */
 int macau_get_iteration(macau_model* m);

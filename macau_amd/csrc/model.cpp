@@ -1,6 +1,6 @@
 // model.cpp -- Gibbs driver, data layer, priors and noise models.
 //
-// Control flow follows MacauX::run (macau.cpp:180-235): per iteration,
+// Control flow follows MacauX::run (macau.cpp:94-148): per iteration,
 // sample every mode's latents, then update every prior, then the noise
 // precision, then evaluate.  All state stays on the GPU; one iteration is a
 // fixed sequence of kernel launches on one stream, captured once into a HIP
@@ -140,7 +140,7 @@ void Model::setRelationData(const int* idx, int nm, const double* vals, int nnz_
     nnz = (long long)v.size();
     double s = 0.0;
     for (double x : v) s += x;
-    mean_value = s / (double)nnz;  // Y.sum() / Y.nonZeros() (sparsetensor.cpp:202)
+    mean_value = s / (double)nnz;  // Y.sum() / Y.nonZeros() (sparsetensor.cpp:103)
     // mode 0: rows of Y, column indices ascending (Yt columns)
     std::vector<size_t> order(r.size());
     std::iota(order.begin(), order.end(), 0);
@@ -163,7 +163,7 @@ void Model::setRelationData(const int* idx, int nm, const double* vals, int nnz_
     nnz = nnz_in;
     double s = 0.0;
     for (int e = 0; e < nnz_in; e++) s += vals[e];
-    mean_value = s / (double)nnz_in;  // vals.mean() (sparsetensor.cpp:247)
+    mean_value = s / (double)nnz_in;  // vals.mean() (sparsetensor.cpp:127)
     std::vector<double> v(vals, vals + nnz_in);
     for (int m = 0; m < nm; m++) {
       std::vector<int> key(nnz_in), other((size_t)nnz_in * (nm - 1));
@@ -555,7 +555,7 @@ void Model::run() {
   rmse_test = noise->getEvalMetric();
 }
 
-void Model::printStatus(int i, double elapsedi) {  // macau.cpp:237-259
+void Model::printStatus(int i, double elapsedi) {  // macau.cpp:151-172
   printf("Iter %3d: %s  ", i, noise->getEvalString().c_str());
   printf("U:[");
   for (int p = 0; p < nmodes; p++) {
@@ -582,7 +582,7 @@ std::vector<double> Model::getPredictions() {
   return p;
 }
 
-std::vector<double> Model::getStds() {  // macau.cpp:261-275
+std::vector<double> Model::getStds() {  // macau.cpp:175-188
   std::vector<double> s(ntest);
   if (nsamples <= 1) {
     std::fill(s.begin(), s.end(), NAN);
@@ -606,7 +606,7 @@ std::vector<double> Model::getTestData() {  // sparsetensor.cpp:63-76, 178-191
   return out;
 }
 
-void Model::saveModel(int isample) {  // macau.cpp:277-285
+void Model::saveModel(int isample) {  // macau.cpp:191-198
   const std::string fprefix = save_prefix + "-sample" + std::to_string(isample) + "-";
   for (int i = 0; i < nmodes; i++) {
     std::vector<double> U;
@@ -616,7 +616,7 @@ void Model::saveModel(int isample) {  // macau.cpp:277-285
   }
 }
 
-void Model::saveGlobalParams() {  // macau.cpp:287-292
+void Model::saveGlobalParams() {  // macau.cpp:201-205
   write_csv(save_prefix + "-meanvalue.csv", std::vector<double>{mean_value}, 1, 1);
 }
 

@@ -8,7 +8,7 @@ runs on the GPU through the C ABI of ``libmacau_amd.so``.
 
 Differences from the reference, all additive:
   * ``seed=`` keyword: the reference seeds from the wall clock
-    (macau.cpp:160); ``seed=None`` keeps that behaviour.
+    (macau.cpp:73); ``seed=None`` keeps that behaviour.
   * ``sampler=`` keyword: ``"auto"`` (default), ``"batched"`` (split-BF16
     MFMA Gram, nnz-bucketed) or ``"plain"`` (FP32, one work-group per row).
   * sparse side information with <= 10000 features is solved with the block

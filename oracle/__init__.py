@@ -7,7 +7,7 @@ and the reported CPU baseline.  The product (``macau_amd``) never imports it.
 The numeric core is ``oracle.c`` (built into ``liboracle.so`` by
 ``oracle/Makefile``).  This module adds the data layer restated in numpy:
 
-* ``matrix_rows`` -- ``MatrixData::setTrain`` (sparsetensor.cpp:198-205)
+* ``matrix_rows`` -- ``MatrixData::setTrain`` (sparsetensor.cpp:92-106)
   through ``sparseFromIJV`` (bpmfutils.h:57-65): Eigen ``setFromTriplets``
   sums duplicates and sorts inner indices; mode 0 iterates ``Yt`` columns
   (rows of Y), mode 1 iterates ``Y`` columns (latentprior.cpp:30-46).

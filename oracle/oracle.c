@@ -676,7 +676,7 @@ out:
 }
 
 /* ------------------------------------------------------------------ */
-/* a4 Gibbs driver (MacauX::init / run, macau.cpp:158-235)            */
+/* a4 Gibbs driver (MacauX::init / run, macau.cpp:72-86, 94-148)            */
 /* ------------------------------------------------------------------ */
 int orc_run(orc_run_cfg* c) {
   const int K = c->K, M = c->nmodes;

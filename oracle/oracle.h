@@ -120,7 +120,7 @@ int orc_blockcg_blocked(int nrhs, const orc_csr* A, const orc_csr* At, double re
 void orc_posterior_lambda_beta(int K, int64_t nfeat, const double* beta, const double* Lambda, double nu,
                                double mu, double* shape, double* scale);
 
-/* ---- a4: full Gibbs run (macau.cpp:158-235) ---- */
+/* ---- a4: full Gibbs run (macau.cpp:72-86, 94-148) ---- */
 typedef struct {
   int      nmodes, K, is_tensor;
   int64_t  dims[8];
