@@ -319,6 +319,7 @@ int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr,
     upload(d_oth[t], f, s);
     HIPCHK(hipStreamSynchronize(s));
     fs.f[t] = d_oth[t].as<float>();
+    fs.n[t] = other_n[t];
   }
   std::vector<float> Lf((size_t)KP * KP, 0.f), Lmu(KP, 0.f);
   for (int i = 0; i < KP; i++)
@@ -364,6 +365,7 @@ int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr,
   std::vector<DevBuf> pb;
   if (sampler == SAMPLER_BATCHED) {
     if (!batched_supported(KP, nother)) throw std::runtime_error("batched sampler: unsupported latent size");
+    if (!batched_fits(fs, nother, KP)) throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
     BatchPlan p = build_batch_plan(rp, KP, pb, s);
     launch_sample_batched(a, p, s);
   } else if (sampler == SAMPLER_PLAIN) {
@@ -413,7 +415,7 @@ int macau_cond_normal_wishart(int K, int N, const double* U, const double* mu0, 
   a.K = K;
   a.KP = KP;
   a.partial = dpart.as<double>();
-  a.nblk = nblk;
+  a.nblk = 1;  // totals in row 0
   a.N = N;
   a.kappa = kappa;
   a.mu0 = dmu0.as<double>();

@@ -6,80 +6,193 @@
 //   T_c = X^{-1};  Lambda ~ Wishart(T_c, nu_c) by Bartlett
 //   (mvnormal.cpp:217-266);  mu ~ N(mu_c, (kappa_c Lambda)^{-1})
 //   (MvNormal_prec, mvnormal.cpp:169-184).
-// The sufficient statistics are reduced over columns by many work-groups in
-// FP64; the K x K draw then runs in one work-group.
+// The sufficient statistics are accumulated over columns by many work-groups
+// in FP64 and reduced in place into the first partial row; the K x K draw
+// then runs in one work-group per mode.
 #include "common.h"
 #include "kernels.h"
 
 namespace mgpu {
 
-constexpr int kStatThreads = 256;
+constexpr int kMaxStatBlocks = 512;
 
 int hyper_stats_blocks(int N) {
-  int b = (N + 511) / 512;
+  int b = (N + 127) / 128;
   if (b < 1) b = 1;
-  if (b > 64) b = 64;
+  if (b > kMaxStatBlocks) b = kMaxStatBlocks;
   return b;
 }
 
-// partial[blk] = (sum_n x_n (K), sum_n x_n x_n^T (K x K, col-major))
+template <int KP>
+struct StatShape {
+  static constexpr int TD = KP / 4;                       // 4 x 4 tiles per dimension
+  static constexpr int GT = TD * TD < 64 ? 64 : TD * TD;  // threads per column group
+  static constexpr int G = GT >= 256 ? 1 : 256 / GT;      // column groups per block
+  static constexpr int THREADS = GT * G;
+  static constexpr int TC = 16;                           // columns per LDS tile
+  static constexpr int EPT = TC * KP / GT;                // tile elements loaded per thread
+  static constexpr int TILE = G * 2 * TC * KP;            // doubles, double buffered
+  static constexpr int RED = G > 1 ? G * (KP + KP * KP) : 0;
+  static constexpr int SMEM = TILE > RED ? TILE : RED;
+};
+
+// Sufficient statistics of x_n = U[:, n] - sub[:, n] (sub optional, FP32,
+// same ld) over the block's run of columns, in FP64:
+//   partial[blk] = (sum_n x_n (K), sum_n x_n x_n^T (K x K, col-major)).
+// The block's columns are split between G column groups; each group streams
+// its columns through a double-buffered LDS tile of 16 columns (the loads of
+// tile t + 1 are in flight while tile t is used), and thread (ti, tj) of a
+// group owns rows 4ti..4ti+3 x columns 4tj..4tj+3 of the K x K sum in
+// registers.  The groups' sums are combined in group order.
 template <typename T, int KP>
-__global__ void __launch_bounds__(kStatThreads)
-    k_hyper_stats(const T* __restrict__ U, const float* __restrict__ sub, int ld, int N, int K,
+__global__ void __launch_bounds__(StatShape<KP>::THREADS)
+    k_hyper_stats(const T* __restrict__ U, const float* __restrict__ sub, int ld, int N, int K, int cols,
                   double* __restrict__ partial) {
-  constexpr int ENT = (KP * KP + kStatThreads - 1) / kStatThreads;
-  constexpr int TC = 32;
-  __shared__ double tile[TC][KP + 1];
-  const int t = threadIdx.x;
-  const int chunk = (N + gridDim.x - 1) / gridDim.x;
-  const int c0 = blockIdx.x * chunk;
-  const int c1 = min(N, c0 + chunk);
+  using S = StatShape<KP>;
+  constexpr int TD = S::TD, GT = S::GT, G = S::G, TC = S::TC, EPT = S::EPT;
+  __shared__ double smem[S::SMEM];
+  const int t = threadIdx.x, g = t / GT, gt = t % GT;
+  const int ti = gt % TD, tj = gt / TD;
+  const bool act = gt < TD * TD;
+  const int b0 = blockIdx.x * cols, b1 = min(N, b0 + cols);
+  const int per = (cols + G - 1) / G;
+  const int c0 = min(b1, b0 + g * per), c1 = min(b1, c0 + per);
+  const int ntiles = (per + TC - 1) / TC;
+  double* tile = smem + (size_t)g * 2 * TC * KP;
+  double reg[EPT];
+  auto load = [&](int tt) {
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int f = gt + e * GT, c = f / KP, i = f % KP;
+      const int col = c0 + tt * TC + c;
+      double x = 0.0;
+      if (col < c1 && i < K) {
+        x = (double)U[(size_t)col * ld + i];
+        if (sub) x -= (double)sub[(size_t)col * ld + i];
+      }
+      reg[e] = x;
+    }
+  };
+  double acc[4][4], sum[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    sum[r] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[r][q] = 0.0;
+  }
+  load(0);
+  for (int tt = 0; tt < ntiles; tt++) {
+    double* tb = tile + (tt & 1) * TC * KP;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) tb[gt + e * GT] = reg[e];
+    __syncthreads();
+    if (tt + 1 < ntiles) load(tt + 1);
+    if (act) {
+#pragma unroll 4
+      for (int c = 0; c < TC; c++) {
+        const double* x = tb + c * KP;
+        double xi[4], xj[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          xi[r] = x[4 * ti + r];
+          xj[r] = x[4 * tj + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          sum[r] += xi[r];
+#pragma unroll
+          for (int q = 0; q < 4; q++) acc[r][q] += xi[r] * xj[q];
+        }
+      }
+    }
+  }
   const int KK = K * K;
-  int ii[ENT], jj[ENT];
-  double acc[ENT];
-#pragma unroll
-  for (int s = 0; s < ENT; s++) {
-    const int q = t + s * kStatThreads;
-    ii[s] = q < KK ? q % K : 0;
-    jj[s] = q < KK ? q / K : 0;
-    acc[s] = 0.0;
-  }
-  double sum = 0.0;
-  for (int base = c0; base < c1; base += TC) {
-    const int nc = min(TC, c1 - base);
-    for (int q = t; q < nc * K; q += kStatThreads) {
-      const int c = q / K, i = q % K;
-      double x = (double)U[(size_t)(base + c) * ld + i];
-      if (sub) x -= (double)sub[(size_t)(base + c) * ld + i];
-      tile[c][i] = x;
-    }
-    __syncthreads();
-    for (int c = 0; c < nc; c++) {
-#pragma unroll
-      for (int s = 0; s < ENT; s++) acc[s] += tile[c][ii[s]] * tile[c][jj[s]];
-      if (t < K) sum += tile[c][t];
-    }
-    __syncthreads();
-  }
   double* out = partial + (size_t)blockIdx.x * (K + KK);
-  if (t < K) out[t] = sum;
+  if (G == 1) {
+    if (!act) return;
 #pragma unroll
-  for (int s = 0; s < ENT; s++) {
-    const int q = t + s * kStatThreads;
-    if (q < KK) out[K + q] = acc[s];
+    for (int r = 0; r < 4; r++) {
+      const int i = 4 * ti + r;
+      if (i >= K) continue;
+      if (tj == 0) out[i] = sum[r];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = 4 * tj + q;
+        if (j < K) out[K + i + j * K] = acc[r][q];
+      }
+    }
+    return;
   }
+  __syncthreads();  // tiles are dead: reuse the LDS for the group sums
+  double* red = smem + (size_t)g * (K + KK);
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 4 * ti + r;
+      if (i >= K) continue;
+      if (tj == 0) red[i] = sum[r];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = 4 * tj + q;
+        if (j < K) red[K + i + j * K] = acc[r][q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < K + KK; q += S::THREADS) {
+    double v = 0.0;
+#pragma unroll
+    for (int h = 0; h < G; h++) v += smem[(size_t)h * (K + KK) + q];
+    out[q] = v;
+  }
+}
+
+// partial[0][q] = sum_b partial[b][q] in a fixed order (four interleaved
+// strands of blocks combined in strand order).  Each work-group owns 64
+// entries q and reads every block's copy of them before it writes row 0.
+__global__ void __launch_bounds__(256) k_stats_reduce(double* partial, int nblk, int width) {
+  __shared__ double red[4][64];
+  const int e = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + e;
+  double s = 0.0;
+  if (q < width) {
+    int b = g;
+    for (; b + 28 < nblk; b += 32) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = partial[(size_t)(b + 4 * u) * width + q];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; b < nblk; b += 4) s += partial[(size_t)b * width + q];
+  }
+  red[g][e] = s;
+  __syncthreads();
+  if (g == 0 && q < width) partial[q] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
 }
 
 template <typename T>
 static void launch_stats(const T* U, const float* sub, int ld, int N, int K, double* partial, int nblk,
                          hipStream_t s) {
   const int KP = pad_latent(K);
+  const int cols = (N + nblk - 1) / nblk;
   switch (KP) {
-    case 8: hipLaunchKernelGGL((k_hyper_stats<T, 8>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
-    case 16: hipLaunchKernelGGL((k_hyper_stats<T, 16>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
-    case 32: hipLaunchKernelGGL((k_hyper_stats<T, 32>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
-    case 64: hipLaunchKernelGGL((k_hyper_stats<T, 64>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
-    default: hipLaunchKernelGGL((k_hyper_stats<T, 128>), dim3(nblk), dim3(kStatThreads), 0, s, U, sub, ld, N, K, partial); break;
+#define MGPU_STATS_CASE(P)                                                                                   \
+  case P:                                                                                                    \
+    hipLaunchKernelGGL((k_hyper_stats<T, P>), dim3(nblk), dim3(StatShape<P>::THREADS), 0, s, U, sub, ld, N, K, \
+                       cols, partial);                                                                       \
+    break;
+    MGPU_STATS_CASE(8)
+    MGPU_STATS_CASE(16)
+    MGPU_STATS_CASE(32)
+    MGPU_STATS_CASE(64)
+    MGPU_STATS_CASE(128)
+#undef MGPU_STATS_CASE
+    default: break;
+  }
+  if (nblk > 1) {
+    const int width = K + K * K;
+    hipLaunchKernelGGL(k_stats_reduce, dim3((width + 63) / 64), dim3(256), 0, s, partial, nblk, width);
   }
 }
 

@@ -19,6 +19,7 @@ struct RowsDev {
 
 struct FactorSet {
   const float* f[kMaxOther] = {nullptr, nullptr, nullptr, nullptr};  // KP x N_m
+  int n[kMaxOther] = {0, 0, 0, 0};                                    // N_m
 };
 
 // Work list of the batched sampler (rows sorted by nnz, heavy rows split).
@@ -56,14 +57,19 @@ struct SampleArgs {
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
 void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
 int batched_supported(int KP, int nother);
+// the batched sampler addresses each other mode's factors with 32-bit byte
+// offsets: N_m * KP * 4 must stay below 2^31
+bool batched_fits(const FactorSet& f, int nother, int KP);
 // grids of the batched finishing kernels; their SSE partials are laid out
 // [light grid | heavy grid] when SampleArgs::sse_partial is set
 int batched_light_grid(const BatchPlan& p);
 int batched_heavy_grid(const BatchPlan& p);
 
 // --- hyperprior (FP64) ---
-// partial sums of x = U[:,n] - sub[:,n] (sub may be null): sum (K) and
-// scatter (K x K) per block.  Returns the number of blocks used.
+// sums of x = U[:,n] - sub[:,n] (sub may be null): sum (K) and scatter
+// (K x K, col-major), accumulated per block into partial[nblk][K + K*K] and
+// reduced in place: the totals are in partial[0] when the launch completes.
+// hyper_stats_blocks(N) is the block count to allocate and pass.
 int hyper_stats_blocks(int N);
 void launch_hyper_stats_f32(const float* U, const float* sub, int ld, int N, int K, double* partial, int nblk,
                             hipStream_t s);
@@ -71,7 +77,7 @@ void launch_hyper_stats_f64(const double* U, int ld, int N, int K, double* parti
 
 struct NWArgs {
   int K = 0, KP = 0;
-  const double* partial = nullptr;  // nblk x (K + K*K)
+  const double* partial = nullptr;  // nblk x (K + K*K), summed over the rows
   int nblk = 0;
   double N = 0;                     // number of columns
   double kappa = 2.0;

@@ -234,7 +234,10 @@ FactorSet Model::others_of(int mode) const {
   FactorSet f;
   int t = 0;
   for (int q = 0; q < nmodes; q++)
-    if (q != mode) f.f[t++] = d_U[q].as<float>();
+    if (q != mode) {
+      f.n[t] = dims[q];
+      f.f[t++] = d_U[q].as<float>();
+    }
   return f;
 }
 
@@ -376,6 +379,8 @@ void Model::build_plans() {
   for (int m = 0; m < nmodes; m++) {
     if (!batched_supported(KP, nmodes - 1))
       throw std::runtime_error("batched sampler: unsupported latent size / mode count");
+    if (!batched_fits(others_of(m), nmodes - 1, KP))
+      throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
     plans[m] = build_batch_plan(rows[m].row_ptr, KP, plan_bufs[m], stream);
   }
 }
@@ -663,7 +668,7 @@ static NWArgs nw_args(Model& m, ILatentPrior& p, int mode, double nu) {
   a.K = m.num_latent;
   a.KP = m.KP;
   a.partial = p.stats.as<double>();
-  a.nblk = p.stats_nblk;
+  a.nblk = 1;  // launch_hyper_stats leaves the totals in row 0
   a.N = m.dims[mode];
   a.kappa = 2.0;  // b0 (latentprior.cpp:97)
   a.mu0 = p.mu0.as<double>();
@@ -835,7 +840,7 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   launch_hyper_stats_f64(beta.as<double>(), K, nfeat, K, bbt.as<double>(), bbt_nblk, s);
   NWArgs a = nw_args(m, *this, mode, (double)K + (double)nfeat);  // df + beta.cols()
   a.BBt_partial = bbt.as<double>();
-  a.bbt_nblk = bbt_nblk;
+  a.bbt_nblk = 1;
   a.lambda_beta = lambda_beta_d.as<double>();
   a.Lchol = Lchol.as<double>();
   launch_nw_draw(a, s);
@@ -873,7 +878,7 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
                      Lmu_rows.as<float>(), s);
   // lambda_beta = sample_lambda_beta(beta, Lambda, nu0, mu0)
   launch_hyper_stats_f64(beta.as<double>(), K, nfeat, K, bbt.as<double>(), bbt_nblk, s);
-  launch_lambda_beta(K, (double)nfeat, bbt.as<double>(), bbt_nblk, Lambda.as<double>(), lambda_beta_d.as<double>(),
+  launch_lambda_beta(K, (double)nfeat, bbt.as<double>(), 1, Lambda.as<double>(), lambda_beta_d.as<double>(),
                      m.seed, m.d_iter.as<int>(), mode, s);
 }
 
