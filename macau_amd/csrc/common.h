@@ -133,6 +133,30 @@ __device__ inline double gamma_f64(uint64_t seed, double shape, uint32_t c0, uin
   return g;
 }
 
+// ---- wave-level helpers (work-groups of one wave) ----
+
+// LDS writes of this wave must land before its lanes read each other's
+// entries: wait for the LDS queue (lgkmcnt(0), vmcnt/expcnt untouched).
+__device__ __forceinline__ void lds_wait() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// An empty asm that "changes" v: predicates built from v after it are formed
+// at that point instead of being hoisted and held in scalar registers.
+__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
+
 // Padded latent dimension used for storage and kernel templates.
 inline int pad_latent(int K) {
   if (K <= 8) return 8;

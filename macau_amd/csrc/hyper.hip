@@ -378,6 +378,201 @@ __device__ void nw_draw(const NWArgs& a, double* base) {
   if (t == 0 && !ok && a.fail) atomicMax(a.fail, 1 << 30);
 }
 
+// ------------------------------------------------------------------
+// The same draw for K <= 32 on one wave, in registers.  The problem is
+// padded to 32 with X_pad = diag(X, I) and C_pad = diag(C, I), and J is the
+// 32-reversal; then chol(X_pad^{-1}) = diag(chol(X^{-1}), I) and
+// B_pad = diag(B, I), so the leading K x K blocks are the quantities above
+// and every loop has a compile-time trip count.  Lane r (r = lane % 32; the
+// upper half of the wave mirrors the lower) holds row r of J X_pad J and of
+// its Cholesky factor M, then column r of Z = M^{-T} J C_pad^T and of
+// B = J Z, then row r of Lambda.
+// ------------------------------------------------------------------
+struct NW32Lds {
+  alignas(16) double C[32][34];   // Bartlett factor C_pad (rows), later B (rows)
+  alignas(16) double Mt[32][34];  // M^T (row i = column i of M)
+  alignas(16) double col[2][32];  // Cholesky column broadcast
+  alignas(16) double muc[32];     // mu_c, then mu
+  alignas(16) double z[32];       // normals of the mu draw
+  alignas(16) double invd[32];    // 1 / M[k][k]
+};
+
+__device__ void nw_draw32(const NWArgs& a, NW32Lds& s) {
+  int lane = threadIdx.x & 63;
+  opaque(lane);
+  const int K = a.K;
+  int r = lane & 31;
+  const int it = *a.iter;
+  const double N = a.N;
+  const double kappa_c = a.kappa + N, nu_c = a.nu + N;
+  // 1. Bartlett factor: C_ii = sqrt(chi2(nu_c - i)), C_ij ~ N(0,1) for
+  //    i < j < K (WishartUnit, mvnormal.cpp:217-241), identity beyond K;
+  //    the K(K+1)/2 draws are spread over the wave, column by column.
+  for (int q = lane; q < 32 * 32; q += 64) {
+    const int i = q >> 5, j = q & 31;
+    if (i <= j && j < K) continue;
+    s.C[i][j] = (i == j) ? 1.0 : 0.0;
+  }
+  const int ndraw = K * (K + 1) / 2;
+  for (int t = lane; t < ndraw; t += 64) {
+    int j = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+    while ((j + 1) * (j + 2) / 2 <= t) j++;
+    while (j * (j + 1) / 2 > t) j--;
+    const int i = t - j * (j + 1) / 2;
+    double c;
+    if (i == j)
+      c = sqrt(2.0 * gamma_f64(a.seed, 0.5 * (nu_c - i), (uint32_t)i, (uint32_t)it, make_tag(TAG_WISHART_CHI, a.mode)));
+    else
+      c = normal_f64(a.seed, (uint32_t)i, (uint32_t)j, (uint32_t)it, make_tag(TAG_WISHART_Z, a.mode));
+    s.C[i][j] = c;
+  }
+  if (lane < 32) {
+    s.z[lane] = lane < K ? normal_f64(a.seed, 0u, (uint32_t)lane, (uint32_t)it, make_tag(TAG_NW_MU, a.mode)) : 0.0;
+    // 2. mu_c (mvnormal.cpp:322-323)
+    s.muc[lane] = lane < K ? (a.kappa * a.mu0[lane] + a.partial[lane]) / kappa_c : 0.0;
+  }
+  lds_wait();
+  // 3. row r of Y = J X_pad J, X = T + S + kappa mu0 mu0^T - kappa_c mu_c mu_c^T
+  //    (mvnormal.cpp:324-325; Macau adds lambda_beta beta beta^T to T,
+  //    latentprior.cpp:258-260)
+  const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
+  const int KK = K * K;
+  // the whole matrix is formed by the wave in LDS (s.Mt as staging, entry
+  // (r, c) at Mt[r][c]) with branch-free loads, then each lane takes its row
+  {
+    const double* bbt = a.BBt_partial ? a.BBt_partial + K : a.partial + K;
+    const double lbb = a.BBt_partial ? lb : 0.0;
+    for (int q = lane; q < 32 * 32; q += 64) {
+      const int rr = q >> 5, c = q & 31;
+      const int xi = 31 - rr, xj = 31 - c;
+      const bool in = xi < K && xj < K;
+      const int xic = xi < K ? xi : 0, xjc = xj < K ? xj : 0;
+      const int e = xic + xjc * K;
+      const double x = a.WI[e] + lbb * bbt[e] + a.partial[K + e] + a.kappa * a.mu0[xic] * a.mu0[xjc] -
+                       kappa_c * s.muc[xic] * s.muc[xjc];
+      s.Mt[rr][c] = in ? x : (rr == c ? 1.0 : 0.0);
+    }
+  }
+  (void)KK;
+  lds_wait();
+  double Y[32];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const double2 t2 = *reinterpret_cast<const double2*>(&s.Mt[r][2 * q]);
+    Y[2 * q] = t2.x;
+    Y[2 * q + 1] = t2.y;
+  }
+  // 4. M = chol(Y), row r on lane r (right-looking, column k + 1 published
+  //    ahead of the rest of step k)
+  bool bad = false;
+  s.col[0][r] = Y[0];
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    opaque(r);
+    lds_wait();
+    double col[32];
+#pragma unroll
+    for (int q = k / 2; q < 16; q++) {
+      const double2 t2 = *reinterpret_cast<const double2*>(&s.col[k & 1][2 * q]);
+      col[2 * q] = t2.x;
+      col[2 * q + 1] = t2.y;
+    }
+    const bool badp = !(col[k] > 0.0);
+    bad = bad || badp;
+    const double d = badp ? 1.0 : col[k];
+    const double rsd = 1.0 / sqrt(d);
+    s.invd[k] = rsd;  // 1 / M[k][k] (every lane stores the same value)
+    const bool gt = r > k;
+    const double sc = gt ? Y[k] * (rsd * rsd) : 0.0;
+    if (k + 1 < 32) {
+      Y[k + 1] -= sc * col[k + 1];
+      s.col[(k + 1) & 1][r] = Y[k + 1];
+    }
+#pragma unroll
+    for (int j = k + 2; j < 32; j++) Y[j] -= sc * col[j];
+    Y[k] = (r >= k) ? Y[k] * rsd : 0.0;  // M[r][k]; zero above the diagonal
+  }
+  // M^T to LDS: row i of Mt = column i of M
+#pragma unroll
+  for (int c = 0; c < 32; c++) s.Mt[c][r] = Y[c];
+  lds_wait();
+  // 5. Z = M^{-T} R, R = J C_pad^T (R[i][c] = C_pad[c][31 - i]); lane r
+  //    solves for column r of Z from the bottom row up
+  double z[32];
+#pragma unroll
+  for (int i = 31; i >= 0; i--) {
+    int o = 0;  // opaque zero offset: the row's reads stay in this step
+    opaque(o);
+    double acc = s.C[r][31 - i];
+#pragma unroll
+    for (int q = (i + 1) / 2; q < 16; q++) {
+      const double2 m2 = *reinterpret_cast<const double2*>(&s.Mt[i][2 * q + o]);
+      if (2 * q > i) acc -= m2.x * z[2 * q];
+      if (2 * q + 1 > i) acc -= m2.y * z[2 * q + 1];
+    }
+    z[i] = acc * s.invd[i + o];
+  }
+  // 6. B = J Z: lane r holds column r of B (b[i] = z[31 - i]); rows to LDS
+  lds_wait();
+#pragma unroll
+  for (int i = 0; i < 32; i++) s.C[i][r] = z[31 - i];
+  lds_wait();
+  // 7. mu = mu_c + B^{-T} z / sqrt(kappa_c) (MvNormal_prec, mvnormal.cpp:169-184):
+  //    B^T x = z from the last row up; lane r holds x_r and column r of B
+  {
+    double acc = s.z[r];
+    const double invd = 1.0 / s.C[r][r];
+#pragma unroll
+    for (int j = 31; j >= 0; j--) {
+      opaque(r);
+      const double xj = rdlane(acc * invd, j);
+      acc = (r < j) ? acc - z[31 - j] * xj : (r == j ? xj : acc);
+    }
+    const double mu = s.muc[r] + acc / sqrt(kappa_c);
+    lds_wait();
+    s.muc[r] = mu;
+  }
+  if (lane < 32 && r < K) {
+    a.mu[r] = s.muc[r];
+    if (a.Lchol) {
+#pragma unroll
+      for (int j = 0; j < 32; j++)
+        if (j < K) a.Lchol[j + r * K] = z[31 - j];  // B[j][r]
+    }
+  }
+  lds_wait();
+  // 8. Lambda = B B^T, row r, written out with Lambda mu as it is formed
+  double brow[32];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const double2 t2 = *reinterpret_cast<const double2*>(&s.C[r][2 * q]);
+    brow[2 * q] = t2.x;
+    brow[2 * q + 1] = t2.y;
+  }
+  const int KP = a.KP;
+  const bool wr = lane < 32;
+  double lm = 0.0;
+  for (int j = 0; j < 32; j++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const double2 t2 = *reinterpret_cast<const double2*>(&s.C[j][2 * q]);
+      acc += brow[2 * q] * t2.x + brow[2 * q + 1] * t2.y;
+    }
+    const bool in = r < K && j < K;
+    if (in) lm += acc * s.muc[j];
+    if (wr && in) a.Lambda[r + j * K] = acc;
+    if (wr && r < KP && j < KP) a.Lambda_f[r + j * KP] = in ? (float)acc : (r == j ? 1.f : 0.f);
+  }
+  if (wr && r < KP) a.Lmu_f[r] = (float)lm;
+  if (lane == 0 && bad && a.fail) atomicMax(a.fail, 1 << 30);
+}
+
+__global__ void __launch_bounds__(64) k_nw_draw32(NWBatch b) {
+  __shared__ NW32Lds s;
+  nw_draw32(b.m[blockIdx.x], s);
+}
+
 // one work-group per mode of the batch
 __global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWBatch b) {
   extern __shared__ double dyn[];
@@ -390,6 +585,12 @@ size_t nw_workspace_bytes(int K) { return nw_lds_bytes(K) <= 48 * 1024 ? 0 : nw_
 
 void launch_nw_draw_batch(const NWBatch& b, hipStream_t s) {
   if (b.n <= 0) return;
+  bool small = true;
+  for (int q = 0; q < b.n; q++) small = small && b.m[q].K <= 32 && b.m[q].nblk == 1;
+  if (small) {
+    hipLaunchKernelGGL(k_nw_draw32, dim3(b.n), dim3(64), 0, s, b);
+    return;
+  }
   const bool lds = b.m[0].ws == nullptr;
   hipLaunchKernelGGL(k_nw_draw, dim3(b.n), dim3(kNWThreads), lds ? nw_lds_bytes(b.m[0].K) : 0, s, b);
 }

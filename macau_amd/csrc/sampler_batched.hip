@@ -110,13 +110,6 @@ __device__ __forceinline__ void load_blk(const SampleArgs& a, int e0, int end, i
   }
 }
 
-// LDS writes of this wave must land before its lanes read each other's
-// entries: wait for the LDS queue (lgkmcnt(0), vmcnt/expcnt untouched).
-__device__ __forceinline__ void lds_wait() {
-  __builtin_amdgcn_s_waitcnt(0xC07F);
-  __builtin_amdgcn_wave_barrier();
-}
-
 template <int NO>
 __device__ __forceinline__ void stage_blk(BlkLds<NO>& L, int buf, int lane, const BlkReg<NO>& b) {
 #pragma unroll
@@ -227,25 +220,31 @@ struct GramStream {
 };
 
 struct PairLds {
-  alignas(16) float colb[2][2][32];  // column broadcast, double buffered, per half
-  alignas(16) float bcol[2][2][32];  // RHS broadcast, double buffered, per half
-  alignas(16) float Lf[2][32][36];   // L per half, row i at Lf[half][i]
-  alignas(16) float us[2][32];       // the new samples (SSE)
+  alignas(16) float WT[2][4][32];   // current block columns of -P, per half (column-major)
+  alignas(16) float Wb[2][32];      // RHS, per half
+  alignas(16) float Lf[2][32][36];  // L per half, row i at Lf[half][i]
+  alignas(16) float us[2][32];      // the new samples (SSE)
 };
 
-__device__ __forceinline__ float rdlane(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-// An empty asm that "changes" v: predicates built from v after it are formed
-// at that point instead of being hoisted and held in scalar registers.
-__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
-
 // Finish two rows whose Gram fragments and RHS partials are in accA/accB,
-// bpA/bpB (rowB < 0: no second row).  Lane i of half h owns row i of P for
-// row h.  Returns this lane's SSE share.
-__device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, const f32x16& accA, const f32x16& accB,
-                             float bpA, float bpB, int it, PairLds& s) {
+// bpA/bpB (rowB < 0: no second row).  Returns this lane's SSE share.
+//
+// P = Lambda + alpha G stays in the MFMA accumulator layout (lane l, register
+// r holds entry (8(r/4) + 4(l/32) + r%4, l%32)), negated, for the whole
+// factorisation.  Right-looking Cholesky in 4-column blocks, k0 = 0, 4, .., 28:
+//   1. the lanes holding columns k0..k0+3 publish them to LDS; lane i of half
+//      h (row i of the pair's row h) reads its 4 entries, the 4 x 4 diagonal
+//      block and its RHS, factors the block redundantly (L_D, y_D), forms its
+//      panel row p = A[i][k0..k0+3] L_D^{-T} -- for a lane inside the block
+//      this recurrence yields its row of L_D, zero above the diagonal -- and
+//      eliminates its RHS;
+//   2. the rank-4 update of both matrices is four v_mfma_f32_32x32x2_f32 on
+//      the panel rows of the lanes below the block, whose A/B operands are
+//      formed by v_permlane32_swap (rows of matrix A live in lanes 0-31, of
+//      matrix B in lanes 32-63).
+// L rows go to LDS as they are produced, for the back solve.
+__device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 accA, f32x16 accB, float bpA,
+                             float bpB, int it, PairLds& s) {
   int lane = threadIdx.x & 63;
   opaque(lane);
   const int half = lane >> 5;
@@ -253,81 +252,111 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, const f32x
   const int row = half ? rowB : rowA;
   const bool active = row >= 0;
   const float alpha = (float)(*a.alpha);
-  // fragments -> row i of this half's P (G is symmetric: column i = row i)
-  float P[32];
+  // -P = -(Lambda + alpha G) in fragment layout (Lambda_f is symmetric and
+  // 4 consecutive registers hold 4 consecutive rows of one column)
+  {
+    const float* lc = a.Lambda_f + i * 32 + 4 * half;
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const float send = half ? accA[r] : accB[r];
-    const float recv = __shfl_xor(send, 32, 64);
-    const float own = half ? accB[r] : accA[r];
-    const int j0 = 8 * (r >> 2) + (r & 3);
-    P[j0] = half ? recv : own;
-    P[j0 + 4] = half ? own : recv;
+    for (int m = 0; m < 4; m++) {
+      const float4 l4 = *reinterpret_cast<const float4*>(lc + 8 * m);
+      accA[4 * m] = -fmaf(alpha, accA[4 * m], l4.x);
+      accA[4 * m + 1] = -fmaf(alpha, accA[4 * m + 1], l4.y);
+      accA[4 * m + 2] = -fmaf(alpha, accA[4 * m + 2], l4.z);
+      accA[4 * m + 3] = -fmaf(alpha, accA[4 * m + 3], l4.w);
+      accB[4 * m] = -fmaf(alpha, accB[4 * m], l4.x);
+      accB[4 * m + 1] = -fmaf(alpha, accB[4 * m + 1], l4.y);
+      accB[4 * m + 2] = -fmaf(alpha, accB[4 * m + 2], l4.z);
+      accB[4 * m + 3] = -fmaf(alpha, accB[4 * m + 3], l4.w);
+    }
   }
   const float recvb = __shfl_xor(half ? bpA : bpB, 32, 64);
   const float c = (half ? bpB : bpA) + recvb;
-  const float* Lrow = a.Lambda_f + i * 32;
-#pragma unroll
-  for (int j = 0; j < 32; j++) P[j] = Lrow[j] + alpha * P[j];
   float b = alpha * c;
   if (a.Lmu_rows) {
     if (active) b += a.Lmu_rows[(size_t)row * 32 + i];
   } else {
     b += a.Lmu_f[i];
   }
-  // Right-looking Cholesky with the forward solve fused in, branch-free.
-  // Step k reads column k (= row k, P is symmetric) and the RHS of lane k
-  // from LDS; lanes i > k eliminate (lanes i < k hold only upper-triangle
-  // values that are never read again).  Column k + 1 is updated first and
-  // published for the next step before the rest of the row, so its LDS
-  // round trip overlaps this step's remaining FMAs.
   bool bad = false;
   float inv_diag = 1.f;  // 1 / L[i][i]
-  s.colb[0][half][i] = P[0];
-  s.bcol[0][half][i] = b;
 #pragma unroll
-  for (int k = 0; k < 32; k++) {
+  for (int k0 = 0; k0 < 32; k0 += 4) {
     opaque(i);
+    // 1. publish columns k0..k0+3 of -P (lanes k0..k0+3 of both halves)
+    const int m = k0 >> 3;
+    if ((unsigned)(i - k0) < 4u) {
+#pragma unroll
+      for (int mm = 0; mm < 4; mm++) {
+        if (mm < m) continue;  // rows above the block row are not read
+        *reinterpret_cast<float4*>(&s.WT[0][i - k0][8 * mm + 4 * half]) =
+            make_float4(accA[4 * mm], accA[4 * mm + 1], accA[4 * mm + 2], accA[4 * mm + 3]);
+        *reinterpret_cast<float4*>(&s.WT[1][i - k0][8 * mm + 4 * half]) =
+            make_float4(accB[4 * mm], accB[4 * mm + 1], accB[4 * mm + 2], accB[4 * mm + 3]);
+      }
+    }
+    s.Wb[half][i] = b;
     lds_wait();
-    const float* cb = s.colb[k & 1][half];
-    float col[32];
-#pragma unroll
-    for (int q = k / 4; q < 8; q++) {
-      const float4 t4 = *reinterpret_cast<const float4*>(cb + 4 * q);
-      col[4 * q] = t4.x;
-      col[4 * q + 1] = t4.y;
-      col[4 * q + 2] = t4.z;
-      col[4 * q + 3] = t4.w;
+    const float w0 = -s.WT[half][0][i], w1 = -s.WT[half][1][i], w2 = -s.WT[half][2][i], w3 = -s.WT[half][3][i];
+    const float4 c0 = *reinterpret_cast<const float4*>(&s.WT[half][0][k0]);  // -D[.][0]
+    const float4 c1 = *reinterpret_cast<const float4*>(&s.WT[half][1][k0]);
+    const float4 c2 = *reinterpret_cast<const float4*>(&s.WT[half][2][k0]);
+    const float4 c3 = *reinterpret_cast<const float4*>(&s.WT[half][3][k0]);
+    const float4 bd = *reinterpret_cast<const float4*>(&s.Wb[half][k0]);
+    // L_D = chol(D), D[t][u] = -c_u.(t), lower entries
+    const float a00 = -c0.x;
+    bool bp = !(a00 > 0.f);
+    const float r0 = __builtin_amdgcn_rsqf(bp ? 1.f : a00);
+    const float l10 = -c0.y * r0, l20 = -c0.z * r0, l30 = -c0.w * r0;
+    const float a11 = -c1.y - l10 * l10;
+    bp = bp || !(a11 > 0.f);
+    const float r1 = __builtin_amdgcn_rsqf(a11 > 0.f ? a11 : 1.f);
+    const float l21 = (-c1.z - l20 * l10) * r1, l31 = (-c1.w - l30 * l10) * r1;
+    const float a22 = -c2.z - l20 * l20 - l21 * l21;
+    bp = bp || !(a22 > 0.f);
+    const float r2 = __builtin_amdgcn_rsqf(a22 > 0.f ? a22 : 1.f);
+    const float l32 = (-c2.w - l30 * l20 - l31 * l21) * r2;
+    const float a33 = -c3.w - l30 * l30 - l31 * l31 - l32 * l32;
+    bp = bp || !(a33 > 0.f);
+    const float r3 = __builtin_amdgcn_rsqf(a33 > 0.f ? a33 : 1.f);
+    bad = bad || bp;
+    const float y0 = bd.x * r0;
+    const float y1 = (bd.y - l10 * y0) * r1;
+    const float y2 = (bd.z - l20 * y0 - l21 * y1) * r2;
+    const float y3 = (bd.w - l30 * y0 - l31 * y1 - l32 * y2) * r3;
+    // panel row of this lane
+    const float p0 = w0 * r0;
+    const float p1 = (w1 - p0 * l10) * r1;
+    const float p2 = (w2 - p0 * l20 - p1 * l21) * r2;
+    const float p3 = (w3 - p0 * l30 - p1 * l31 - p2 * l32) * r3;
+    *reinterpret_cast<float4*>(&s.Lf[half][i][k0]) = make_float4(p0, p1, p2, p3);
+    const int t = i - k0;
+    const bool inb = t >= 0 && t < 4, below = t >= 4;
+    const float yt = t == 0 ? y0 : (t == 1 ? y1 : (t == 2 ? y2 : y3));
+    const float rt = t == 0 ? r0 : (t == 1 ? r1 : (t == 2 ? r2 : r3));
+    b = inb ? yt : (below ? b - (p0 * y0 + p1 * y1 + p2 * y2 + p3 * y3) : b);
+    inv_diag = inb ? rt : inv_diag;
+    // 2. -P += p p^T over the rows below the block, both matrices
+    if (k0 + 4 < 32) {
+      const float q0 = below ? p0 : 0.f, q1 = below ? p1 : 0.f, q2 = below ? p2 : 0.f, q3 = below ? p3 : 0.f;
+      // swap(old, src): old's lanes 32-63 <-> src's lanes 0-31, so
+      // .x = (q0 of A rows | q1 of A rows), .y = (q0 of B rows | q1 of B rows)
+      const auto s01 = __builtin_amdgcn_permlane32_swap(__float_as_uint(q0), __float_as_uint(q1), false, false);
+      const auto s23 = __builtin_amdgcn_permlane32_swap(__float_as_uint(q2), __float_as_uint(q3), false, false);
+      const float a01 = __uint_as_float(s01[0]), b01 = __uint_as_float(s01[1]);
+      const float a23 = __uint_as_float(s23[0]), b23 = __uint_as_float(s23[1]);
+      accA = __builtin_amdgcn_mfma_f32_32x32x2f32(a01, a01, accA, 0, 0, 0);
+      accA = __builtin_amdgcn_mfma_f32_32x32x2f32(a23, a23, accA, 0, 0, 0);
+      accB = __builtin_amdgcn_mfma_f32_32x32x2f32(b01, b01, accB, 0, 0, 0);
+      accB = __builtin_amdgcn_mfma_f32_32x32x2f32(b23, b23, accB, 0, 0, 0);
     }
-    const float bk = s.bcol[k & 1][half][k];
-    const bool badp = !(col[k] > 0.f);
-    bad = bad || badp;
-    const float rs = __builtin_amdgcn_rsqf(badp ? 1.f : col[k]);  // 1 / sqrt(d)
-    const bool gt = i > k, eq = i == k;
-    const float sc = gt ? P[k] * (rs * rs) : 0.f;
-    const float bn = eq ? bk * rs : b - sc * bk;
-    if (k + 1 < 32) {
-      P[k + 1] -= sc * col[k + 1];
-      s.colb[(k + 1) & 1][half][i] = P[k + 1];
-      s.bcol[(k + 1) & 1][half][i] = bn;
-    }
-#pragma unroll
-    for (int j = k + 2; j < 32; j++) P[j] -= sc * col[j];
-    P[k] *= rs;  // L[i][k] for i >= k
-    inv_diag = eq ? rs : inv_diag;
-    b = bn;
   }
   if (bad && active && i == 0) atomicMax(a.fail, row + 1);
   if (i < a.K) b += normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
                                make_tag(TAG_LATENT, a.mode));
   const float w = b;  // L^{-1} b + z
   // back solve L^T u = w.  Lane i needs column i of L (L[j][i], j > i),
-  // which is row data of other lanes: rows go to LDS and each lane reads
-  // its column once; the solved values are broadcast with readlane.
-  float* Lr = s.Lf[half][i];
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-    *reinterpret_cast<float4*>(Lr + 4 * q) = make_float4(P[4 * q], P[4 * q + 1], P[4 * q + 2], P[4 * q + 3]);
+  // i.e. entries of other lanes' rows: each lane reads its column from the
+  // rows in LDS once; the solved values are broadcast with readlane.
   lds_wait();
   float Lc[32];
 #pragma unroll
@@ -336,7 +365,8 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, const f32x
   for (int j = 31; j >= 0; j--) {
     opaque(i);
     const float xl = b * inv_diag;  // final value on lane j
-    const float xj = half ? rdlane(xl, 32 + j) : rdlane(xl, j);
+    const float x0 = rdlane(xl, j), x1 = rdlane(xl, 32 + j);
+    const float xj = half ? x1 : x0;
     b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
   }
   if (active) a.U[(size_t)row * 32 + i] = b;
@@ -344,6 +374,7 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, const f32x
   // train SSE share of this lane: (w^2 - u (Lambda u)) / alpha - 2 u c (+ sum r^2)
   s.us[half][i] = b;
   lds_wait();
+  const float* Lrow = a.Lambda_f + i * 32;
   float lu = 0.f;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
