@@ -15,7 +15,8 @@ its duration is measured with HIP events on the model's stream
 uses: algorithmic FLOPs per half-sweep = nnz (K^2 + 3K) + rows (K^3/3 + 4K^2).
 
 cpu_baseline: the FP64 C/OpenMP oracle timed on this host for a bounded
-sample of the same workload (a few full Gibbs iterations of C2).
+sample of the same workload (25 full Gibbs iterations, about 10 s on 16
+cores).
 """
 import argparse
 import ctypes
@@ -128,12 +129,15 @@ def cpu_baseline(train, test, shape, settings, iters, seed, log):
 
 
 def load_traffic(name):
+    """HBM bytes of one iteration's sampler launches, from the FETCH_SIZE /
+    WRITE_SIZE counter runs recorded in profiles/traffic.json (None if absent)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get(name)
+    e = d.get(name)
+    return None if e is None else e.get("bytes_per_iteration")
 
 
 def main():
@@ -145,7 +149,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--sampler", choices=["batched", "plain"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:

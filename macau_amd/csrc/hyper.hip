@@ -442,6 +442,7 @@ __device__ void nw_draw32(const NWArgs& a, NW32Lds& s) {
   {
     const double* bbt = a.BBt_partial ? a.BBt_partial + K : a.partial + K;
     const double lbb = a.BBt_partial ? lb : 0.0;
+#pragma unroll 8
     for (int q = lane; q < 32 * 32; q += 64) {
       const int rr = q >> 5, c = q & 31;
       const int xi = 31 - rr, xj = 31 - c;
