@@ -1,0 +1,204 @@
+"""The Python entry points on the GPU, mirroring the reference's own tests
+(python/macau/test_macau.py).  Every case runs the full Gibbs chain through
+the HIP library; quality bounds are the reference's (tensor RMSE < 0.5 at
+precision 50, test_macau.py:132-207).  Cases the reference runs with probit
+noise or the univariate sampler check that those raise NotImplementedError,
+since this library does not provide them."""
+import itertools
+import os
+
+import numpy as np
+import pandas as pd
+import pytest
+import scipy.sparse
+
+import macau_amd
+from _gpu_helpers import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(m, n, d, seed):
+    return scipy.sparse.random(m, n, density=d, random_state=seed, format="coo")
+
+
+def test_bpmf():  # test_macau.py:9-16
+    announce("api bpmf")
+    Y = _rand(10, 20, 0.2, 1)
+    Y, Ytest = macau_amd.make_train_test(Y, 0.5)
+    res = macau_amd.bpmf(Y, Ytest=Ytest, num_latent=4, verbose=False, burnin=50, nsamples=50,
+                         univariate=False, seed=1)
+    assert Ytest.nnz == res.prediction.shape[0]
+    assert list(res.prediction.columns[0:2]) == ["row", "col"]
+    assert np.all(np.isfinite(res.prediction.y_pred))
+    assert np.all(res.prediction.y_pred_std >= 0)
+
+
+def test_bpmf_numerictest():  # test_macau.py:18-21
+    announce("api bpmf numeric test fraction")
+    X = _rand(15, 10, 0.2, 2)
+    res = macau_amd.bpmf(X, 0.3, num_latent=10, burnin=10, nsamples=15, verbose=False, seed=2)
+    assert res.ntest == res.prediction.shape[0] > 0
+
+
+def test_macau():  # test_macau.py:23-32
+    announce("api macau dense-in-coo side on both modes")
+    rng = np.random.default_rng(3)
+    Y = _rand(10, 20, 0.2, 3)
+    Y, Ytest = macau_amd.make_train_test(Y, 0.5)
+    side1 = scipy.sparse.coo_matrix(rng.random((10, 2)))
+    side2 = scipy.sparse.coo_matrix(rng.random((20, 3)))
+    res = macau_amd.macau(Y, Ytest=Ytest, side=[side1, side2], num_latent=4, verbose=False, burnin=50,
+                          nsamples=50, univariate=False, seed=3)
+    assert Ytest.nnz == res.prediction.shape[0]
+    assert np.isfinite(res.rmse_test)
+
+
+def test_macau_side_bin():  # test_macau.py:34-39
+    announce("api macau binary side")
+    X = _rand(15, 10, 0.2, 4)
+    Xt = _rand(15, 10, 0.1, 5)
+    F = _rand(15, 2, 0.5, 6)
+    F.data[:] = 1
+    res = macau_amd.macau(X, Xt, side=[F, None], num_latent=5, burnin=10, nsamples=5, verbose=False, seed=4)
+    assert res.prediction.shape[0] == Xt.nnz
+
+
+def test_macau_dense():  # test_macau.py:41-45
+    announce("api macau dense side")
+    rng = np.random.default_rng(7)
+    Y = _rand(15, 10, 0.2, 7)
+    Yt = _rand(15, 10, 0.1, 8)
+    F = rng.standard_normal((15, 2))
+    res = macau_amd.macau(Y, Yt, side=[F, None], num_latent=5, burnin=10, nsamples=5, verbose=False, seed=5)
+    assert np.isfinite(res.rmse_test)
+
+
+def test_probit_and_univariate_not_provided():  # test_macau.py:47-73
+    Y = _rand(10, 20, 0.2, 9)
+    with pytest.raises(NotImplementedError):
+        macau_amd.macau(Y, verbose=False, precision="probit", burnin=2, nsamples=2)
+    with pytest.raises(NotImplementedError):
+        macau_amd.bpmf(Y, verbose=False, univariate=True, burnin=2, nsamples=2)
+
+
+def test_bpmf_emptytest():  # test_macau.py:80-82
+    announce("api bpmf empty test set")
+    X = _rand(15, 10, 0.2, 10)
+    r0 = macau_amd.bpmf(X, Ytest=0, num_latent=10, burnin=10, nsamples=15, verbose=False, seed=6)
+    r1 = macau_amd.bpmf(X, Ytest=None, num_latent=10, burnin=10, nsamples=15, verbose=False, seed=6)
+    assert np.isnan(r0.rmse_test) and np.isnan(r1.rmse_test)
+    assert r0.prediction.shape[0] == 0
+
+
+def test_bpmf_tensor():  # test_macau.py:114-130
+    announce("api bpmf tiny tensor")
+    rng = np.random.default_rng(1234)
+    Y = pd.DataFrame({"A": rng.integers(0, 5, 7), "B": rng.integers(0, 4, 7), "C": rng.integers(0, 3, 7),
+                      "value": rng.standard_normal(7)})
+    Ytest = pd.DataFrame({"A": rng.integers(0, 5, 5), "B": rng.integers(0, 4, 5), "C": rng.integers(0, 3, 5),
+                          "value": rng.standard_normal(5)})
+    res = macau_amd.bpmf(Y, Ytest=Ytest, num_latent=4, verbose=False, burnin=50, nsamples=50, univariate=False,
+                         seed=7)
+    assert res.prediction.shape[0] == 5
+    assert list(res.prediction.columns[0:3]) == ["A", "B", "C"]
+
+
+def _cp_frame(dims, seed):
+    rng = np.random.default_rng(seed)
+    F = [rng.standard_normal((d, 2)) for d in dims]
+    idx = np.asarray(list(itertools.product(*[np.arange(d) for d in dims])))
+    cols = ["A", "B", "C"][:len(dims)]
+    df = pd.DataFrame(idx, columns=cols)
+    prod = np.ones((len(idx), 2))
+    for m, f in enumerate(F):
+        prod *= f[idx[:, m]]
+    df["value"] = prod.sum(1)
+    return df, F
+
+
+def test_bpmf_tensor2():  # test_macau.py:132-146
+    announce("api bpmf tensor2 quality")
+    df, _ = _cp_frame((15, 20, 3), 11)
+    Ytrain, Ytest = macau_amd.make_train_test_df(df, 0.2)
+    res = macau_amd.bpmf(Y=Ytrain, Ytest=Ytest, num_latent=4, verbose=False, burnin=20, nsamples=20,
+                         univariate=False, precision=50, seed=8)
+    assert res.rmse_test < 0.5
+
+
+def test_bpmf_tensor3():  # test_macau.py:148-168
+    announce("api bpmf tensor3 quality and matrix form")
+    df, _ = _cp_frame((15, 20, 1), 12)
+    Ytrain, Ytest = macau_amd.make_train_test_df(df, 0.2)
+    res = macau_amd.bpmf(Y=Ytrain, Ytest=Ytest, num_latent=4, verbose=False, burnin=20, nsamples=20,
+                         univariate=False, precision=50, seed=9)
+    assert res.rmse_test < 0.5
+    Ytrain_sp = scipy.sparse.coo_matrix((Ytrain.value, (Ytrain.A, Ytrain.B)))
+    Ytest_sp = scipy.sparse.coo_matrix((Ytest.value, (Ytest.A, Ytest.B)))
+    res_mat = macau_amd.bpmf(Y=Ytrain_sp, Ytest=Ytest_sp, num_latent=4, verbose=False, burnin=20, nsamples=20,
+                             univariate=False, precision=50, seed=9)
+    assert np.isfinite(res_mat.rmse_test)
+
+
+def test_macau_tensor():  # test_macau.py:170-189
+    announce("api macau tensor with side on mode 0")
+    df, F = _cp_frame((30, 4, 2), 13)
+    Ytrain, Ytest = macau_amd.make_train_test_df(df, 0.2)
+    Acoo = scipy.sparse.coo_matrix(F[0])
+    res = macau_amd.macau(Y=Ytrain, Ytest=Ytest, side=[Acoo, None, None], num_latent=4, verbose=False, burnin=20,
+                          nsamples=20, univariate=False, precision=50, seed=10)
+    assert list(res.prediction.columns[0:3]) == ["A", "B", "C"]
+    assert res.rmse_test < 0.5
+
+
+def test_macau_tensor_empty():  # test_macau.py:208-222
+    announce("api macau tensor empty test set")
+    df, _ = _cp_frame((30, 4, 2), 14)
+    r0 = macau_amd.macau(df, Ytest=0, num_latent=2, burnin=5, nsamples=5, precision=1.0, verbose=False, seed=11)
+    r1 = macau_amd.macau(df, Ytest=None, num_latent=2, burnin=5, nsamples=5, precision=1.0, verbose=False, seed=11)
+    assert np.isnan(r0.rmse_test) and np.isnan(r1.rmse_test)
+
+
+@pytest.mark.parametrize("sampler", ["plain", "batched", "auto"])
+def test_samplers_agree(sampler):
+    announce("api sampler=%s" % sampler)
+    rng = np.random.default_rng(15)
+    U, V = rng.standard_normal((300, 3)), rng.standard_normal((200, 3))
+    Y = _rand(300, 200, 0.1, 15)
+    Y.data = (U[Y.row] * V[Y.col]).sum(1) + 0.1 * rng.standard_normal(Y.nnz)
+    np.random.seed(15)  # make_train_test draws its split from numpy's global stream
+    Ytr, Yte = macau_amd.make_train_test(Y, 0.2)
+    res = macau_amd.bpmf(Ytr, Ytest=Yte, num_latent=32, burnin=20, nsamples=20, precision=100, verbose=False,
+                         seed=16, sampler=sampler)
+    plain = macau_amd.bpmf(Ytr, Ytest=Yte, num_latent=32, burnin=20, nsamples=20, precision=100, verbose=False,
+                           seed=16, sampler="plain")
+    assert abs(res.rmse_test / plain.rmse_test - 1.0) < 5e-3
+    assert res.rmse_test < 0.3
+
+
+def test_save_model(tmp_path):
+    announce("api save_prefix model files")
+    prefix = str(tmp_path / "m")
+    X = _rand(20, 15, 0.3, 17)
+    F = _rand(20, 6, 0.5, 18)
+    macau_amd.macau(X, 0.2, side=[F, None], num_latent=3, burnin=2, nsamples=2, verbose=False, save_prefix=prefix,
+                    seed=12)
+    names = sorted(os.listdir(tmp_path))
+    assert "m-meanvalue.csv" in names
+    for s in (1, 2):
+        for u in (1, 2):
+            assert "m-sample%d-U%d-latents.csv" % (s, u) in names
+            assert "m-sample%d-U%d-latentmean.csv" % (s, u) in names
+        assert "m-sample%d-U1-link.csv" % s in names
+    lat = np.loadtxt(tmp_path / "m-sample2-U1-latents.csv", delimiter=",")
+    assert lat.shape == (3, 20)
+
+
+def test_blockcg_python():  # macau.pyx:67-91
+    announce("api blockcg")
+    X = _rand(60, 25, 0.2, 19)
+    X.data[:] = 1.0
+    rng = np.random.default_rng(20)
+    B = np.asfortranarray(rng.standard_normal((3, 25)))
+    out = macau_amd.blockcg(X, B, 0.5, tol=1e-10)
+    assert out.shape == (2,) and out[0] >= 1
