@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
-"""Benchmark: Gibbs iterations/s of the BPMF sampler on one MI355X.
+"""Benchmark: Gibbs iterations/s of the BPMF/Macau sampler on one MI355X.
 
-Workload: the 138,000 x 27,000 recommender-scale BPMF with 20M observed cells
-(18M train, 2M test), K = 32, adaptive noise precision (macau_amd.synthetic.c2).
-A step is one full Gibbs iteration (both half-sweeps, both Normal-Wishart
-draws, the noise update and the test evaluation) with all inputs resident in
-HBM.  The headline value is the batched mixed-precision sampler; the plain
-FP32 sampler is timed beside it on the same data, and the posterior-mean test
-RMSE of both full chains (burnin 50 + 100 samples) is reported.
+Default workload (the headline line): the 138,000 x 27,000 recommender-scale
+BPMF with 20M observed cells (18M train, 2M test), K = 32, adaptive noise
+precision (macau_amd.synthetic.c2).  A step is one full Gibbs iteration (both
+half-sweeps, both Normal-Wishart draws, the noise update and the test
+evaluation) with all inputs resident in HBM.  The headline value is the
+batched mixed-precision sampler; the plain FP32 sampler is timed beside it on
+the same data, and the posterior-mean test RMSE of both full chains
+(burnin 50 + 100 samples) is reported.
 
 The dominant kernel group is the per-row latent sampler (both half-sweeps);
 its duration is measured with HIP events on the model's stream
@@ -17,9 +18,14 @@ uses: algorithmic FLOPs per half-sweep = nnz (K^2 + 3K) + rows (K^3/3 + 4K^2).
 cpu_baseline: the FP64 C/OpenMP oracle timed on this host for a bounded
 sample of the same workload (25 full Gibbs iterations, about 10 s on 16
 cores).
+
+Other workloads (`--workload`, one JSON line each, not the headline):
+  c3  Macau with 100,000-dim sparse binary row features (block CG for the
+      link matrix every iteration); reports the CG iterations per solve and
+      the achieved HBM bandwidth of the row prior update;
+  c4  3-mode tensor 50,000 x 2,000 x 100, 10M cells, K = 64.
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -36,49 +42,86 @@ from macau_amd._lib import check, ptr  # noqa: E402
 METRIC = "Gibbs iters/sec, batched-mixed vs plain FP32, 20M nnz K=32, RMSE≤+0.5%"
 PEAK_TFLOPS = {"plain": 157.3, "batched": 2500.0}  # FP32 vector/MFMA; BF16 dense MFMA
 UNIT_OF = {"plain": "f32", "batched": "bf16x3-f32acc"}
+PEAK_HBM_GBS = 8000.0
 
 
-def build_model(train, test, shape, settings, sampler, seed, graph=True):
+def load_workload(name, seed):
+    """Generic N-mode form: idx [nnz x nmodes], vals, test idx/vals, dims,
+    settings, side (per mode: scipy COO or None), label."""
+    if name == "c2":
+        train, test, dims, settings = synthetic.c2(seed)
+        idx = np.stack([train[0], train[1]], 1)
+        tidx = np.stack([test[0], test[1]], 1)
+        return dict(idx=idx, vals=train[2], tidx=tidx, tvals=test[2], dims=dims, settings=settings,
+                    side=[None, None],
+                    label="BPMF 138000x27000, 20M observed (18M train / 2M test), Zipf-1.1 degrees, K=32, "
+                          "adaptive noise")
+    if name == "c3":
+        (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c3(seed)
+        return dict(idx=idx, vals=vals, tidx=tidx, tvals=tvals, dims=dims, settings=settings, side=side,
+                    label="Macau 15000x350, 60k observed, 100000-dim binary row features (~%d nnz), K=32, "
+                          "lambda_beta 5, CG tol 1e-6" % side[0].nnz)
+    if name == "c4":
+        (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c4(seed)
+        return dict(idx=idx, vals=vals, tidx=tidx, tvals=tvals, dims=dims, settings=settings, side=side,
+                    label="BPMF tensor 50000x2000x100, 10M observed (9M train / 1M test), Zipf-1.2 degrees, "
+                          "K=64, fixed precision 11.1")
+    raise ValueError(name)
+
+
+def build_model(w, sampler_kind, seed, graph=True):
     L = _lib.lib()
-    K = settings["num_latent"]
-    if settings["precision"] == "adaptive":
-        h = L.macau_make_adaptive(2, K, 1.0, 10.0)
+    st = w["settings"]
+    K = st["num_latent"]
+    nm = len(w["dims"])
+    if st["precision"] == "adaptive":
+        h = L.macau_make_adaptive(nm, K, 1.0, 10.0)
     else:
-        h = L.macau_make_fixed(2, K, float(settings["precision"]))
+        h = L.macau_make_fixed(nm, K, float(st["precision"]))
     if not h:
         raise RuntimeError(_lib.last_error())
-    r, c, v = train
-    idx = np.asfortranarray(np.stack([r, c], 1).astype(np.int32))
-    dims = np.array(shape, dtype=np.int32)
-    check(L.macau_set_relation_data(h, ptr(idx), 2, ptr(np.ascontiguousarray(v, dtype=np.float64)), len(v),
-                                    ptr(dims)))
-    rt, ct, vt = test
-    tidx = np.asfortranarray(np.stack([rt, ct], 1).astype(np.int32))
-    check(L.macau_set_relation_data_test(h, ptr(tidx), 2, ptr(np.ascontiguousarray(vt, dtype=np.float64)),
-                                         len(vt), ptr(dims)))
-    check(L.macau_add_bpmf_prior(h))
-    check(L.macau_add_bpmf_prior(h))
-    check(L.macau_set_samples(h, settings["burnin"], settings["nsamples"]))
+    dims = np.array(w["dims"], dtype=np.int32)
+    idx = np.asfortranarray(w["idx"].astype(np.int32))
+    check(L.macau_set_relation_data(h, ptr(idx), nm, ptr(np.ascontiguousarray(w["vals"], dtype=np.float64)),
+                                    idx.shape[0], ptr(dims)))
+    tidx = np.asfortranarray(w["tidx"].astype(np.int32))
+    check(L.macau_set_relation_data_test(h, ptr(tidx), nm, ptr(np.ascontiguousarray(w["tvals"], dtype=np.float64)),
+                                         tidx.shape[0], ptr(dims)))
+    for s in w["side"]:
+        if s is None:
+            check(L.macau_add_bpmf_prior(h))
+        else:
+            r = np.ascontiguousarray(s.row, dtype=np.int32)
+            c = np.ascontiguousarray(s.col, dtype=np.int32)
+            v = np.ascontiguousarray(s.data, dtype=np.float64)
+            check(L.macau_add_macau_prior(h, s.shape[0], s.shape[1], len(v), ptr(r), ptr(c), ptr(v),
+                                          float(st["lambda_beta"]), float(st["tol"])))
+    check(L.macau_set_samples(h, st["burnin"], st["nsamples"]))
     check(L.macau_set_seed(h, seed))
-    check(L.macau_set_sampler(h, sampler))
+    check(L.macau_set_sampler(h, sampler_kind))
     check(L.macau_set_graph(h, int(graph)))
     check(L.macau_set_verbose(h, 0))
     check(L.macau_init(h))
     return h
 
 
-def sampler_flops(train, shape, K):
-    r, c, _ = train
-    nnz = len(r)
-    per_nnz = K * K + 3 * K
+def sampler_flops(w):
+    """Algorithmic FLOPs of the per-row samplers in one Gibbs iteration."""
+    K = w["settings"]["num_latent"]
+    nnz, nm = w["idx"].shape
+    per_nnz = K * K + 3 * K + (nm - 2) * K  # Gram + RHS (+ the Hadamard gather for tensors)
     per_row = K ** 3 / 3.0 + 4 * K * K
-    return 2 * nnz * per_nnz + (shape[0] + shape[1]) * per_row
+    return nm * nnz * per_nnz + sum(w["dims"]) * per_row
 
 
-def time_sampler(name, train, test, shape, settings, steps, warmup, seed, log):
+def kind_of(name):
+    return _lib.SAMPLER_BATCHED if name == "batched" else _lib.SAMPLER_PLAIN_FP32
+
+
+def time_sampler(name, w, steps, warmup, seed, log):
     L = _lib.lib()
-    kind = _lib.SAMPLER_BATCHED if name == "batched" else _lib.SAMPLER_PLAIN_FP32
-    h = build_model(train, test, shape, settings, kind, seed)
+    h = build_model(w, kind_of(name), seed)
+    nm = len(w["dims"])
     try:
         check(L.macau_iterate(h, warmup))
         check(L.macau_synchronize(h))
@@ -87,21 +130,21 @@ def time_sampler(name, train, test, shape, settings, steps, warmup, seed, log):
         check(L.macau_synchronize(h))
         t1 = time.perf_counter()
         nprof = max(3, min(steps, 10))
-        phase = np.zeros(2 + 3)
+        phase = np.zeros(2 * nm + 2)
         check(L.macau_profile(h, nprof, ptr(phase)))
+        cg = [int(L.macau_get_last_cg_iters(h, m)) for m in range(nm)]
         log("%s: %d steps in %.3f s -> %.1f it/s; phases(ms) %s" %
             (name, steps, t1 - t0, steps / (t1 - t0), np.round(phase, 3).tolist()))
     finally:
         L.macau_free(h)
-    return (t1 - t0) / steps, phase
+    return (t1 - t0) / steps, phase, cg
 
 
-def full_chain_rmse(name, train, test, shape, settings, seed):
+def full_chain_rmse(name, w, seed):
     L = _lib.lib()
-    kind = _lib.SAMPLER_BATCHED if name == "batched" else _lib.SAMPLER_PLAIN_FP32
-    h = build_model(train, test, shape, settings, kind, seed)
+    h = build_model(w, kind_of(name), seed)
     try:
-        n = settings["burnin"] + settings["nsamples"]
+        n = w["settings"]["burnin"] + w["settings"]["nsamples"]
         t0 = time.perf_counter()
         check(L.macau_iterate(h, n))
         check(L.macau_synchronize(h))
@@ -111,16 +154,18 @@ def full_chain_rmse(name, train, test, shape, settings, seed):
         L.macau_free(h)
 
 
-def cpu_baseline(train, test, shape, settings, iters, seed, log):
+def cpu_baseline(w, iters, seed, log):
     import oracle as O  # the FP64 oracle, as the CPU baseline
     threads = os.cpu_count() or 1
     O.set_threads(threads)
-    r, c, v = train
-    modes, mean, nnz = O.matrix_rows(r, c, v, shape[0], shape[1])
-    ti, tv = O.matrix_test(test[0], test[1], test[2], shape[0], shape[1])
+    st = w["settings"]
+    r, c = w["idx"][:, 0], w["idx"][:, 1]
+    dims = w["dims"]
+    modes, mean, nnz = O.matrix_rows(r, c, w["vals"], dims[0], dims[1])
+    ti, tv = O.matrix_test(w["tidx"][:, 0], w["tidx"][:, 1], w["tvals"], dims[0], dims[1])
     t0 = time.perf_counter()
-    O.run(modes, mean, nnz, list(shape), settings["num_latent"], ti, tv, adaptive=True,
-          burnin=settings["burnin"], nsamples=settings["nsamples"], seed=seed, max_iters=iters)
+    O.run(modes, mean, nnz, list(dims), st["num_latent"], ti, tv, adaptive=True, burnin=st["burnin"],
+          nsamples=st["nsamples"], seed=seed, max_iters=iters)
     dt = time.perf_counter() - t0
     log("cpu oracle: %d iterations in %.2f s on %d threads" % (iters, dt, threads))
     return {"value": iters / dt, "unit": "iters/s", "cores": O.get_threads(), "kind": "port",
@@ -140,12 +185,25 @@ def load_traffic(name):
     return None if e is None else e.get("bytes_per_iteration")
 
 
+def cg_bytes_per_iteration(w):
+    """Algorithmic HBM bytes of one block-CG iteration (linop.h:279-332) for
+    the row side information: F and F^T read once each by the two SpMMs
+    (int32 index + FP64 value per nonzero), and 13 streaming passes over an
+    nrhs x nfeat FP64 block (KP written, P read by the SpMM gather and the
+    P^T KP product, KP read twice, X and R read and written, R read for
+    R R^T, P read and written by the direction update, reg P read)."""
+    F = w["side"][0]
+    K = w["settings"]["num_latent"]
+    return 2 * F.nnz * 12 + 13 * K * F.shape[1] * 8
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
@@ -161,33 +219,44 @@ def main():
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    train, test, shape, settings = synthetic.c2(args.seed)
-    K = settings["num_latent"]
-    log("generated C2 in %.1f s: %d train, %d test" % (time.perf_counter() - t0, len(train[2]), len(test[2])))
-
-    head = args.sampler or "batched"
+    w = load_workload(args.workload, args.seed)
+    st = w["settings"]
+    K, nm = st["num_latent"], len(w["dims"])
+    log("generated %s in %.1f s: %d train, %d test" % (args.workload, time.perf_counter() - t0, len(w["vals"]),
+                                                        len(w["tvals"])))
+    batched_ok = bool(_lib.lib().macau_sampler_supported(_lib.SAMPLER_BATCHED, K, nm))
+    head = args.sampler or ("batched" if batched_ok else "plain")
     res = {}
     names = [head] + ([] if (args.no_plain or head == "plain") else ["plain"])
     for name in names:
-        sec, phase = time_sampler(name, train, test, shape, settings, args.steps, args.warmup, args.seed, log)
-        res[name] = {"sec": sec, "phase": phase}
+        sec, phase, cg = time_sampler(name, w, args.steps, args.warmup, args.seed, log)
+        res[name] = {"sec": sec, "phase": phase, "cg": cg}
         if not args.no_chain:
-            rmse, ips = full_chain_rmse(name, train, test, shape, settings, args.seed)
+            rmse, ips = full_chain_rmse(name, w, args.seed)
             res[name]["rmse"] = rmse
             log("%s full chain (%d iterations): posterior-mean RMSE %.6f" %
-                (name, settings["burnin"] + settings["nsamples"], rmse))
+                (name, st["burnin"] + st["nsamples"], rmse))
 
-    flops = sampler_flops(train, shape, K)
+    flops = sampler_flops(w)
     ph = res[head]["phase"]
-    sampler_ms = ph[0] + ph[1]
+    sampler_ms = float(sum(ph[:nm]))
     achieved = flops / (sampler_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
-            "frac": achieved / PEAK_TFLOPS[head], "traffic": load_traffic(head),
-            "kernel": "per-row latent sampler, both half-sweeps (%s)" % head,
+            "frac": achieved / PEAK_TFLOPS[head],
+            "traffic": load_traffic(head) if args.workload == "c2" else None,
+            "kernel": "per-row latent sampler, all half-sweeps (%s)" % head,
             "flops_per_iteration": flops, "ms_per_iteration": sampler_ms}
     sec = res[head]["sec"]
+    mode_names = ["rows", "cols"] if nm == 2 else ["mode%d" % m for m in range(nm)]
+    phase_ms = {}
+    for m in range(nm):
+        phase_ms["sample_" + mode_names[m]] = float(ph[m])
+    for m in range(nm):
+        phase_ms["prior_" + mode_names[m]] = float(ph[nm + m])
+    phase_ms["noise"] = float(ph[2 * nm])
+    phase_ms["eval"] = float(ph[2 * nm + 1])
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "c2" else "Gibbs iters/sec (%s)" % args.workload,
         "value": 1.0 / sec,
         "unit": "iters/s",
         "n_gpus": 1,
@@ -199,13 +268,21 @@ def main():
         "vs_baseline": None,
         "dtype": UNIT_OF[head],
         "data": "synthetic",
-        "config": {"workload": "BPMF 138000x27000, 20M observed (18M train / 2M test), Zipf-1.1 degrees, "
-                               "K=32, adaptive noise", "sampler": head, "global_batch": len(train[2]),
+        "config": {"workload": w["label"], "sampler": head, "global_batch": len(w["vals"]),
                    "parallelism": "single-gpu"},
-        "row_updates_per_sec": (shape[0] + shape[1]) / sec,
-        "phase_ms": {"sample_rows": ph[0], "sample_cols": ph[1], "priors": ph[2], "noise": ph[3], "eval": ph[4]},
+        "row_updates_per_sec": sum(w["dims"]) / sec,
+        "phase_ms": phase_ms,
         "roofline": roof,
     }
+    if w["side"][0] is not None:
+        # the row prior update is dominated by the block CG for the link matrix
+        iters = res[head]["cg"][0]
+        cgb = cg_bytes_per_iteration(w) * max(iters, 1)
+        gbs = cgb / (ph[nm] * 1e-3) / 1e9
+        out["block_cg"] = {"iters_per_solve": iters, "prior_rows_ms": float(ph[nm]),
+                           "algorithmic_bytes_per_solve": cgb, "achieved_GBs_lower_bound": gbs,
+                           "frac_of_hbm_peak": gbs / PEAK_HBM_GBS,
+                           "note": "time includes the beta noise draw, Uhat and lambda_beta updates"}
     if "plain" in res and head != "plain":
         out["plain_iters_per_sec"] = 1.0 / res["plain"]["sec"]
         out["speedup_vs_plain"] = res["plain"]["sec"] / sec
@@ -213,8 +290,8 @@ def main():
         out["rmse"] = {k: v.get("rmse") for k, v in res.items()}
         if "plain" in res and head != "plain":
             out["rmse"]["rel_diff_vs_plain"] = res[head]["rmse"] / res["plain"]["rmse"] - 1.0
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(train, test, shape, settings, args.cpu_iters, args.seed, log)
+    if not args.no_cpu and args.workload == "c2":
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_iters, args.seed, log)
     print(json.dumps(out), flush=True)
 
 

@@ -76,8 +76,8 @@ int macau_iterate(macau_model* m, int n);                      /* n Gibbs iterat
 int macau_synchronize(macau_model* m);                         /* wait; raise sampler failures */
 int macau_get_iteration(macau_model* m);
 /* n iterations without graph replay, timed with HIP events on the model's
- * stream; phase_ms[nmodes + 3] receives the mean ms of: sampling of each
- * mode, the prior updates of all modes, the noise update, the evaluation */
+ * stream; phase_ms[2 * nmodes + 2] receives the mean ms of: sampling of each
+ * mode, the prior update of each mode, the noise update, the evaluation */
 int macau_profile(macau_model* m, int n, double* phase_ms);
 double macau_get_rmse_onesample(macau_model* m);
 double macau_get_rmse_avg(macau_model* m);

@@ -453,12 +453,7 @@ int macau_solve_blockcg(int nrhs, int nrow, int ncol, long long nnz, const int* 
     hipStream_t s;
     ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
   } guard{s};
-  upload(F.dF_rowptr, F.F_rowptr, s);
-  upload(F.dF_cols, F.F_cols, s);
-  upload(F.dF_vals, F.F_vals, s);
-  upload(F.dFt_rowptr, F.Ft_rowptr, s);
-  upload(F.dFt_cols, F.Ft_cols, s);
-  upload(F.dFt_vals, F.Ft_vals, s);
+  F.upload_side(nrhs, s);
   DevBuf dB, dX, dreg;
   upload(dB, std::vector<double>(B, B + (size_t)nrhs * ncol), s);
   upload(dreg, std::vector<double>{reg}, s);

@@ -103,6 +103,15 @@ __device__ __forceinline__ double normal_f64(uint64_t seed, uint32_t c0, uint32_
   box_muller_f64(w[2 * p], w[2 * p + 1], z0, z1);
   return (j & 1) ? z1 : z0;
 }
+// normals 4b .. 4b+3 of stream (c0, it, tag): the four values normal_f64
+// returns for j = 4b .. 4b+3, from one Philox block
+__device__ __forceinline__ void normal_f64x4(uint64_t seed, uint32_t c0, uint32_t b, uint32_t it, uint32_t tag,
+                                             double z[4]) {
+  uint32_t w[4];
+  philox_at(seed, c0, b, it, tag, w);
+  box_muller_f64(w[0], w[1], z[0], z[1]);
+  box_muller_f64(w[2], w[3], z[2], z[3]);
+}
 // Gamma(shape, 1): Marsaglia & Tsang, ACM TOMS 26(3), 2000; shape < 1 by
 // the boost Gamma(shape+1) * U^(1/shape), U from block (c0, 0xffffffff).
 __device__ inline double gamma_f64(uint64_t seed, double shape, uint32_t c0, uint32_t it, uint32_t tag) {
@@ -156,6 +165,9 @@ __device__ __forceinline__ double rdlane(double v, int l) {
 // An empty asm that "changes" v: predicates built from v after it are formed
 // at that point instead of being hoisted and held in scalar registers.
 __device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
+// Same, and v "depends" on dep: memory accesses addressed through v wait for
+// dep to be computed (keeps a long unrolled chain from hoisting all its loads).
+__device__ __forceinline__ void opaque_after(int& v, double dep) { asm volatile("" : "+v"(v) : "v"(dep)); }
 
 // Padded latent dimension used for storage and kernel templates.
 inline int pad_latent(int K) {

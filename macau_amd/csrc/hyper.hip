@@ -171,6 +171,10 @@ __global__ void __launch_bounds__(256) k_stats_reduce(double* partial, int nblk,
   if (g == 0 && q < width) partial[q] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
 }
 
+void launch_partial_reduce(double* partial, int nblk, int width, hipStream_t s) {
+  if (nblk > 1) hipLaunchKernelGGL(k_stats_reduce, dim3((width + 63) / 64), dim3(256), 0, s, partial, nblk, width);
+}
+
 template <typename T>
 static void launch_stats(const T* U, const float* sub, int ld, int N, int K, double* partial, int nblk,
                          hipStream_t s) {

@@ -161,19 +161,38 @@ struct CsrDev {
   const int* row_ptr = nullptr;
   const int* cols = nullptr;
   const double* vals = nullptr;  // null = binary
+  // optional nnz-balanced work list (used by launch_spmm when nseg > 0):
+  // segments (row, first nnz, end nnz, slot) of at most kSpmmSegNnz
+  // nonzeros; slot < 0: the segment is its whole row and writes the output,
+  // else it writes partial[slot] and the rows split into several segments
+  // (multi_row, slots multi_slot0[m] .. multi_slot0[m + 1] - 1) are summed
+  // in slot order afterwards
+  int nseg = 0;
+  const int4* seg = nullptr;
+  int nmulti = 0;
+  const int* multi_row = nullptr;
+  const int* multi_slot0 = nullptr;
+  double* partial = nullptr;  // slots x nr
 };
+constexpr int kSpmmSegNnz = 256;
 // out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr;
-// reg is a device scalar, read only when add is non-null
+// reg is a device scalar, read only when add is non-null; nothing is done
+// when skip is non-null and *skip != 0 (a converged block CG)
 void launch_spmm(const CsrDev& A, const double* in, int nr, double* out, const double* reg, const double* add,
-                 hipStream_t s);
+                 hipStream_t s, const double* skip = nullptr);
+// partial[0][q] = sum_b partial[b][q] (fixed order), q < width
+void launch_partial_reduce(double* partial, int nblk, int width, hipStream_t s);
 // copy rows [r0, r0 + nr) of a (K x n, ld K) matrix to (nr x n, ld nr), or back
 void launch_rows_copy(const double* src, int lds, double* dst, int ldd, int r0s, int r0d, int nr, int n, hipStream_t s);
 // sum of squares of n values (partial per block, FP64)
 void launch_sumsq_f32(const float* x, size_t n, double* partial, int nblk, hipStream_t s);
 void launch_sumsq_f64(const double* x, size_t n, double* partial, int nblk, hipStream_t s);
-// partial lower dots: per block nr*nr of sum_f a[:,f] b[:,f]^T
+// sum_f a[:,f] b[:,f]^T (nr x nr, col-major) accumulated per block into
+// partial[nblk][nr * nr] and reduced in place: the total is in partial[0]
+// when the launch completes (nblk = dots_blocks(n))
 int dots_blocks(int n);
-void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s);
+void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
+                 const double* skip = nullptr);
 
 struct CgState {
   int nr = 0, n = 0;
@@ -205,8 +224,9 @@ struct BetaArgs {
 };
 void launch_beta_tmp(const BetaArgs& a, double* tmp, hipStream_t s);
 void launch_beta_noise_add(const BetaArgs& a, double* Fty, hipStream_t s);
+// tmp: K x nrow FP64 scratch
 void launch_uhat_finish(const CsrDev& F, const double* beta, int K, int KP, const double* mu, const double* Lambda,
-                        float* Uhat, float* Lmu_rows, hipStream_t s);
+                        float* Uhat, float* Lmu_rows, double* tmp, hipStream_t s);
 void launch_lambda_beta(int K, double nfeat, const double* bbt_partial, int nblk, const double* Lambda,
                         double* lambda_beta, uint64_t seed, const int* iter, int mode, hipStream_t s);
 

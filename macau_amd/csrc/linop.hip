@@ -13,10 +13,12 @@ namespace mgpu {
 
 constexpr int kSpmmThreads = 256;
 
+// one wave per row (matrices without a work list)
 template <int NP>
 __global__ void __launch_bounds__(kSpmmThreads)
     k_spmm(CsrDev A, const double* __restrict__ in, int nr, double* __restrict__ out, const double* __restrict__ reg,
-           const double* __restrict__ add) {
+           const double* __restrict__ add, const double* skip) {
+  if (skip && *skip != 0.0) return;
   constexpr int S = 64 / NP;
   const int lane = threadIdx.x & 63;
   const int w = (blockIdx.x * kSpmmThreads + threadIdx.x) >> 6;
@@ -40,75 +42,232 @@ __global__ void __launch_bounds__(kSpmmThreads)
   }
 }
 
-void launch_spmm(const CsrDev& A, const double* in, int nr, double* out, const double* reg, const double* add,
-                 hipStream_t s) {
-  if (A.nrow == 0) return;
+// one wave per segment of at most kSpmmSegNnz nonzeros: power-law row
+// lengths (a popular feature appears in most samples) are split so no wave
+// walks a long row alone
+template <int NP>
+__global__ void __launch_bounds__(kSpmmThreads)
+    k_spmm_seg(CsrDev A, const double* __restrict__ in, int nr, double* __restrict__ out,
+               const double* __restrict__ reg, const double* __restrict__ add, const double* skip) {
+  if (skip && *skip != 0.0) return;
+  constexpr int S = 64 / NP;
+  const int lane = threadIdx.x & 63;
+  const int w = (blockIdx.x * kSpmmThreads + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * kSpmmThreads) >> 6;
+  const int comp = lane % NP, sub = lane / NP;
+  for (int sg = w; sg < A.nseg; sg += nw) {
+    const int4 d = A.seg[sg];  // row, first nnz, end nnz, slot
+    double acc = 0.0;
+    if (comp < nr) {
+#pragma unroll 4
+      for (int e = d.y + sub; e < d.z; e += S) {
+        const double v = A.vals ? A.vals[e] : 1.0;
+        acc += v * in[(size_t)A.cols[e] * nr + comp];
+      }
+    }
+#pragma unroll
+    for (int off = NP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+    if (sub == 0 && comp < nr) {
+      if (d.w < 0) {
+        if (add) acc += (*reg) * add[(size_t)d.x * nr + comp];
+        out[(size_t)d.x * nr + comp] = acc;
+      } else {
+        A.partial[(size_t)d.w * nr + comp] = acc;
+      }
+    }
+  }
+}
+
+// rows split into several segments: a work-group per row sums the row's
+// partials over 8 strands of slots (slot s on strand s % 8), the strands
+// combined in strand order
+__global__ void __launch_bounds__(kSpmmThreads)
+    k_spmm_fix(CsrDev A, int nr, double* __restrict__ out, const double* __restrict__ reg,
+               const double* __restrict__ add, const double* skip) {
+  if (skip && *skip != 0.0) return;
+  constexpr int NS = kSpmmThreads / 32;  // 8 strands of 32 components (nr <= 32)
+  __shared__ double red[NS][32];
+  const int comp = threadIdx.x & 31, st = threadIdx.x >> 5;
+  for (int m = blockIdx.x; m < A.nmulti; m += gridDim.x) {
+    const int s0 = A.multi_slot0[m], s1 = A.multi_slot0[m + 1];
+    const int row = A.multi_row[m];
+    for (int c0 = 0; c0 < nr; c0 += 32) {
+      const int c = c0 + comp;
+      double acc = 0.0;
+      if (c < nr)
+        for (int sl = s0 + st; sl < s1; sl += NS) acc += A.partial[(size_t)sl * nr + c];
+      red[st][comp] = acc;
+      __syncthreads();
+      if (st == 0 && c < nr) {
+        double v = 0.0;
+#pragma unroll
+        for (int h = 0; h < NS; h++) v += red[h][comp];
+        if (add) v += (*reg) * add[(size_t)row * nr + c];
+        out[(size_t)row * nr + c] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int NP>
+static void launch_spmm_t(const CsrDev& A, const double* in, int nr, double* out, const double* reg,
+                          const double* add, hipStream_t s, const double* skip) {
+  if (A.nseg > 0) {
+    int grid = (A.nseg + 3) / 4;
+    if (grid > 16384) grid = 16384;
+    hipLaunchKernelGGL((k_spmm_seg<NP>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add, skip);
+    if (A.nmulti > 0) {
+      const int g2 = A.nmulti < 4096 ? A.nmulti : 4096;
+      hipLaunchKernelGGL(k_spmm_fix, dim3(g2), dim3(kSpmmThreads), 0, s, A, nr, out, reg, add, skip);
+    }
+    return;
+  }
   int grid = (A.nrow + 3) / 4;
   if (grid > 4096) grid = 4096;
-  if (nr <= 8) hipLaunchKernelGGL((k_spmm<8>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add);
-  else if (nr <= 16) hipLaunchKernelGGL((k_spmm<16>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add);
-  else if (nr <= 32) hipLaunchKernelGGL((k_spmm<32>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add);
-  else hipLaunchKernelGGL((k_spmm<64>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add);
+  hipLaunchKernelGGL((k_spmm<NP>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add, skip);
+}
+
+void launch_spmm(const CsrDev& A, const double* in, int nr, double* out, const double* reg, const double* add,
+                 hipStream_t s, const double* skip) {
+  if (A.nrow == 0) return;
+  if (nr <= 8) launch_spmm_t<8>(A, in, nr, out, reg, add, s, skip);
+  else if (nr <= 16) launch_spmm_t<16>(A, in, nr, out, reg, add, s, skip);
+  else if (nr <= 32) launch_spmm_t<32>(A, in, nr, out, reg, add, s, skip);
+  else launch_spmm_t<64>(A, in, nr, out, reg, add, s, skip);
 }
 
 // ------------------------------------------------------------------
-// partial lower dot products  sum_f a[:,f] b[:,f]^T  (A_mul_Bt_omp_sym,
-// linop.cpp:320-372), FP64, one partial per block
+// dot products  sum_f a[:,f] b[:,f]^T  (A_mul_Bt_omp_sym / A_mul_At_omp,
+// linop.cpp:271-372), FP64.  Same structure as the hyperprior statistics:
+// column groups stream double-buffered LDS tiles of 8 columns of a and b,
+// thread (ti, tj) of a group owns a 4 x 4 tile of the nr x nr sum; the
+// groups of a block are combined in order, then the block partials are
+// reduced in place (launch_partial_reduce).
 // ------------------------------------------------------------------
-constexpr int kDotThreads = 256;
 constexpr int kMaxNr = 40;
 
 int dots_blocks(int n) {
-  int b = (n + 1023) / 1024;
+  int b = (n + 255) / 256;
   if (b < 1) b = 1;
-  if (b > 256) b = 256;
+  if (b > 512) b = 512;
   return b;
 }
 
-__global__ void __launch_bounds__(kDotThreads)
-    k_dots(const double* __restrict__ a, const double* __restrict__ b, int nr, int n, double* __restrict__ partial,
-           const int* done) {
-  if (done && *done) return;
-  constexpr int ENT = (kMaxNr * kMaxNr + kDotThreads - 1) / kDotThreads;
-  constexpr int TC = 32;
-  __shared__ double ta[TC][kMaxNr + 1];
-  __shared__ double tb[TC][kMaxNr + 1];
-  const int t = threadIdx.x;
-  const int chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int c0 = blockIdx.x * chunk, c1 = min(n, c0 + chunk);
+template <int NRP>
+struct DotShape {
+  static constexpr int TD = NRP / 4;
+  static constexpr int GT = TD * TD;
+  static constexpr int G = GT >= 128 ? 1 : ((256 / GT) < 4 ? (256 / GT) : 4);
+  static constexpr int THREADS = G * GT;
+  static constexpr int TC = 8;
+  static constexpr int TILE = TC * NRP;
+  static constexpr int EPT = (TILE + GT - 1) / GT;
+  static constexpr int RED = G > 1 ? G * NRP * NRP : 0;
+  static constexpr int SMEM = (G * 4 * TILE > RED) ? G * 4 * TILE : RED;
+};
+
+template <int NRP>
+__global__ void __launch_bounds__(DotShape<NRP>::THREADS)
+    k_dots(const double* __restrict__ a, const double* __restrict__ b, int nr, int n, int cols,
+           double* __restrict__ partial, const double* skip) {
+  if (skip && *skip != 0.0) return;
+  using S = DotShape<NRP>;
+  constexpr int TD = S::TD, GT = S::GT, G = S::G, TC = S::TC, TILE = S::TILE, EPT = S::EPT;
+  __shared__ double smem[S::SMEM];
+  const int t = threadIdx.x, g = t / GT, gt = t % GT;
+  const int ti = gt % TD, tj = gt / TD;
+  const int b0 = blockIdx.x * cols, b1 = min(n, b0 + cols);
+  const int per = (cols + G - 1) / G;
+  const int c0 = min(b1, b0 + g * per), c1 = min(b1, c0 + per);
+  const int ntiles = (per + TC - 1) / TC;
+  double* ta = smem + (size_t)g * 4 * TILE;  // [2][TILE] of a, then [2][TILE] of b
+  double ra[EPT], rb[EPT];
+  auto load = [&](int tt) {
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int f = gt + e * GT, c = f / NRP, i = f % NRP;
+      const int col = c0 + tt * TC + c;
+      const bool ok = f < TILE && col < c1 && i < nr;
+      ra[e] = ok ? a[(size_t)col * nr + i] : 0.0;
+      rb[e] = ok ? b[(size_t)col * nr + i] : 0.0;
+    }
+  };
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[r][q] = 0.0;
+  load(0);
+  for (int tt = 0; tt < ntiles; tt++) {
+    double* xa = ta + (tt & 1) * TILE;
+    double* xb = ta + 2 * TILE + (tt & 1) * TILE;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int f = gt + e * GT;
+      if (f < TILE) {
+        xa[f] = ra[e];
+        xb[f] = rb[e];
+      }
+    }
+    __syncthreads();
+    if (tt + 1 < ntiles) load(tt + 1);
+#pragma unroll 2
+    for (int c = 0; c < TC; c++) {
+      double ai[4], bj[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        ai[r] = xa[c * NRP + 4 * ti + r];
+        bj[r] = xb[c * NRP + 4 * tj + r];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[r][q] += ai[r] * bj[q];
+    }
+  }
   const int NN = nr * nr;
-  int ii[ENT], jj[ENT];
-  double acc[ENT];
+  double* out = partial + (size_t)blockIdx.x * NN;
+  if (G == 1) {
 #pragma unroll
-  for (int s = 0; s < ENT; s++) {
-    const int q = t + s * kDotThreads;
-    ii[s] = q < NN ? q % nr : 0;
-    jj[s] = q < NN ? q / nr : 0;
-    acc[s] = 0.0;
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = 4 * ti + r, j = 4 * tj + q;
+        if (i < nr && j < nr) out[i + j * nr] = acc[r][q];
+      }
+    return;
   }
-  for (int base = c0; base < c1; base += TC) {
-    const int nc = min(TC, c1 - base);
-    for (int q = t; q < nc * nr; q += kDotThreads) {
-      const int c = q / nr, i = q % nr;
-      ta[c][i] = a[(size_t)(base + c) * nr + i];
-      tb[c][i] = b[(size_t)(base + c) * nr + i];
-    }
-    __syncthreads();
-    for (int c = 0; c < nc; c++) {
+  __syncthreads();  // tiles are dead: reuse the LDS for the group sums
+  double* red = smem + (size_t)g * NN;
 #pragma unroll
-      for (int s = 0; s < ENT; s++) acc[s] += ta[c][ii[s]] * tb[c][jj[s]];
-    }
-    __syncthreads();
-  }
+  for (int r = 0; r < 4; r++)
 #pragma unroll
-  for (int s = 0; s < ENT; s++) {
-    const int q = t + s * kDotThreads;
-    if (q < NN) partial[(size_t)blockIdx.x * NN + q] = acc[s];
+    for (int q = 0; q < 4; q++) {
+      const int i = 4 * ti + r, j = 4 * tj + q;
+      if (i < nr && j < nr) red[i + j * nr] = acc[r][q];
+    }
+  __syncthreads();
+  for (int q = t; q < NN; q += S::THREADS) {
+    double v = 0.0;
+#pragma unroll
+    for (int h = 0; h < G; h++) v += smem[(size_t)h * NN + q];
+    out[q] = v;
   }
 }
 
-void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_dots, dim3(nblk), dim3(kDotThreads), 0, s, a, b, nr, n, partial, (const int*)nullptr);
+void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
+                 const double* skip) {
+  const int cols = (n + nblk - 1) / nblk;
+#define MGPU_DOTS(P)                                                                                        \
+  hipLaunchKernelGGL((k_dots<P>), dim3(nblk), dim3(DotShape<P>::THREADS), 0, s, a, b, nr, n, cols, partial, \
+                     skip)
+  if (nr <= 8) MGPU_DOTS(8);
+  else if (nr <= 16) MGPU_DOTS(16);
+  else if (nr <= 32) MGPU_DOTS(32);
+  else MGPU_DOTS(40);
+#undef MGPU_DOTS
+  if (nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
 }
 
 // ------------------------------------------------------------------
@@ -131,61 +290,117 @@ struct SmallView {
 };
 
 // reduce partials -> lower-symmetric matrix in LDS
+// The small-step kernels run one wave (64 threads); their nr x nr operands
+// are moved with all loads of a thread issued together.
+constexpr int kSmallPer = (kMaxNr * kMaxNr + 63) / 64;
+
+__device__ __forceinline__ void load_nn(const double* __restrict__ src, double* dst, int NN) {
+  double v[kSmallPer];
+#pragma unroll
+  for (int u = 0; u < kSmallPer; u++) {
+    const int q = threadIdx.x + 64 * u;
+    v[u] = q < NN ? src[q] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kSmallPer; u++) {
+    const int q = threadIdx.x + 64 * u;
+    if (q < NN) dst[q] = v[u];
+  }
+}
+
+// M = the symmetric matrix whose lower triangle is partial[0] (makeSymmetric,
+// linop.cpp:241-243; the dots leave their total in the first partial)
 __device__ void reduce_sym(const double* partial, int nblk, int nr, double* M) {
+  (void)nblk;
   const int NN = nr * nr;
-  for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+  double v[kSmallPer];
+#pragma unroll
+  for (int u = 0; u < kSmallPer; u++) {
+    const int q = threadIdx.x + 64 * u;
     const int i = q % nr, j = q / nr;
-    const int src = (i >= j) ? q : (j + i * nr);  // lower triangle (makeSymmetric)
-    double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += partial[(size_t)b * NN + src];
-    M[q] = s;
+    v[u] = q < NN ? partial[(i >= j) ? q : (j + i * nr)] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kSmallPer; u++) {
+    const int q = threadIdx.x + 64 * u;
+    if (q < NN) M[q] = v[u];
   }
   __syncthreads();
 }
 
 // X = (M^{-1} Q)^T with M factored as Eigen's unblocked LLT does it
 // (left-looking; stops at the first non-positive pivot and solves with the
-// partially factored lower triangle).  The reference's block CG relies on
-// that behaviour once the block Krylov space is exhausted.  All nr x nr
-// operands live in LDS.
-__device__ void small_solve_t(const double* M, const double* Q, double* X, int nr, double* L, int* sstop) {
-  const int t = threadIdx.x;
-  for (int q = t; q < nr * nr; q += blockDim.x) L[q] = M[q];
-  if (t == 0) *sstop = 0;
-  __syncthreads();
-  for (int k = 0; k < nr; k++) {
-    if (t == 0) {
-      double x = L[k + k * nr];
-      for (int j = 0; j < k; j++) x -= L[k + j * nr] * L[k + j * nr];
-      if (x <= 0.0) *sstop = 1;
-      else L[k + k * nr] = sqrt(x);
-    }
-    __syncthreads();
-    if (*sstop) break;
-    const double d = L[k + k * nr];
-    for (int i = k + 1 + t; i < nr; i += blockDim.x) {
-      double s = L[i + k * nr];
-      for (int j = 0; j < k; j++) s -= L[i + j * nr] * L[k + j * nr];
-      L[i + k * nr] = s / d;
-    }
-    __syncthreads();
+// partially factored lower triangle, whose unfactored columns keep M's
+// values).  The reference's block CG relies on that behaviour once the
+// block Krylov space is exhausted.  One wave, problem padded to NRP with an
+// identity block:
+//   * factorisation (left-looking, in LDS): step k forms
+//     s_i = A[i][k] - sum_{j<k} L[i][j] L[k][j] on lane i, the pivot s_k
+//     comes from lane k, and lanes i >= k write column k;
+//   * solves: lane j holds column j of the right-hand side in registers and
+//     runs the forward and back substitution with rows/columns of L read as
+//     LDS broadcasts.
+// M, Q are nr x nr col-major in LDS, X is written col-major (X[j][i] =
+// S[i][j]); Ls is NRP x (NRP + 1) scratch (row-major).
+template <int NRP>
+__device__ __forceinline__ void small_solve_w(const double* M, const double* Q, double* X, int nr, double* Ls) {
+  constexpr int LD = NRP + 1;  // padded row stride: lanes reading their own rows hit distinct banks
+  int lane = threadIdx.x & 63;
+  opaque(lane);
+  const int i = lane < NRP ? lane : NRP - 1;  // lanes >= NRP shadow the last row
+#pragma unroll
+  for (int j = 0; j < NRP; j++) {
+    const bool in = i < nr && j < nr;
+    Ls[i * LD + j] = in ? M[i + j * nr] : (i == j ? 1.0 : 0.0);
   }
-  // column j of S = M^{-1} Q; X[j][i] = S[i][j]
-  for (int j = t; j < nr; j += blockDim.x) {
-    double x[kMaxNr];
-    for (int i = 0; i < nr; i++) {
-      double s = Q[i + j * nr];
-      for (int k = 0; k < i; k++) s -= L[i + k * nr] * x[k];
-      x[i] = s / L[i + i * nr];
-    }
-    for (int i = nr - 1; i >= 0; i--) {
-      double s = x[i];
-      for (int k = i + 1; k < nr; k++) s -= L[k + i * nr] * x[k];
-      x[i] = s / L[i + i * nr];
-    }
-    for (int i = 0; i < nr; i++) X[j + i * nr] = x[i];
+  lds_wait();
+  for (int k = 0; k < NRP; k++) {
+    double s = Ls[i * LD + k];
+#pragma unroll 8
+    for (int j = 0; j < k; j++) s -= Ls[i * LD + j] * Ls[k * LD + j];
+    const double d = __shfl(s, k, 64);
+    if (!(d > 0.0)) break;  // the pivot is the same on every lane
+    const double lkk = sqrt(d);
+    if (i == k) Ls[i * LD + k] = lkk;
+    else if (i > k) Ls[i * LD + k] = s / lkk;
+    lds_wait();
   }
-  __syncthreads();
+  // forward L y = q and back L^T x = y, column j of the right-hand side on lane j
+  const int j = lane < NRP ? lane : NRP - 1;
+  double y[NRP];
+#pragma unroll
+  for (int m = 0; m < NRP; m++) y[m] = (j < nr && m < nr) ? Q[m + j * nr] : 0.0;
+#pragma unroll
+  for (int r = 0; r < NRP; r++) {
+    int o = 0;  // opaque zero offset: the row's reads wait for the previous row
+    opaque_after(o, r > 0 ? y[r - 1] : 0.0);
+    double acc = y[r];
+#pragma unroll
+    for (int m = 0; m < r; m++) acc -= Ls[r * LD + m + o] * y[m];
+    y[r] = acc / Ls[r * LD + r + o];
+  }
+#pragma unroll
+  for (int r = NRP - 1; r >= 0; r--) {
+    int o = 0;
+    opaque_after(o, r < NRP - 1 ? y[r + 1] : 0.0);
+    double acc = y[r];
+#pragma unroll
+    for (int m = r + 1; m < NRP; m++) acc -= Ls[m * LD + r + o] * y[m];
+    y[r] = acc / Ls[r * LD + r + o];
+  }
+  if (lane < nr) {
+#pragma unroll
+    for (int r = 0; r < NRP; r++)
+      if (r < nr) X[lane + r * nr] = y[r];
+  }
+  lds_wait();
+}
+
+__device__ __forceinline__ void small_solve(const double* M, const double* Q, double* X, int nr, double* Ls) {
+  if (nr <= 8) small_solve_w<8>(M, Q, X, nr, Ls);
+  else if (nr <= 16) small_solve_w<16>(M, Q, X, nr, Ls);
+  else if (nr <= 32) small_solve_w<32>(M, Q, X, nr, Ls);
+  else small_solve_w<40>(M, Q, X, nr, Ls);
 }
 
 // norms from diag(B B^T): small.norms; flags reset
@@ -225,16 +440,13 @@ __global__ void k_cg_rtr(const double* partial, int nblk, int nr, double* small)
 __global__ void __launch_bounds__(64) k_cg_step_a(const double* partial, int nblk, int nr, double* small) {
   SmallView v(small, nr);
   if (v.flags[0] != 0.0) return;
-  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kMaxNr * kMaxNr], Xs[kMaxNr * kMaxNr];
-  __shared__ int sstop;
+  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kMaxNr * (kMaxNr + 1)], Xs[kMaxNr * kMaxNr];
   reduce_sym(partial, nblk, nr, M);
-  for (int q = threadIdx.x; q < nr * nr; q += blockDim.x) {
-    v.PtKP[q] = M[q];
-    Q[q] = v.RtR[q];
-  }
+  load_nn(v.RtR, Q, nr * nr);
+  load_nn(M, v.PtKP, nr * nr);
   __syncthreads();
-  small_solve_t(M, Q, Xs, nr, L, &sstop);
-  for (int q = threadIdx.x; q < nr * nr; q += blockDim.x) v.A[q] = Xs[q];
+  small_solve(M, Q, Xs, nr, L);
+  load_nn(Xs, v.A, nr * nr);
 }
 
 template <int MODE>  // 0: X += A P, R -= A KP;  1: P = R + Psi P
@@ -282,8 +494,8 @@ __global__ void __launch_bounds__(64) k_cg_step_b(const double* partial, int nbl
                                                    double tolsq) {
   SmallView v(small, nr);
   if (v.flags[0] != 0.0) return;
-  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kMaxNr * kMaxNr], Xs[kMaxNr * kMaxNr];
-  __shared__ int sstop, conv;
+  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kMaxNr * (kMaxNr + 1)], Xs[kMaxNr * kMaxNr];
+  __shared__ int conv;
   if (threadIdx.x == 0) conv = 1;
   reduce_sym(partial, nblk, nr, Q);  // RtR2
   for (int i = threadIdx.x; i < nr; i += blockDim.x)
@@ -293,13 +505,11 @@ __global__ void __launch_bounds__(64) k_cg_step_b(const double* partial, int nbl
     if (threadIdx.x == 0) v.flags[0] = 1.0;
     return;
   }
-  for (int q = threadIdx.x; q < nr * nr; q += blockDim.x) M[q] = v.RtR[q];
+  load_nn(v.RtR, M, nr * nr);
   __syncthreads();
-  small_solve_t(M, Q, Xs, nr, L, &sstop);
-  for (int q = threadIdx.x; q < nr * nr; q += blockDim.x) {
-    v.Psi[q] = Xs[q];
-    v.RtR[q] = Q[q];  // R R' = R2 R2' (linop.h:329)
-  }
+  small_solve(M, Q, Xs, nr, L);
+  load_nn(Xs, v.Psi, nr * nr);
+  load_nn(Q, v.RtR, nr * nr);  // R R' = R2 R2' (linop.h:329)
   if (threadIdx.x == 0) v.flags[1] += 1.0;
 }
 
@@ -317,19 +527,22 @@ static int grid_for(size_t work, int per) {
   return (int)g;
 }
 
+// the done flag of the CG (SmallView::flags[0]): kernels of iterations after
+// convergence return at once
+static const double* done_flag(const CgState& c) { return c.small + 5 * c.nr * c.nr + c.nr; }
+
 void launch_cg_init(const CgState& c, const double* B, hipStream_t s) {
   launch_dots(B, B, c.nr, c.n, c.partial, c.nblk, s);
-  hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(64), 0, s, c.partial, c.nblk, c.nr, c.small);
+  hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
   hipLaunchKernelGGL(k_cg_setup, dim3(grid_for((size_t)c.nr * c.n, 256)), dim3(256), 0, s, B, c.nr, c.n,
                      (const double*)c.small, c.X, c.R, c.P);
   launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s);
-  hipLaunchKernelGGL(k_cg_rtr, dim3(1), dim3(64), 0, s, c.partial, c.nblk, c.nr, c.small);
+  hipLaunchKernelGGL(k_cg_rtr, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
 }
 
 void launch_cg_step_a(const CgState& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_dots, dim3(c.nblk), dim3(kDotThreads), 0, s, (const double*)c.P, (const double*)c.KPm, c.nr,
-                     c.n, c.partial, (const int*)nullptr);
-  hipLaunchKernelGGL(k_cg_step_a, dim3(1), dim3(64), 0, s, c.partial, c.nblk, c.nr, c.small);
+  launch_dots(c.P, c.KPm, c.nr, c.n, c.partial, c.nblk, s, done_flag(c));
+  hipLaunchKernelGGL(k_cg_step_a, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
 }
 
 void launch_cg_update_xr(const CgState& c, hipStream_t s) {
@@ -338,8 +551,8 @@ void launch_cg_update_xr(const CgState& c, hipStream_t s) {
 }
 
 void launch_cg_step_b(const CgState& c, double tolsq, hipStream_t s) {
-  launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s);
-  hipLaunchKernelGGL(k_cg_step_b, dim3(1), dim3(64), 0, s, c.partial, c.nblk, c.nr, c.small, tolsq);
+  launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s, done_flag(c));
+  hipLaunchKernelGGL(k_cg_step_b, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small, tolsq);
 }
 
 void launch_cg_update_p(const CgState& c, hipStream_t s) {
@@ -399,64 +612,106 @@ __global__ void __launch_bounds__(256) k_beta_draw(BetaArgs a, double* out) {
   }
 }
 
-void launch_beta_tmp(const BetaArgs& a, double* tmp, hipStream_t s) {
-  hipLaunchKernelGGL((k_beta_draw<0>), dim3(grid_for(a.N, 4)), dim3(256), 0, s, a, tmp);
-}
-void launch_beta_noise_add(const BetaArgs& a, double* Fty, hipStream_t s) {
-  hipLaunchKernelGGL((k_beta_draw<1>), dim3(grid_for(a.nfeat, 4)), dim3(256), 0, s, a, Fty);
+// The same draws for K <= 32, one column per thread: the K normals of a
+// column come four per Philox block, and L^T x = z is solved in registers
+// with chol(Lambda) and its reciprocal diagonal read as LDS broadcasts.
+template <int WHICH, int KP>
+__global__ void __launch_bounds__(256) k_beta_draw_t(BetaArgs a, double* out) {
+  __shared__ double Ls[KP * KP], inv[KP];
+  const int K = a.K;
+  for (int q = threadIdx.x; q < KP * KP; q += 256) {
+    const int i = q % KP, j = q / KP;
+    Ls[q] = (i < K && j < K) ? a.Lchol[i + j * K] : 0.0;
+  }
+  for (int q = threadIdx.x; q < KP; q += 256) inv[q] = q < K ? 1.0 / a.Lchol[q + q * K] : 0.0;
+  __syncthreads();
+  const int it = *a.iter;
+  const int ncol = WHICH == 0 ? a.N : a.nfeat;
+  const uint32_t tg = make_tag(WHICH == 0 ? TAG_BETA_U : TAG_BETA_F, a.mode);
+  const double sl = WHICH == 1 ? sqrt(*a.lambda_beta) : 0.0;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < ncol; c += gridDim.x * 256) {
+    double x[KP];
+#pragma unroll
+    for (int b = 0; b < KP / 4; b++) {
+      // one Philox block at a time (the column index is passed through an
+      // opaque dependency on the previous block, which bounds live registers)
+      int cb = c;
+      opaque_after(cb, b > 0 ? x[4 * b - 1] : 0.0);
+      double z[4];
+      normal_f64x4(a.seed, (uint32_t)cb, (uint32_t)b, (uint32_t)it, tg, z);
+#pragma unroll
+      for (int u = 0; u < 4; u++) x[4 * b + u] = 4 * b + u < K ? z[u] : 0.0;
+    }
+    // L^T x = z from the last row up: x_j = (z_j - sum_{i>j} L[i][j] x_i) / L[j][j]
+#pragma unroll
+    for (int j = KP - 1; j >= 0; j--) {
+      int o = 0;
+      opaque_after(o, j < KP - 1 ? x[j + 1] : 0.0);
+      double acc = x[j];
+#pragma unroll
+      for (int i = j + 1; i < KP; i++) acc -= Ls[i + j * KP + o] * x[i];
+      x[j] = acc * inv[j + o];
+    }
+    if (WHICH == 0) {
+      const float* u = a.U + (size_t)c * a.KP;
+#pragma unroll
+      for (int i = 0; i < KP; i++)
+        if (i < K) out[(size_t)c * K + i] = (double)u[i] + x[i] - a.mu[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < KP; i++)
+        if (i < K) out[(size_t)c * K + i] += sl * x[i];
+    }
+  }
 }
 
-// Uhat[:,n] = sum_f F[n,f] beta[:,f] (compute_uhat, linop.h:196-198) and the
-// per-row prior term Lambda (mu + Uhat_n) used by MacauPrior::sample_latents
-// (latentprior.cpp:218-227), wave per row.
+template <int WHICH>
+static void launch_beta_draw(const BetaArgs& a, double* out, int ncol, hipStream_t s) {
+  if (ncol <= 0) return;
+  const int KP = pad_latent(a.K);
+  const int g = grid_for(ncol, 256);
+  switch (KP) {
+    case 8: hipLaunchKernelGGL((k_beta_draw_t<WHICH, 8>), dim3(g), dim3(256), 0, s, a, out); return;
+    case 16: hipLaunchKernelGGL((k_beta_draw_t<WHICH, 16>), dim3(g), dim3(256), 0, s, a, out); return;
+    case 32: hipLaunchKernelGGL((k_beta_draw_t<WHICH, 32>), dim3(g), dim3(256), 0, s, a, out); return;
+    default: hipLaunchKernelGGL((k_beta_draw<WHICH>), dim3(grid_for(ncol, 4)), dim3(256), 0, s, a, out); return;
+  }
+}
+
+void launch_beta_tmp(const BetaArgs& a, double* tmp, hipStream_t s) { launch_beta_draw<0>(a, tmp, a.N, s); }
+void launch_beta_noise_add(const BetaArgs& a, double* Fty, hipStream_t s) {
+  launch_beta_draw<1>(a, Fty, a.nfeat, s);
+}
+
+// Uhat[:,n] = sum_f F[n,f] beta[:,f] (compute_uhat, linop.h:196-198) as an
+// SpMM into FP64 (tmp, K x N), then the per-row prior term
+// Lambda (mu + Uhat_n) used by MacauPrior::sample_latents
+// (latentprior.cpp:218-227), one thread per output entry.
 __global__ void __launch_bounds__(256)
-    k_uhat_finish(CsrDev F, const double* __restrict__ beta, int K, int KP, const double* __restrict__ mu,
-                  const double* __restrict__ Lambda, float* __restrict__ Uhat, float* __restrict__ Lmu_rows) {
-  __shared__ double us[4][128];
-  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
-  const int w = (blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int nw = (gridDim.x * 256) >> 6;
-  for (int r = w; r < F.nrow; r += nw) {
-    double acc[2] = {0.0, 0.0};
-    const int beg = F.row_ptr[r], end = F.row_ptr[r + 1];
-    for (int e = beg; e < end; e++) {
-      const double v = F.vals ? F.vals[e] : 1.0;
-      const double* bcol = beta + (size_t)F.cols[e] * K;
-      for (int s = 0; s < 2; s++) {
-        const int i = lane + 64 * s;
-        if (i < K) acc[s] += v * bcol[i];
-      }
+    k_uhat_post(int N, int K, int KP, const double* __restrict__ uh, const double* __restrict__ mu,
+                const double* __restrict__ Lambda, float* __restrict__ Uhat, float* __restrict__ Lmu_rows) {
+  const size_t tot = (size_t)N * KP;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < tot; q += (size_t)gridDim.x * 256) {
+    const int r = (int)(q / KP), i = (int)(q % KP);
+    float u = 0.f, lm = 0.f;
+    if (i < K) {
+      const double* col = uh + (size_t)r * K;
+      double t = 0.0;
+      for (int j = 0; j < K; j++) t += Lambda[i + j * K] * (mu[j] + col[j]);
+      u = (float)col[i];
+      lm = (float)t;
     }
-    for (int s = 0; s < 2; s++) {
-      const int i = lane + 64 * s;
-      if (i < K) us[wl][i] = mu[i] + acc[s];
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (int s = 0; s < 2; s++) {
-      const int i = lane + 64 * s;
-      if (i < KP) {
-        float uh = 0.f, lm = 0.f;
-        if (i < K) {
-          double t = 0.0;
-          for (int j = 0; j < K; j++) t += Lambda[i + j * K] * us[wl][j];
-          uh = (float)acc[s];
-          lm = (float)t;
-        }
-        Uhat[(size_t)r * KP + i] = uh;
-        Lmu_rows[(size_t)r * KP + i] = lm;
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    Uhat[q] = u;
+    Lmu_rows[q] = lm;
   }
 }
 
 void launch_uhat_finish(const CsrDev& F, const double* beta, int K, int KP, const double* mu, const double* Lambda,
-                        float* Uhat, float* Lmu_rows, hipStream_t s) {
+                        float* Uhat, float* Lmu_rows, double* tmp, hipStream_t s) {
   if (F.nrow == 0) return;
-  hipLaunchKernelGGL(k_uhat_finish, dim3(grid_for(F.nrow, 4)), dim3(256), 0, s, F, beta, K, KP, mu, Lambda, Uhat,
-                     Lmu_rows);
+  launch_spmm(F, beta, K, tmp, nullptr, nullptr, s);
+  hipLaunchKernelGGL(k_uhat_post, dim3(grid_for((size_t)F.nrow * KP, 256)), dim3(256), 0, s, F.nrow, K, KP,
+                     (const double*)tmp, mu, Lambda, Uhat, Lmu_rows);
 }
 
 // lambda_beta ~ Gamma(b, c) with posterior_lambda_beta (latentprior.cpp:317-331),

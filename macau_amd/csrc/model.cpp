@@ -447,7 +447,7 @@ void Model::iterate(int n) {
 
 std::vector<double> Model::profile(int n) {
   if (!initialized) throw std::runtime_error("model not initialized");
-  const int nph = nmodes + 3;
+  const int nph = 2 * nmodes + 2;
   std::vector<hipEvent_t> ev((size_t)n * (nph + 1));
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (int i = 0; i < n; i++) {
@@ -458,8 +458,10 @@ std::vector<double> Model::profile(int n) {
       priors[m]->sample_latents(*this, m);
       HIPCHK(hipEventRecord(E[q++], stream));
     }
-    update_priors();
-    HIPCHK(hipEventRecord(E[q++], stream));
+    for (int m = 0; m < nmodes; m++) {
+      priors[m]->update_prior(*this, m);
+      HIPCHK(hipEventRecord(E[q++], stream));
+    }
     noise->update(*this);
     HIPCHK(hipEventRecord(E[q++], stream));
     noise->evalModel(*this);
@@ -727,7 +729,7 @@ MacauPrior::MacauPrior(int nr, int nc, const std::vector<int>& rows, const std::
 }
 
 CsrDev MacauPrior::F() const {
-  CsrDev c;
+  CsrDev c = F_plan;
   c.nrow = nrow;
   c.ncol = nfeat;
   c.row_ptr = dF_rowptr.as<int>();
@@ -736,7 +738,7 @@ CsrDev MacauPrior::F() const {
   return c;
 }
 CsrDev MacauPrior::Ft() const {
-  CsrDev c;
+  CsrDev c = Ft_plan;
   c.nrow = nfeat;
   c.ncol = nrow;
   c.row_ptr = dFt_rowptr.as<int>();
@@ -745,17 +747,57 @@ CsrDev MacauPrior::Ft() const {
   return c;
 }
 
+// nnz-balanced SpMM work list of a CSR matrix (see CsrDev): rows longer
+// than kSpmmSegNnz are cut into segments whose partial sums are added in
+// segment order; those segments come first so the long work starts early
+static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBuf>& store, hipStream_t s) {
+  const int nrow = (int)rp.size() - 1;
+  std::vector<int4> seg;
+  std::vector<int> mrow, mslot0;
+  int slots = 0;
+  for (int r = 0; r < nrow; r++) {
+    if (rp[r + 1] - rp[r] <= kSpmmSegNnz) continue;
+    mrow.push_back(r);
+    mslot0.push_back(slots);
+    for (int b = rp[r]; b < rp[r + 1]; b += kSpmmSegNnz)
+      seg.push_back(make_int4(r, b, std::min(b + kSpmmSegNnz, rp[r + 1]), slots++));
+  }
+  mslot0.push_back(slots);
+  for (int r = 0; r < nrow; r++)
+    if (rp[r + 1] - rp[r] <= kSpmmSegNnz) seg.push_back(make_int4(r, rp[r], rp[r + 1], -1));
+  store.clear();
+  store.resize(4);
+  upload(store[0], seg, s);
+  upload(store[1], mrow.empty() ? std::vector<int>{0} : mrow, s);
+  upload(store[2], mslot0, s);
+  store[3].alloc((size_t)std::max(slots, 1) * nrmax * sizeof(double));
+  CsrDev c;
+  c.nseg = (int)seg.size();
+  c.seg = store[0].as<int4>();
+  c.nmulti = (int)mrow.size();
+  c.multi_row = store[1].as<int>();
+  c.multi_slot0 = store[2].as<int>();
+  c.partial = store[3].as<double>();
+  return c;
+}
+
+void MacauPrior::upload_side(int nrmax, hipStream_t s) {
+  upload(dF_rowptr, F_rowptr, s);
+  upload(dF_cols, F_cols, s);
+  upload(dF_vals, F_vals, s);
+  upload(dFt_rowptr, Ft_rowptr, s);
+  upload(dFt_cols, Ft_cols, s);
+  upload(dFt_vals, Ft_vals, s);
+  F_plan = spmm_plan(F_rowptr, nrmax, F_plan_bufs, s);
+  Ft_plan = spmm_plan(Ft_rowptr, nrmax, Ft_plan_bufs, s);
+}
+
 void MacauPrior::init(Model& m, int mode) {
   ILatentPrior::init(m, mode);
   if (nrow != m.dims[mode])
     throw std::runtime_error("side information rows must equal the size of its data mode");
   const int K = m.num_latent, KP = m.KP, N = nrow;
-  upload(dF_rowptr, F_rowptr, m.stream);
-  upload(dF_cols, F_cols, m.stream);
-  upload(dF_vals, F_vals, m.stream);
-  upload(dFt_rowptr, Ft_rowptr, m.stream);
-  upload(dFt_cols, Ft_cols, m.stream);
-  upload(dFt_vals, Ft_vals, m.stream);
+  upload_side(std::max(K, 40), m.stream);
   const size_t kf = (size_t)K * nfeat;
   beta.alloc(kf * sizeof(double));
   HIPCHK(hipMemsetAsync(beta.as<void>(), 0, beta.bytes(), m.stream));
@@ -815,8 +857,8 @@ int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const d
   // convergence return immediately (device-side done flag)
   for (int it = 0; it < 100000; it += 8) {
     for (int q = 0; q < 8; q++) {
-      launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, s);
-      launch_spmm(Ft, c.inner, nr, c.KPm, reg, c.P, s);
+      launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, s, dflags);
+      launch_spmm(Ft, c.inner, nr, c.KPm, reg, c.P, s, dflags);
       launch_cg_step_a(c, s);
       launch_cg_update_xr(c, s);
       launch_cg_step_b(c, tolsq, s);
@@ -875,7 +917,7 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   }
   // compute_uhat and the per-row prior term for the next half-sweep
   launch_uhat_finish(F(), beta.as<double>(), K, KP, mu.as<double>(), Lambda.as<double>(), Uhat.as<float>(),
-                     Lmu_rows.as<float>(), s);
+                     Lmu_rows.as<float>(), tmp.as<double>(), s);
   // lambda_beta = sample_lambda_beta(beta, Lambda, nu0, mu0)
   launch_hyper_stats_f64(beta.as<double>(), K, nfeat, K, bbt.as<double>(), bbt_nblk, s);
   launch_lambda_beta(K, (double)nfeat, bbt.as<double>(), 1, Lambda.as<double>(), lambda_beta_d.as<double>(),

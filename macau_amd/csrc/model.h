@@ -118,6 +118,10 @@ class MacauPrior : public ILatentPrior {
   int last_cg_iters = 0;
   CsrDev F() const;
   CsrDev Ft() const;
+  // uploads F and F^T and builds their SpMM work lists (nrmax right-hand sides)
+  void upload_side(int nrmax, hipStream_t s);
+  CsrDev F_plan, Ft_plan;
+  std::vector<DevBuf> F_plan_bufs, Ft_plan_bufs;
 };
 
 // noisemodels.h:13-29

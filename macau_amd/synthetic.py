@@ -15,6 +15,23 @@ Each generator returns numpy arrays only (no GPU work):
   rank-32 truth with N(0,1) factors, standardised over the observed cells to
   mean 3.5 and sd 1.0, rounded to 0.5 steps and clipped to [0.5, 5]; 10% held
   out; K = 32, adaptive noise precision, burnin 50, nsamples 100.
+* ``c3`` -- Macau with sparse binary row side information: 15,000 x 350
+  with 60,000 observed cells (column degrees rank-frequency power law,
+  exponent 1.3; rows uniform), values from a rank-32 truth rescaled to mean 6,
+  sd 1.5 plus N(0, 0.6^2) noise; row features 15,000 x 100,000 binary with
+  ~100 distinct features per row drawn with probability ~ rank^-1.5; 10% held
+  out; K = 32, lambda_beta 5, CG tolerance 1e-6, fixed precision 5.0 (noise
+  model and chain length are not fixed by the configuration), burnin 20,
+  nsamples 50.
+* ``c4`` -- 3-mode tensor 50,000 x 2,000 x 100 with 10,000,000 observed
+  cells, each mode's index drawn from a rank-frequency power law with exponent
+  1.2; values from a CP rank-64 truth with factor entries of variance 1/8
+  (= 1/sqrt(64)) plus N(0, 0.3^2) noise; 10% held out; K = 64, fixed
+  precision 1/0.09, burnin 30, nsamples 50.
+
+``c3`` and ``c4`` return the generic form used for N-mode data:
+  (train_idx [nnz x nmodes] int32, train_vals), (test_idx, test_vals), dims,
+  settings, side (one scipy COO matrix or None per mode).
 """
 import numpy as np
 
@@ -107,3 +124,72 @@ def c2(seed=0, nrow=138000, ncol=27000, nnz=20_000_000, K=32):
     train, test = _split(rng, r, c, y, 0.1)
     settings = dict(num_latent=K, precision="adaptive", burnin=50, nsamples=100)
     return train, test, (nrow, ncol), settings
+
+
+def c3(seed=0, nrow=15000, ncol=350, ncell=60000, nfeat=100000, feat_per_row=100, K=32):
+    import scipy.sparse
+    rng = np.random.default_rng(seed)
+    deg_c = zipf_degrees(ncol, ncell, 1.3, 1, nrow).astype(np.float64)
+    keys = np.zeros(0, dtype=np.int64)
+    cdf = np.cumsum(deg_c / deg_c.sum())
+    cdf[-1] = 1.0
+    while len(keys) < ncell:
+        m = 2 * (ncell - len(keys))
+        cols = np.searchsorted(cdf, rng.random(m))
+        rows = rng.integers(0, nrow, m)
+        keys = np.unique(np.concatenate([keys, rows.astype(np.int64) * ncol + cols]))
+    keys = np.sort(rng.choice(keys, ncell, replace=False))
+    r, c = (keys // ncol).astype(np.int32), (keys % ncol).astype(np.int32)
+    U, V = rng.standard_normal((nrow, K)), rng.standard_normal((ncol, K))
+    y = np.einsum("ij,ij->i", U[r], V[c])
+    y = (y - y.mean()) / y.std() * 1.5 + 6.0 + 0.6 * rng.standard_normal(ncell)
+    # binary features: ~feat_per_row distinct ids per row, P(id = k) ~ (k+1)^-1.5
+    w = np.arange(1, nfeat + 1, dtype=np.float64) ** -1.5
+    fcdf = np.cumsum(w / w.sum())
+    fcdf[-1] = 1.0
+    draws = np.searchsorted(fcdf, rng.random((nrow, 12 * feat_per_row))).astype(np.int64)
+    fk = np.unique(np.arange(nrow, dtype=np.int64)[:, None] * nfeat + draws)
+    fr, fc = (fk // nfeat).astype(np.int32), (fk % nfeat).astype(np.int32)
+    # keep at most feat_per_row per row, in id order
+    first = np.searchsorted(fr, np.arange(nrow))
+    keep = (np.arange(len(fr)) - first[fr]) < feat_per_row
+    F = scipy.sparse.coo_matrix((np.ones(keep.sum()), (fr[keep], fc[keep])), shape=(nrow, nfeat))
+    idx = np.stack([r, c], 1)
+    perm = rng.permutation(ncell)
+    nte = ncell // 10
+    te, tr = perm[:nte], perm[nte:]
+    settings = dict(num_latent=K, precision=5.0, burnin=20, nsamples=50, lambda_beta=5.0, tol=1e-6)
+    return (idx[tr], y[tr]), (idx[te], y[te]), (nrow, ncol), settings, [F, None]
+
+
+def c4(seed=0, dims=(50000, 2000, 100), ncell=10_000_000, K=64, R=64):
+    rng = np.random.default_rng(seed)
+    cdfs = []
+    for d in dims:
+        w = zipf_degrees(d, ncell, 1.2, 1, ncell).astype(np.float64)[rng.permutation(d)]
+        cdf = np.cumsum(w / w.sum())
+        cdf[-1] = 1.0
+        cdfs.append(cdf)
+    mult = np.array([dims[1] * dims[2], dims[2], 1], dtype=np.int64)
+    keys = np.zeros(0, dtype=np.int64)
+    while len(keys) < ncell:
+        m = int(1.1 * (ncell - len(keys))) + 1000
+        k = np.zeros(m, dtype=np.int64)
+        for q in range(3):
+            k += np.searchsorted(cdfs[q], rng.random(m)).astype(np.int64) * mult[q]
+        keys = np.unique(np.concatenate([keys, k]))
+    keys = np.sort(rng.choice(keys, ncell, replace=False))
+    idx = np.stack([keys // mult[0], (keys // mult[1]) % dims[1], keys % dims[2]], 1).astype(np.int32)
+    sd = 64 ** -0.25  # entries of variance 1/sqrt(64)
+    Fs = [(rng.standard_normal((d, R)) * sd).astype(np.float32) for d in dims]
+    y = np.empty(ncell, dtype=np.float64)
+    step = 1 << 20
+    for s in range(0, ncell, step):
+        i = idx[s:s + step]
+        y[s:s + step] = (Fs[0][i[:, 0]] * Fs[1][i[:, 1]] * Fs[2][i[:, 2]]).sum(1)
+    y += 0.3 * rng.standard_normal(ncell)
+    perm = rng.permutation(ncell)
+    nte = ncell // 10
+    te, tr = perm[:nte], perm[nte:]
+    settings = dict(num_latent=K, precision=1.0 / 0.09, burnin=30, nsamples=50)
+    return (idx[tr], y[tr]), (idx[te], y[te]), tuple(dims), settings, [None, None, None]
