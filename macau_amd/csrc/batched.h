@@ -1,0 +1,65 @@
+// batched.h -- helpers shared by the batched (split-BF16 MFMA) samplers.
+#pragma once
+#include "common.h"
+#include "kernels.h"
+
+namespace mgpu {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// one lane's nonzero of a staged block: indices into the other modes and
+// the residual
+template <int NO>
+struct BlkReg {
+  int ix[NO];
+  float r;
+};
+
+template <int NO>
+struct Factors {
+  __amdgpu_buffer_rsrc_t rs[NO];
+};
+
+// buffer descriptors of the other modes' factors (CS floats per column);
+// word 3 = 0x00020000 selects 32-bit data, raw (unswizzled) addressing
+template <int NO, int CS>
+__device__ __forceinline__ Factors<NO> make_factors(const FactorSet& f) {
+  Factors<NO> r;
+#pragma unroll
+  for (int t = 0; t < NO; t++)
+    r.rs[t] = __builtin_amdgcn_make_buffer_rsrc((void*)f.f[t], (short)0, f.n[t] * CS * (int)sizeof(float),
+                                                0x00020000);
+  return r;
+}
+
+// component comp of factor column ix; ix = -1 gives an offset past
+// num_records and the buffer load returns 0
+template <int CS>
+__device__ __forceinline__ float ld_factor(__amdgpu_buffer_rsrc_t rs, int ix, int comp) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, ix * (CS * 4) + comp * 4, 0, 0));
+}
+
+// nonzero e0 + lane of [e0, end) (index -1 and residual 0 past the end)
+template <int NO>
+__device__ __forceinline__ void load_blk(const SampleArgs& a, int e0, int end, int lane, BlkReg<NO>& b) {
+  const int e = e0 + lane;
+  if (e < end) {
+#pragma unroll
+    for (int t = 0; t < NO; t++) b.ix[t] = a.rows.idx[(size_t)e * NO + t];
+    b.r = a.rows.r[e];
+  } else {
+#pragma unroll
+    for (int t = 0; t < NO; t++) b.ix[t] = -1;
+    b.r = 0.f;
+  }
+}
+
+// wave-sum of the SSE shares into sse_partial[slot]
+__device__ inline void write_sse(const SampleArgs& a, double acc, int slot) {
+  if (!a.sse_partial) return;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (threadIdx.x == 0) a.sse_partial[slot] = acc;
+}
+
+}  // namespace mgpu

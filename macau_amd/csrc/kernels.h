@@ -24,6 +24,7 @@ struct FactorSet {
 
 // Work list of the batched sampler (rows sorted by nnz, heavy rows split).
 struct BatchPlan {
+  int KP = 0;                  // padded latent size the plan's partials are sized for
   int n_light = 0;             // light rows (sorted by nnz, descending)
   const int* light = nullptr;  // row ids
   int n_seg = 0;               // segments of heavy rows
@@ -56,6 +57,10 @@ struct SampleArgs {
 
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
 void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
+// K in (32, 64] (KP = 64): one row per wave, three 32 x 32 Gram blocks
+void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
+int batched64_light_grid(const BatchPlan& p);
+int batched64_heavy_grid(const BatchPlan& p);
 int batched_supported(int KP, int nother);
 // the batched sampler addresses each other mode's factors with 32-bit byte
 // offsets: N_m * KP * 4 must stay below 2^31

@@ -357,6 +357,7 @@ BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<
   // per segment: the KP x KP Gram fragment and one RHS partial per lane (64)
   B[6].alloc(std::max<size_t>(sb.size(), 1) * (size_t)(KP * KP + 64) * sizeof(float));
   BatchPlan p;
+  p.KP = KP;
   p.n_light = (int)light.size();
   p.light = B[0].as<int>();
   p.n_seg = (int)sb.size();

@@ -32,17 +32,13 @@
 // so no separate pass over the nonzeros is needed.
 #include <stdexcept>
 
-#include "common.h"
-#include "kernels.h"
+#include "batched.h"
 
 namespace mgpu {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 constexpr int kPartial = 1024 + 64;  // floats per segment partial: fragment + RHS lanes
 
-int batched_supported(int KP, int nother) { return KP == 32 && nother >= 1 && nother <= kMaxOther; }
+int batched_supported(int KP, int nother) { return (KP == 32 || KP == 64) && nother >= 1 && nother <= kMaxOther; }
 
 bool batched_fits(const FactorSet& f, int nother, int KP) {
   for (int t = 0; t < nother; t++)
@@ -73,42 +69,6 @@ struct BlkLds {
   float r[2][kBlk];
 };
 
-template <int NO>
-struct BlkReg {
-  int ix[NO];
-  float r;
-};
-
-template <int NO>
-struct Factors {
-  __amdgpu_buffer_rsrc_t rs[NO];
-};
-
-// buffer descriptors of the other modes' factors (KP = 32 floats per column);
-// word 3 = 0x00020000 selects 32-bit data, raw (unswizzled) addressing
-template <int NO>
-__device__ __forceinline__ Factors<NO> make_factors(const FactorSet& f) {
-  Factors<NO> r;
-#pragma unroll
-  for (int t = 0; t < NO; t++)
-    r.rs[t] = __builtin_amdgcn_make_buffer_rsrc((void*)f.f[t], (short)0, f.n[t] * 32 * (int)sizeof(float),
-                                                0x00020000);
-  return r;
-}
-
-template <int NO>
-__device__ __forceinline__ void load_blk(const SampleArgs& a, int e0, int end, int lane, BlkReg<NO>& b) {
-  const int e = e0 + lane;
-  if (e < end) {
-#pragma unroll
-    for (int t = 0; t < NO; t++) b.ix[t] = a.rows.idx[(size_t)e * NO + t];
-    b.r = a.rows.r[e];
-  } else {
-#pragma unroll
-    for (int t = 0; t < NO; t++) b.ix[t] = -1;
-    b.r = 0.f;
-  }
-}
 
 template <int NO>
 __device__ __forceinline__ void stage_blk(BlkLds<NO>& L, int buf, int lane, const BlkReg<NO>& b) {
@@ -117,10 +77,6 @@ __device__ __forceinline__ void stage_blk(BlkLds<NO>& L, int buf, int lane, cons
   L.r[buf][lane] = b.r;
 }
 
-__device__ __forceinline__ float ld_factor(__amdgpu_buffer_rsrc_t rs, int ix, int comp) {
-  // ix = -1 gives an offset past num_records: the buffer load returns 0
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, ix * 128 + comp * 4, 0, 0));
-}
 
 template <int NO>
 __device__ __forceinline__ void gather_blk(const Factors<NO>& F, const BlkLds<NO>& L, int buf, int half, int comp,
@@ -135,9 +91,9 @@ __device__ __forceinline__ void gather_blk(const Factors<NO>& F, const BlkLds<NO
       for (int i = 0; i < 8; i++) ix[t][i] = L.ix[buf][t][o + i];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      float x = ld_factor(F.rs[0], ix[0][i], comp);
+      float x = ld_factor<32>(F.rs[0], ix[0][i], comp);
 #pragma unroll
-      for (int t = 1; t < NO; t++) x *= ld_factor(F.rs[t], ix[t][i], comp);
+      for (int t = 1; t < NO; t++) x *= ld_factor<32>(F.rs[t], ix[t][i], comp);
       v[c * 8 + i] = x;
     }
   }
@@ -390,17 +346,12 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
   return t;
 }
 
-__device__ inline void write_sse(const SampleArgs& a, double acc, int slot) {
-  if (!a.sse_partial) return;
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (threadIdx.x == 0) a.sse_partial[slot] = acc;
-}
 
 template <int NO>
 __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
   __shared__ PairLds s;
   __shared__ BlkLds<NO> blk;
-  const Factors<NO> F = make_factors<NO>(a.others);
+  const Factors<NO> F = make_factors<NO, 32>(a.others);
   const int it = *a.iter;
   double sse = 0.0;
   const int npairs = (p.n_light + 1) / 2;
@@ -429,7 +380,7 @@ __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
 template <int NO>
 __global__ void __launch_bounds__(64) k_seg32(SampleArgs a, BatchPlan p) {
   __shared__ BlkLds<NO> blk;
-  const Factors<NO> F = make_factors<NO>(a.others);
+  const Factors<NO> F = make_factors<NO, 32>(a.others);
   const int lane = threadIdx.x;
   for (int sg = blockIdx.x; sg < p.n_seg; sg += gridDim.x) {
     f32x16 acc, unused;
@@ -478,10 +429,12 @@ __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, in
 }
 
 int batched_light_grid(const BatchPlan& p) {
+  if (p.KP == 64) return batched64_light_grid(p);
   const int npairs = (p.n_light + 1) / 2;
   return npairs < 8192 ? npairs : 8192;
 }
 int batched_heavy_grid(const BatchPlan& p) {
+  if (p.KP == 64) return batched64_heavy_grid(p);
   const int npairs = (p.n_heavy + 1) / 2;
   return npairs < 2048 ? npairs : 2048;
 }
@@ -500,6 +453,9 @@ static void launch_batched_t(const SampleArgs& a, const BatchPlan& p, hipStream_
 void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s) {
   if (!batched_fits(a.others, a.rows.nother, a.KP))
     throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
+  if (a.KP != p.KP || !batched_supported(a.KP, a.rows.nother))
+    throw std::runtime_error("batched sampler: plan and latent size disagree or are unsupported");
+  if (a.KP == 64) return launch_sample_batched64(a, p, s);
   switch (a.rows.nother) {
     case 1: return launch_batched_t<1>(a, p, s);
     case 2: return launch_batched_t<2>(a, p, s);

@@ -259,7 +259,7 @@ BATCHED = _lib.SAMPLER_BATCHED
 TOL_BATCHED = 5e-3
 
 
-@pytest.mark.parametrize("K", [17, 24, 32])
+@pytest.mark.parametrize("K", [17, 24, 32, 33, 48, 64])
 def test_batched_sampler_matrix(K):
     announce("batched sampler matrix K=%d" % K)
     # heavy rows above the 1024-nnz segment length, an empty row, odd row count
@@ -271,9 +271,9 @@ def test_batched_sampler_matrix(K):
     assert err < TOL_BATCHED
 
 
-def test_batched_sampler_item_side_and_tensor():
-    announce("batched sampler item side / tensor")
-    K = 32
+@pytest.mark.parametrize("K", [32, 64])
+def test_batched_sampler_item_side_and_tensor(K):
+    announce("batched sampler item side / tensor K=%d" % K)
     rng, modes, mean, V, Lam, mu = _matrix_case(2, K, 3000, 200, 60000)
     U = f32(rng.standard_normal((3000, K)) * 0.5)
     got = gpu_sample_latents(K, modes[1], mean, [U], 1.3, mu, Lam, SEED, 11, 1, BATCHED)
@@ -291,9 +291,9 @@ def test_batched_sampler_item_side_and_tensor():
         assert row_rel_err(got, ref) < TOL_BATCHED
 
 
-def test_batched_sampler_per_row_mean():
-    announce("batched sampler per-row mean")
-    K = 32
+@pytest.mark.parametrize("K", [32, 64])
+def test_batched_sampler_per_row_mean(K):
+    announce("batched sampler per-row mean K=%d" % K)
     rng, modes, mean, V, Lam, mu = _matrix_case(8, K, 400, 3000, 8000, heavy=(2500,))
     mu_rows = f32(mu + rng.standard_normal((400, K)) * 0.2)
     got = gpu_sample_latents(K, modes[0], mean, [V], 2.0, mu, Lam, SEED, 4, 0, BATCHED, mu_rows=mu_rows)
@@ -301,10 +301,10 @@ def test_batched_sampler_per_row_mean():
     assert row_rel_err(got, ref) < TOL_BATCHED
 
 
-@pytest.mark.parametrize("precision", [5.0, "adaptive"])
-def test_batched_model_matches_oracle(precision):
-    announce("batched model precision=%s" % precision)
-    nrow, ncol, K = 3000, 1500, 32
+@pytest.mark.parametrize("precision,K", [(5.0, 32), ("adaptive", 32), ("adaptive", 64)])
+def test_batched_model_matches_oracle(precision, K):
+    announce("batched model precision=%s K=%d" % (precision, K))
+    nrow, ncol = 3000, 1500
     rng = np.random.default_rng(21)
     r, c, y = power_law_rows(rng, nrow, ncol, 90000, heavy=(1400, 1300))
     y = np.round(y * 2) / 2
