@@ -40,6 +40,32 @@ __device__ __forceinline__ float ld_factor(__amdgpu_buffer_rsrc_t rs, int ix, in
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, ix * (CS * 4) + comp * 4, 0, 0));
 }
 
+// x = hi + lo with hi = bf16(x) (round to nearest) and lo = bf16(x - hi),
+// eight values packed in the MFMA operand order (element 2p in the low half
+// of register p).  hi is converted once, packed (v_cvt_pk_bf16_f32), and
+// widened back by a shift or a mask of the packed word; the empty asm keeps
+// the compiler from re-deriving the low half's widening as a second
+// conversion.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  const bf16x2 v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+__device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo) {
+  uint32_t h[4], l[4];
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    h[p] = pack_bf16(x[2 * p], x[2 * p + 1]);
+    asm volatile("" : "+v"(h[p]));
+    const float h0 = __uint_as_float(h[p] << 16), h1 = __uint_as_float(h[p] & 0xffff0000u);
+    l[p] = pack_bf16(x[2 * p] - h0, x[2 * p + 1] - h1);
+  }
+  hi = __builtin_bit_cast(bf16x8, h);
+  lo = __builtin_bit_cast(bf16x8, l);
+}
+
 // nonzero e0 + lane of [e0, end) (index -1 and residual 0 past the end)
 template <int NO>
 __device__ __forceinline__ void load_blk(const SampleArgs& a, int e0, int end, int lane, BlkReg<NO>& b) {

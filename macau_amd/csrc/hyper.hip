@@ -383,199 +383,224 @@ __device__ void nw_draw(const NWArgs& a, double* base) {
 }
 
 // ------------------------------------------------------------------
-// The same draw for K <= 32 on one wave, in registers.  The problem is
-// padded to 32 with X_pad = diag(X, I) and C_pad = diag(C, I), and J is the
-// 32-reversal; then chol(X_pad^{-1}) = diag(chol(X^{-1}), I) and
-// B_pad = diag(B, I), so the leading K x K blocks are the quantities above
-// and every loop has a compile-time trip count.  Lane r (r = lane % 32; the
-// upper half of the wave mirrors the lower) holds row r of J X_pad J and of
-// its Cholesky factor M, then column r of Z = M^{-T} J C_pad^T and of
-// B = J Z, then row r of Lambda.
+// The same draw in one work-group of four waves for K <= 64, every loop
+// rolled over K with the matrices in LDS.  The factorisation and solves are
+// chains of dependent FP64 operations on one wave, so their dot products
+// carry four partial sums, the draws of the Bartlett factor (which the
+// factorisation does not need) run on the other three waves meanwhile, and
+// Lambda = B B^T is split over all four.
+//   A: J X J, then M = chol(J X J) in its lower triangle (left-looking, lane
+//      i = row i), then B (rows);
+//   C: the Bartlett factor C (rows); Z = M^{-T} R, R = J C^T, is solved in
+//      place (lane c = column c of Z; R[i][c] = C[c][K-1-i] is read once, at
+//      the step that writes Z[i][c] to the same place), which leaves
+//      C[c][m] = Z[K-1-m][c] = B[m][c], i.e. C = B^T with B = J Z.
 // ------------------------------------------------------------------
-struct NW32Lds {
-  alignas(16) double C[32][34];   // Bartlett factor C_pad (rows), later B (rows)
-  alignas(16) double Mt[32][34];  // M^T (row i = column i of M)
-  alignas(16) double col[2][32];  // Cholesky column broadcast
-  alignas(16) double muc[32];     // mu_c, then mu
-  alignas(16) double z[32];       // normals of the mu draw
-  alignas(16) double invd[32];    // 1 / M[k][k]
+constexpr int kNWW = 64;         // largest K of this draw
+constexpr int kNWLd = kNWW + 1;  // row stride (doubles): lanes on their own rows hit distinct banks
+constexpr int kNWThreadsW = 256;
+
+struct NWWaveLds {
+  double A[kNWW][kNWLd];
+  double C[kNWW][kNWLd];
+  double muc[kNWW];     // mu_c, then mu
+  double z[kNWW];       // normals of the mu draw
+  double invd[kNWW];    // 1 / M[k][k]
+  double lm[4][kNWW];   // Lambda mu, per wave
 };
 
-__device__ void nw_draw32(const NWArgs& a, NW32Lds& s) {
-  int lane = threadIdx.x & 63;
-  opaque(lane);
+// Philox block (row i, block b) of the strictly upper Bartlett normals
+// C[i][4b .. 4b+3]: item t of the blocks holding any j with i < j < K, in
+// block-column order
+__device__ __forceinline__ bool bartlett_block(int t, int K, int& i, int& b) {
+  for (int q = 0; 4 * q < K; q++) {
+    const int rows = min(4 * q + 3, K - 1);  // i = 0 .. rows-1 have a j > i in the block
+    if (t < rows) {
+      i = t;
+      b = q;
+      return true;
+    }
+    t -= rows;
+  }
+  return false;
+}
+
+// 1/sqrt(d), d > 0: the hardware estimate (v_rsq_f64) refined by two
+// Newton-Raphson steps for f(y) = y^-2 - d, y <- y (3 - d y^2) / 2, each of
+// which roughly squares the relative error
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// sum_{j in [j0, j1)} x[j] y[j] with four partial sums
+__device__ __forceinline__ double dot4(const double* x, const double* y, int j0, int j1) {
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+    p0 += x[j] * y[j];
+    p1 += x[j + 1] * y[j + 1];
+    p2 += x[j + 2] * y[j + 2];
+    p3 += x[j + 3] * y[j + 3];
+  }
+  for (; j < j1; j++) p0 += x[j] * y[j];
+  return (p0 + p1) + (p2 + p3);
+}
+
+// Lambda = B B^T (B rows in A, zero beyond column K) on all waves: lane r =
+// row r, wave w takes columns j = w, w + 4, ..., four at a time.  KB bounds
+// the row length.
+template <int KB>
+__device__ __forceinline__ void nw_lambda(const NWArgs& a, NWWaveLds& s, int wave, int lane) {
+  const int K = a.K, KP = a.KP;
+  const int r = lane;
+  double lm = 0.0;
+  for (int j0 = 4 * wave; j0 < KP; j0 += 16) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* bj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) bj[u] = s.A[min(j0 + u, kNWW - 1)];
+#pragma unroll 2
+    for (int k = 0; k < KB; k += 8) {
+      double br[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) br[t] = s.A[r][k + t];
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc[u] += br[t] * bj[u][k + t];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = j0 + u;
+      if (j >= KP) break;
+      const bool in = r < K && j < K;
+      if (in) {
+        lm += acc[u] * s.muc[j];
+        a.Lambda[r + j * K] = acc[u];
+      }
+      if (r < KP) a.Lambda_f[r + j * KP] = in ? (float)acc[u] : (r == j ? 1.f : 0.f);
+    }
+  }
+  s.lm[wave][r] = lm;
+}
+
+__device__ void nw_draw_w(const NWArgs& a, NWWaveLds& s) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int K = a.K;
-  int r = lane & 31;
+  const bool act = lane < K;
   const int it = *a.iter;
   const double N = a.N;
   const double kappa_c = a.kappa + N, nu_c = a.nu + N;
-  // 1. Bartlett factor: C_ii = sqrt(chi2(nu_c - i)), C_ij ~ N(0,1) for
-  //    i < j < K (WishartUnit, mvnormal.cpp:217-241), identity beyond K;
-  //    the K(K+1)/2 draws are spread over the wave, column by column.
-  for (int q = lane; q < 32 * 32; q += 64) {
-    const int i = q >> 5, j = q & 31;
-    if (i <= j && j < K) continue;
-    s.C[i][j] = (i == j) ? 1.0 : 0.0;
-  }
-  const int ndraw = K * (K + 1) / 2;
-  for (int t = lane; t < ndraw; t += 64) {
-    int j = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
-    while ((j + 1) * (j + 2) / 2 <= t) j++;
-    while (j * (j + 1) / 2 > t) j--;
-    const int i = t - j * (j + 1) / 2;
-    double c;
-    if (i == j)
-      c = sqrt(2.0 * gamma_f64(a.seed, 0.5 * (nu_c - i), (uint32_t)i, (uint32_t)it, make_tag(TAG_WISHART_CHI, a.mode)));
-    else
-      c = normal_f64(a.seed, (uint32_t)i, (uint32_t)j, (uint32_t)it, make_tag(TAG_WISHART_Z, a.mode));
-    s.C[i][j] = c;
-  }
-  if (lane < 32) {
-    s.z[lane] = lane < K ? normal_f64(a.seed, 0u, (uint32_t)lane, (uint32_t)it, make_tag(TAG_NW_MU, a.mode)) : 0.0;
-    // 2. mu_c (mvnormal.cpp:322-323)
-    s.muc[lane] = lane < K ? (a.kappa * a.mu0[lane] + a.partial[lane]) / kappa_c : 0.0;
-  }
-  lds_wait();
-  // 3. row r of Y = J X_pad J, X = T + S + kappa mu0 mu0^T - kappa_c mu_c mu_c^T
-  //    (mvnormal.cpp:324-325; Macau adds lambda_beta beta beta^T to T,
-  //    latentprior.cpp:258-260)
-  const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
-  const int KK = K * K;
-  // the whole matrix is formed by the wave in LDS (s.Mt as staging, entry
-  // (r, c) at Mt[r][c]) with branch-free loads, then each lane takes its row
-  {
+  bool bad = false;
+  if (wave == 0) {
+    // 1. mu_c (mvnormal.cpp:322-323) and the normals of the mu draw
+    if (act) {
+      s.z[lane] = normal_f64(a.seed, 0u, (uint32_t)lane, (uint32_t)it, make_tag(TAG_NW_MU, a.mode));
+      s.muc[lane] = (a.kappa * a.mu0[lane] + a.partial[lane]) / kappa_c;
+    }
+    lds_wait();
+    // 2. A = J X J, X = T + S + kappa mu0 mu0^T - kappa_c mu_c mu_c^T
+    //    (mvnormal.cpp:324-325; Macau adds lambda_beta beta beta^T to T,
+    //    latentprior.cpp:258-260); lane = row, loads coalesced along it
+    const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
     const double* bbt = a.BBt_partial ? a.BBt_partial + K : a.partial + K;
     const double lbb = a.BBt_partial ? lb : 0.0;
+    const int xi = act ? K - 1 - lane : 0;
+    const double m0i = a.mu0[xi], mci = s.muc[xi];
 #pragma unroll 8
-    for (int q = lane; q < 32 * 32; q += 64) {
-      const int rr = q >> 5, c = q & 31;
-      const int xi = 31 - rr, xj = 31 - c;
-      const bool in = xi < K && xj < K;
-      const int xic = xi < K ? xi : 0, xjc = xj < K ? xj : 0;
-      const int e = xic + xjc * K;
-      const double x = a.WI[e] + lbb * bbt[e] + a.partial[K + e] + a.kappa * a.mu0[xic] * a.mu0[xjc] -
-                       kappa_c * s.muc[xic] * s.muc[xjc];
-      s.Mt[rr][c] = in ? x : (rr == c ? 1.0 : 0.0);
+    for (int c = 0; c < K; c++) {
+      const int xj = K - 1 - c;
+      const int e = xi + xj * K;
+      const double x = a.WI[e] + lbb * bbt[e] + a.partial[K + e] + a.kappa * m0i * a.mu0[xj] - kappa_c * mci * s.muc[xj];
+      if (act) s.A[lane][c] = x;
     }
-  }
-  (void)KK;
-  lds_wait();
-  double Y[32];
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const double2 t2 = *reinterpret_cast<const double2*>(&s.Mt[r][2 * q]);
-    Y[2 * q] = t2.x;
-    Y[2 * q + 1] = t2.y;
-  }
-  // 4. M = chol(Y), row r on lane r (right-looking, column k + 1 published
-  //    ahead of the rest of step k)
-  bool bad = false;
-  s.col[0][r] = Y[0];
-#pragma unroll
-  for (int k = 0; k < 32; k++) {
-    opaque(r);
     lds_wait();
-    double col[32];
-#pragma unroll
-    for (int q = k / 2; q < 16; q++) {
-      const double2 t2 = *reinterpret_cast<const double2*>(&s.col[k & 1][2 * q]);
-      col[2 * q] = t2.x;
-      col[2 * q + 1] = t2.y;
+    // 3. M = chol(A), left-looking: s_i = A[i][k] - sum_{j<k} M[i][j] M[k][j]
+    //    on lane i, the pivot from lane k
+    for (int k = 0; k < K; k++) {
+      const double sk = s.A[lane][k] - dot4(s.A[lane], s.A[k], 0, k);
+      const double d = rdlane(sk, k);
+      const bool badp = !(d > 0.0);
+      bad = bad || badp;
+      const double rs = rsqrt_nr(badp ? 1.0 : d);
+      if (act && lane >= k) s.A[lane][k] = sk * rs;  // M[i][k]
+      s.invd[k] = rs;                               // every lane stores the same value
+      lds_wait();
     }
-    const bool badp = !(col[k] > 0.0);
-    bad = bad || badp;
-    const double d = badp ? 1.0 : col[k];
-    const double rsd = 1.0 / sqrt(d);
-    s.invd[k] = rsd;  // 1 / M[k][k] (every lane stores the same value)
-    const bool gt = r > k;
-    const double sc = gt ? Y[k] * (rsd * rsd) : 0.0;
-    if (k + 1 < 32) {
-      Y[k + 1] -= sc * col[k + 1];
-      s.col[(k + 1) & 1][r] = Y[k + 1];
-    }
+  } else {
+    // 4. Bartlett factor (waves 1-3): C_ii = sqrt(chi2(nu_c - i)),
+    //    C_ij ~ N(0,1) for i < j < K (WishartUnit, mvnormal.cpp:217-241),
+    //    zero below the diagonal; normals C[i][4b..4b+3] from one Philox block
+    const int t0 = threadIdx.x - 64;
+    for (int i = wave; i < K; i += 3)
+      if (lane < i) s.C[i][lane] = 0.0;
+    for (int t = t0;; t += kNWThreadsW - 64) {
+      int i, b;
+      if (!bartlett_block(t, K, i, b)) break;
+      double zz[4];
+      normal_f64x4(a.seed, (uint32_t)i, (uint32_t)b, (uint32_t)it, make_tag(TAG_WISHART_Z, a.mode), zz);
 #pragma unroll
-    for (int j = k + 2; j < 32; j++) Y[j] -= sc * col[j];
-    Y[k] = (r >= k) ? Y[k] * rsd : 0.0;  // M[r][k]; zero above the diagonal
+      for (int u = 0; u < 4; u++) {
+        const int j = 4 * b + u;
+        if (j > i && j < K) s.C[i][j] = zz[u];
+      }
+    }
+    for (int i = t0; i < K; i += kNWThreadsW - 64)
+      s.C[i][i] = sqrt(2.0 * gamma_f64(a.seed, 0.5 * (nu_c - i), (uint32_t)i, (uint32_t)it,
+                                       make_tag(TAG_WISHART_CHI, a.mode)));
   }
-  // M^T to LDS: row i of Mt = column i of M
-#pragma unroll
-  for (int c = 0; c < 32; c++) s.Mt[c][r] = Y[c];
-  lds_wait();
-  // 5. Z = M^{-T} R, R = J C_pad^T (R[i][c] = C_pad[c][31 - i]); lane r
-  //    solves for column r of Z from the bottom row up
-  double z[32];
-#pragma unroll
-  for (int i = 31; i >= 0; i--) {
-    int o = 0;  // opaque zero offset: the row's reads stay in this step
-    opaque(o);
-    double acc = s.C[r][31 - i];
-#pragma unroll
-    for (int q = (i + 1) / 2; q < 16; q++) {
-      const double2 m2 = *reinterpret_cast<const double2*>(&s.Mt[i][2 * q + o]);
-      if (2 * q > i) acc -= m2.x * z[2 * q];
-      if (2 * q + 1 > i) acc -= m2.y * z[2 * q + 1];
+  __syncthreads();
+  if (wave == 0) {
+    // 5. Z = M^{-T} R from the bottom row up, lane c = column c, in place in C
+    for (int i = K - 1; i >= 0; i--) {
+      double p0 = s.C[lane][K - 1 - i], p1 = 0.0, p2 = 0.0, p3 = 0.0;
+      int k = i + 1;
+      for (; k + 4 <= K; k += 4) {
+        p0 -= s.A[k][i] * s.C[lane][K - 1 - k];
+        p1 -= s.A[k + 1][i] * s.C[lane][K - 2 - k];
+        p2 -= s.A[k + 2][i] * s.C[lane][K - 3 - k];
+        p3 -= s.A[k + 3][i] * s.C[lane][K - 4 - k];
+      }
+      for (; k < K; k++) p0 -= s.A[k][i] * s.C[lane][K - 1 - k];
+      if (act) s.C[lane][K - 1 - i] = ((p0 + p1) + (p2 + p3)) * s.invd[i];
     }
-    z[i] = acc * s.invd[i + o];
-  }
-  // 6. B = J Z: lane r holds column r of B (b[i] = z[31 - i]); rows to LDS
-  lds_wait();
-#pragma unroll
-  for (int i = 0; i < 32; i++) s.C[i][r] = z[31 - i];
-  lds_wait();
-  // 7. mu = mu_c + B^{-T} z / sqrt(kappa_c) (MvNormal_prec, mvnormal.cpp:169-184):
-  //    B^T x = z from the last row up; lane r holds x_r and column r of B
-  {
-    double acc = s.z[r];
-    const double invd = 1.0 / s.C[r][r];
-#pragma unroll
-    for (int j = 31; j >= 0; j--) {
-      opaque(r);
-      const double xj = rdlane(acc * invd, j);
-      acc = (r < j) ? acc - z[31 - j] * xj : (r == j ? xj : acc);
-    }
-    const double mu = s.muc[r] + acc / sqrt(kappa_c);
     lds_wait();
-    s.muc[r] = mu;
-  }
-  if (lane < 32 && r < K) {
-    a.mu[r] = s.muc[r];
-    if (a.Lchol) {
-#pragma unroll
-      for (int j = 0; j < 32; j++)
-        if (j < K) a.Lchol[j + r * K] = z[31 - j];  // B[j][r]
+    // 6. B rows into A (B[i][c] = C[c][i]), zero beyond column K
+    for (int i = 0; i < K; i++) s.A[i][lane] = act ? s.C[lane][i] : 0.0;
+    // 7. mu = mu_c + B^{-T} z / sqrt(kappa_c) (MvNormal_prec, mvnormal.cpp:169-184):
+    //    B^T x = z from the last row up; lane r holds x_r, B[j][r] = C[r][j]
+    double acc = act ? s.z[lane] : 0.0;
+    const double invb = act ? 1.0 / s.C[lane][lane] : 0.0;
+    for (int j = K - 1; j >= 0; j--) {
+      const double xj = rdlane(acc * invb, j);
+      acc = (lane < j) ? acc - s.C[lane][j] * xj : (lane == j ? xj : acc);
     }
-  }
-  lds_wait();
-  // 8. Lambda = B B^T, row r, written out with Lambda mu as it is formed
-  double brow[32];
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const double2 t2 = *reinterpret_cast<const double2*>(&s.C[r][2 * q]);
-    brow[2 * q] = t2.x;
-    brow[2 * q + 1] = t2.y;
-  }
-  const int KP = a.KP;
-  const bool wr = lane < 32;
-  double lm = 0.0;
-  for (int j = 0; j < 32; j++) {
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const double2 t2 = *reinterpret_cast<const double2*>(&s.C[j][2 * q]);
-      acc += brow[2 * q] * t2.x + brow[2 * q + 1] * t2.y;
+    if (act) {
+      const double mu = s.muc[lane] + acc / sqrt(kappa_c);
+      s.muc[lane] = mu;
+      a.mu[lane] = mu;
     }
-    const bool in = r < K && j < K;
-    if (in) lm += acc * s.muc[j];
-    if (wr && in) a.Lambda[r + j * K] = acc;
-    if (wr && r < KP && j < KP) a.Lambda_f[r + j * KP] = in ? (float)acc : (r == j ? 1.f : 0.f);
+    lds_wait();
+    if (a.Lchol && act)
+      for (int r = 0; r < K; r++) a.Lchol[lane + r * K] = s.A[lane][r];  // B[lane][r]
   }
-  if (wr && r < KP) a.Lmu_f[r] = (float)lm;
-  if (lane == 0 && bad && a.fail) atomicMax(a.fail, 1 << 30);
+  __syncthreads();
+  // 8. Lambda = B B^T and Lambda mu
+  if (K <= 32) nw_lambda<32>(a, s, wave, lane);
+  else nw_lambda<64>(a, s, wave, lane);
+  __syncthreads();
+  if (wave == 0 && lane < a.KP)
+    a.Lmu_f[lane] = (float)((s.lm[0][lane] + s.lm[1][lane]) + (s.lm[2][lane] + s.lm[3][lane]));
+  if (threadIdx.x == 0 && bad && a.fail) atomicMax(a.fail, 1 << 30);
 }
 
-__global__ void __launch_bounds__(64) k_nw_draw32(NWBatch b) {
-  __shared__ NW32Lds s;
-  nw_draw32(b.m[blockIdx.x], s);
+__global__ void __launch_bounds__(kNWThreadsW) k_nw_draw_w(NWBatch b) {
+  __shared__ NWWaveLds s;
+  nw_draw_w(b.m[blockIdx.x], s);
 }
 
 // one work-group per mode of the batch
@@ -590,10 +615,11 @@ size_t nw_workspace_bytes(int K) { return nw_lds_bytes(K) <= 48 * 1024 ? 0 : nw_
 
 void launch_nw_draw_batch(const NWBatch& b, hipStream_t s) {
   if (b.n <= 0) return;
-  bool small = true;
-  for (int q = 0; q < b.n; q++) small = small && b.m[q].K <= 32 && b.m[q].nblk == 1;
-  if (small) {
-    hipLaunchKernelGGL(k_nw_draw32, dim3(b.n), dim3(64), 0, s, b);
+  // one wave per mode when the statistics arrive reduced and K <= 64
+  bool wave = true;
+  for (int q = 0; q < b.n; q++) wave = wave && b.m[q].nblk == 1 && b.m[q].K <= kNWW;
+  if (wave) {
+    hipLaunchKernelGGL(k_nw_draw_w, dim3(b.n), dim3(kNWThreadsW), 0, s, b);
     return;
   }
   const bool lds = b.m[0].ws == nullptr;

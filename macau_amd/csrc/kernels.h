@@ -35,6 +35,11 @@ struct BatchPlan {
   const int* heavy = nullptr;      // row id of each heavy slot
   const int* heavy_seg0 = nullptr; // first segment of each heavy slot (n_heavy + 1)
   float* partial = nullptr;        // per-segment Gram (KP*KP) and rhs (KP)
+  // light rows as the finishing kernel takes them: pairs of consecutive
+  // light rows (KP = 32) or single rows (KP = 64, row y = -1)
+  int n_pair = 0;
+  const int4* pair_rng = nullptr;  // nnz ranges [x, y) and [z, w) (z = w = 0: no second row)
+  const int2* pair_row = nullptr;  // row ids (y = -1: no second row)
 };
 
 struct SampleArgs {

@@ -314,11 +314,19 @@ __device__ void reduce_sym(const double* partial, int nblk, int nr, double* M) {
   (void)nblk;
   const int NN = nr * nr;
   double v[kSmallPer];
+  // (i, j) of entry q = lane + 64 u, stepped without divisions
+  const int di = 64 % nr, dj = 64 / nr;
+  int i = threadIdx.x % nr, j = threadIdx.x / nr;
 #pragma unroll
   for (int u = 0; u < kSmallPer; u++) {
     const int q = threadIdx.x + 64 * u;
-    const int i = q % nr, j = q / nr;
     v[u] = q < NN ? partial[(i >= j) ? q : (j + i * nr)] : 0.0;
+    i += di;
+    j += dj;
+    if (i >= nr) {
+      i -= nr;
+      j++;
+    }
   }
 #pragma unroll
   for (int u = 0; u < kSmallPer; u++) {

@@ -228,6 +228,12 @@ Model::~Model() {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
   }
+  if (side) {
+    (void)hipStreamSynchronize(side);
+    (void)hipStreamDestroy(side);
+  }
+  for (auto& e : ev_fork) (void)hipEventDestroy(e);
+  if (ev_join) (void)hipEventDestroy(ev_join);
 }
 
 FactorSet Model::others_of(int mode) const {
@@ -247,6 +253,12 @@ void Model::init() {
     throw std::runtime_error("number of priors must equal the number of data modes");
   if (rows.empty()) throw std::runtime_error("training data not set");
   if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  if (!side) HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  if (ev_fork.empty()) {
+    ev_fork.resize(nmodes);
+    for (auto& e : ev_fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  }
   if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
   if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
   d_rowptr.clear(); d_idx.clear(); d_r.clear(); d_U.clear(); rdev.clear();
@@ -346,8 +358,22 @@ BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<
     }
   }
   hseg0.push_back((int)sb.size());
+  // the light rows as the finishing kernel takes them (pairs for KP = 32)
+  const int per = KP == 32 ? 2 : 1;
+  const int npair = ((int)light.size() + per - 1) / per;
+  std::vector<int> prng(4 * (size_t)npair, 0), prow(2 * (size_t)npair, -1);
+  for (int q = 0; q < npair; q++) {
+    for (int u = 0; u < per && q * per + u < (int)light.size(); u++) {
+      const int r = light[q * per + u];
+      prow[2 * q + u] = r;
+      prng[4 * q + 2 * u] = row_ptr[r];
+      prng[4 * q + 2 * u + 1] = row_ptr[r + 1];
+    }
+  }
   B.clear();
-  B.resize(7);
+  B.resize(9);
+  upload(B[7], prng, s);
+  upload(B[8], prow, s);
   upload(B[0], light, s);
   upload(B[1], sr, s);
   upload(B[2], sb, s);
@@ -368,6 +394,9 @@ BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<
   p.heavy = B[4].as<int>();
   p.heavy_seg0 = B[5].as<int>();
   p.partial = B[6].as<float>();
+  p.n_pair = npair;
+  p.pair_rng = reinterpret_cast<const int4*>(B[7].as<int>());
+  p.pair_row = reinterpret_cast<const int2*>(B[8].as<int>());
   HIPCHK(hipStreamSynchronize(s));
   return p;
 }
@@ -408,20 +437,35 @@ SampleArgs Model::sample_args(int mode, const ILatentPrior& p) {
   return a;
 }
 
-void Model::update_priors() {
-  // priors are independent of each other (each reads only its own mode's
-  // samples); BPMF draws of all modes share one launch
-  NWBatch b;
-  for (int m = 0; m < nmodes; m++)
-    if (!priors[m]->batch_update(*this, m, b)) priors[m]->update_prior(*this, m);
-  launch_nw_draw_batch(b, stream);
-}
+namespace {
+// makes Model::stream the side stream for the lifetime of the guard, so the
+// prior code (which launches on m.stream) runs there
+struct OnSideStream {
+  Model& m;
+  explicit OnSideStream(Model& mm) : m(mm) { std::swap(m.stream, m.side); }
+  ~OnSideStream() { std::swap(m.stream, m.side); }
+};
+}  // namespace
 
 void Model::step_launch() {
-  for (int m = 0; m < nmodes; m++) priors[m]->sample_latents(*this, m);
-  update_priors();
+  // The reference samples every mode, then updates every prior
+  // (macau.cpp:120-128).  The update of mode m reads only mode m's new
+  // samples and its draws are keyed by (iteration, mode), so it is launched
+  // on the side stream as soon as mode m is sampled and runs while the next
+  // modes are sampled (for the last mode, while the noise and the test
+  // predictions are updated).  The streams join before the iteration
+  // counter advances, so the next iteration sees every update.
+  for (int m = 0; m < nmodes; m++) {
+    priors[m]->sample_latents(*this, m);
+    HIPCHK(hipEventRecord(ev_fork[m], stream));
+    HIPCHK(hipStreamWaitEvent(side, ev_fork[m], 0));
+    OnSideStream guard(*this);
+    priors[m]->update_prior(*this, m);
+  }
   noise->update(*this);
   noise->evalModel(*this);
+  HIPCHK(hipEventRecord(ev_join, side));
+  HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   launch_advance_iter(d_iter.as<int>(), stream);
 }
 
@@ -690,16 +734,9 @@ static NWArgs nw_args(Model& m, ILatentPrior& p, int mode, double nu) {
 }
 
 void BPMFPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:79-81
-  NWBatch b;
-  batch_update(m, mode, b);
-  launch_nw_draw_batch(b, m.stream);
-}
-
-bool BPMFPrior::batch_update(Model& m, int mode, NWBatch& b) {
   launch_hyper_stats_f32(m.d_U[mode].as<float>(), nullptr, m.KP, m.dims[mode], m.num_latent, stats.as<double>(),
                          stats_nblk, m.stream);
-  b.m[b.n++] = nw_args(m, *this, mode, (double)m.num_latent);  // df = K
-  return true;
+  launch_nw_draw(nw_args(m, *this, mode, (double)m.num_latent), m.stream);  // df = K
 }
 
 // ---- MacauPrior ----

@@ -74,9 +74,6 @@ class ILatentPrior {
   virtual void init(Model& m, int mode);
   virtual void sample_latents(Model& m, int mode);
   virtual void update_prior(Model& m, int mode) = 0;
-  // enqueue the statistics and append the Normal-Wishart draw to a batch
-  // launched for several modes at once; false: use update_prior
-  virtual bool batch_update(Model&, int, NWBatch&) { return false; }
   virtual double getLinkNorm(Model&) { return NAN; }
   virtual double getLinkLambda(Model&) { return NAN; }
   virtual void saveModel(Model& m, int mode, const std::string& prefix);
@@ -91,7 +88,6 @@ class ILatentPrior {
 class BPMFPrior : public ILatentPrior {
  public:
   void update_prior(Model& m, int mode) override;
-  bool batch_update(Model& m, int mode, NWBatch& b) override;
 };
 
 // MacauPrior<SparseDoubleFeat> (latentprior.h:82-125, latentprior.cpp:180-331)
@@ -221,6 +217,9 @@ class Model {
 
   // -- device state --
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // prior updates overlapped with sampling (step_launch)
+  std::vector<hipEvent_t> ev_fork;
+  hipEvent_t ev_join = nullptr;
   std::vector<DevBuf> d_rowptr, d_idx, d_r, d_U;
   std::vector<RowsDev> rdev;
   std::vector<BatchPlan> plans;
@@ -238,7 +237,6 @@ class Model {
   FactorSet others_of(int mode) const;
   SampleArgs sample_args(int mode, const ILatentPrior& p);
   void step_launch();  // enqueue one Gibbs iteration
-  void update_priors();
   double factor_norm(int mode);
   double host_eval_out(int which);
 

@@ -107,14 +107,9 @@ __device__ __forceinline__ void mfma_blk(const BlkLds<NO>& L, int buf, int half,
     if (c * 16 >= nvalid) break;
     const int o = c * 16 + half * 8;
     bf16x8 hi, lo;
+    split8(v + c * 8, hi, lo);
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const float x = v[c * 8 + i];
-      const __bf16 h = (__bf16)x;
-      hi[i] = h;
-      lo[i] = (__bf16)(x - (float)h);
-      bp += x * L.r[buf][o + i];
-    }
+    for (int i = 0; i < 8; i++) bp += v[c * 8 + i] * L.r[buf][o + i];
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc, 0, 0, 0);
@@ -159,12 +154,21 @@ struct GramStream {
     nxt = after;
   }
 
-  __device__ __forceinline__ void run(f32x16& accA, float& bpA, f32x16& accB, float& bpB) {
+  // index/residual loads of the stream's first two blocks for the ranges
+  // [r.x, r.y) and [r.z, r.w) (what start(0..1) / stop(0..1) address); a
+  // caller issues them ahead of time and hands them to run()
+  static __device__ __forceinline__ void first_blocks(const SampleArgs& a, int4 r, int lane, BlkReg<NO>& b0,
+                                                      BlkReg<NO>& b1) {
+    const int na = (r.y - r.x + kBlk - 1) / kBlk;
+    load_blk<NO>(a, na > 0 ? r.x : r.z, na > 0 ? r.y : r.w, lane, b0);
+    load_blk<NO>(a, na > 1 ? r.x + kBlk : r.z + (1 - na) * kBlk, na > 1 ? r.y : r.w, lane, b1);
+  }
+
+  __device__ __forceinline__ void run(const BlkReg<NO>& b0, const BlkReg<NO>& b1, f32x16& accA, float& bpA,
+                                      f32x16& accB, float& bpB) {
     if (nb == 0) return;
     float v0[32], v1[32];
-    BlkReg<NO> b0;
-    load_blk<NO>(a, start(0), stop(0), lane, b0);
-    if (nb > 1) load_blk<NO>(a, start(1), stop(1), lane, nxt);
+    nxt = b1;
     stage_blk<NO>(L, 0, lane, b0);
     lds_wait();
     gather_blk<NO>(F, L, 0, half, comp, v0);
@@ -200,14 +204,13 @@ struct PairLds {
 //      matrix B in lanes 32-63).
 // L rows go to LDS as they are produced, for the back solve.
 __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 accA, f32x16 accB, float bpA,
-                             float bpB, int it, PairLds& s) {
+                             float bpB, int it, float alpha, PairLds& s) {
   int lane = threadIdx.x & 63;
   opaque(lane);
   const int half = lane >> 5;
   int i = lane & 31;
   const int row = half ? rowB : rowA;
   const bool active = row >= 0;
-  const float alpha = (float)(*a.alpha);
   // -P = -(Lambda + alpha G) in fragment layout (Lambda_f is symmetric and
   // 4 consecutive registers hold 4 consecutive rows of one column)
   {
@@ -347,22 +350,35 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
 }
 
 
+// Each wave takes pairs q, q + G, ... (G = grid).  The pair's row ids and nnz
+// ranges come from the plan in one load, and the next pair's ranges and its
+// first two index blocks are loaded while the current pair is multiplied and
+// finished, so a pair's stream starts with its indices already in registers.
 template <int NO>
 __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
   __shared__ PairLds s;
   __shared__ BlkLds<NO> blk;
   const Factors<NO> F = make_factors<NO, 32>(a.others);
   const int it = *a.iter;
+  const float alpha = (float)(*a.alpha);
+  const int lane = threadIdx.x;
   double sse = 0.0;
-  const int npairs = (p.n_light + 1) / 2;
-  for (int q = blockIdx.x; q < npairs; q += gridDim.x) {
-    const int rowA = p.light[2 * q];
-    const int rowB = (2 * q + 1 < p.n_light) ? p.light[2 * q + 1] : -1;
-    const int begA = a.rows.row_ptr[rowA], endA = a.rows.row_ptr[rowA + 1];
-    int begB = 0, endB = 0;
-    if (rowB >= 0) {
-      begB = a.rows.row_ptr[rowB];
-      endB = a.rows.row_ptr[rowB + 1];
+  int q = blockIdx.x;
+  int4 rng = make_int4(0, 0, 0, 0);
+  int2 rows = make_int2(-1, -1);
+  BlkReg<NO> b0, b1;
+  if (q < p.n_pair) {
+    rng = p.pair_rng[q];
+    rows = p.pair_row[q];
+    GramStream<NO, true>::first_blocks(a, rng, lane, b0, b1);
+  }
+  for (; q < p.n_pair; q += gridDim.x) {
+    const int qn = q + gridDim.x;
+    int4 rng_n = make_int4(0, 0, 0, 0);
+    int2 rows_n = make_int2(-1, -1);
+    if (qn < p.n_pair) {
+      rng_n = p.pair_rng[qn];
+      rows_n = p.pair_row[qn];
     }
     f32x16 accA, accB;
 #pragma unroll
@@ -371,8 +387,11 @@ __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
       accB[r] = 0.f;
     }
     float bpA = 0.f, bpB = 0.f;
-    GramStream<NO, true>(a, F, blk, begA, endA, begB, endB).run(accA, bpA, accB, bpB);
-    sse += (double)finish_pair(a, rowA, rowB, accA, accB, bpA, bpB, it, s);
+    GramStream<NO, true>(a, F, blk, rng.x, rng.y, rng.z, rng.w).run(b0, b1, accA, bpA, accB, bpB);
+    if (qn < p.n_pair) GramStream<NO, true>::first_blocks(a, rng_n, lane, b0, b1);
+    sse += (double)finish_pair(a, rows.x, rows.y, accA, accB, bpA, bpB, it, alpha, s);
+    rng = rng_n;
+    rows = rows_n;
   }
   write_sse(a, sse, blockIdx.x);
 }
@@ -387,7 +406,10 @@ __global__ void __launch_bounds__(64) k_seg32(SampleArgs a, BatchPlan p) {
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0.f;
     float bp = 0.f, unused_bp = 0.f;
-    GramStream<NO, false>(a, F, blk, p.seg_beg[sg], p.seg_end[sg], 0, 0).run(acc, bp, unused, unused_bp);
+    const int4 r = make_int4(p.seg_beg[sg], p.seg_end[sg], 0, 0);
+    BlkReg<NO> b0, b1;
+    GramStream<NO, false>::first_blocks(a, r, lane, b0, b1);
+    GramStream<NO, false>(a, F, blk, r.x, r.y, 0, 0).run(b0, b1, acc, bp, unused, unused_bp);
     float* out = p.partial + (size_t)sg * kPartial;
 #pragma unroll
     for (int r = 0; r < 16; r++) out[r * 64 + lane] = acc[r];
@@ -409,6 +431,7 @@ __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, in
   __shared__ PairLds s;
   const int lane = threadIdx.x;
   const int it = *a.iter;
+  const float alpha = (float)(*a.alpha);
   double sse = 0.0;
   const int npairs = (p.n_heavy + 1) / 2;
   for (int q = blockIdx.x; q < npairs; q += gridDim.x) {
@@ -423,15 +446,14 @@ __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, in
     const int rowB = (2 * q + 1 < p.n_heavy) ? p.heavy[2 * q + 1] : -1;
     sum_segments(p, 2 * q, lane, accA, bpA);
     if (rowB >= 0) sum_segments(p, 2 * q + 1, lane, accB, bpB);
-    sse += (double)finish_pair(a, rowA, rowB, accA, accB, bpA, bpB, it, s);
+    sse += (double)finish_pair(a, rowA, rowB, accA, accB, bpA, bpB, it, alpha, s);
   }
   write_sse(a, sse, sse_slot0 + blockIdx.x);
 }
 
 int batched_light_grid(const BatchPlan& p) {
   if (p.KP == 64) return batched64_light_grid(p);
-  const int npairs = (p.n_light + 1) / 2;
-  return npairs < 8192 ? npairs : 8192;
+  return p.n_pair < 8192 ? p.n_pair : 8192;
 }
 int batched_heavy_grid(const BatchPlan& p) {
   if (p.KP == 64) return batched64_heavy_grid(p);

@@ -98,15 +98,6 @@ __device__ __forceinline__ void gather_blk64(const Factors<NO>& F, const BlkLds6
   }
 }
 
-__device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const __bf16 h = (__bf16)v[i];
-    hi[i] = h;
-    lo[i] = (__bf16)(v[i] - (float)h);
-  }
-}
-
 template <int NO>
 __device__ __forceinline__ void mfma_blk64(const BlkLds64<NO>& L, int buf, int half, const float v0[16],
                                            const float v1[16], int nvalid, Gram64& g) {
@@ -166,13 +157,19 @@ struct GramStream64 {
     nxt = after;
   }
 
-  __device__ __forceinline__ void run(Gram64& g) {
+  // index/residual loads of the first two blocks of [b, e), issued ahead
+  // of run() by the caller
+  static __device__ __forceinline__ void first_blocks(const SampleArgs& a, int b, int e, int lane,
+                                                      BlkReg<NO>& f0, BlkReg<NO>& f1) {
+    load_blk64<NO>(a, b, e, lane, f0);
+    load_blk64<NO>(a, b + kBlk64, e, lane, f1);
+  }
+
+  __device__ __forceinline__ void run(const BlkReg<NO>& f0, const BlkReg<NO>& f1, Gram64& g) {
     if (nb == 0) return;
     float a0[16], a1[16], b0[16], b1[16];
-    BlkReg<NO> first;
-    load_blk64<NO>(a, beg, end, lane, first);
-    if (nb > 1) load_blk64<NO>(a, beg + kBlk64, end, lane, nxt);
-    stage_blk64<NO>(L, 0, lane, first);
+    nxt = f1;
+    stage_blk64<NO>(L, 0, lane, f0);
     lds_wait();
     gather_blk64<NO>(F, L, 0, half, comp, a0, a1);
     for (int k = 0; k < nb; k += 2) {
@@ -191,13 +188,12 @@ struct Fin64Lds {
 
 // Finish one row whose Gram is in g (row < 0: no row); returns this lane's
 // SSE share.  Lane i owns row i of P.
-__device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, Fin64Lds& s) {
+__device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float alpha, Fin64Lds& s) {
   int lane = threadIdx.x & 63;
   opaque(lane);
   const int half = lane >> 5, c32 = lane & 31;
   int i = lane;
   const bool active = row >= 0;
-  const float alpha = (float)(*a.alpha);
   // -P = -(Lambda + alpha G) in the accumulator layout: block (bi, bj), lane l,
   // register 4m + t is entry (32 bi + 8m + 4h + t, 32 bj + l%32)
 #pragma unroll
@@ -345,13 +341,35 @@ __global__ void __launch_bounds__(64, 2) k_light64(SampleArgs a, BatchPlan p) {
   __shared__ BlkLds64<NO> blk;
   const Factors<NO> F = make_factors<NO, 64>(a.others);
   const int it = *a.iter;
+  const float alpha = (float)(*a.alpha);
+  const int lane = threadIdx.x;
   double sse = 0.0;
-  for (int q = blockIdx.x; q < p.n_light; q += gridDim.x) {
-    const int row = p.light[q];
+  // rows q, q + G, ...: the next row's nnz range and first two index blocks
+  // are loaded while the current row is multiplied and finished
+  int q = blockIdx.x;
+  int4 rng = make_int4(0, 0, 0, 0);
+  int row = -1;
+  BlkReg<NO> f0, f1;
+  if (q < p.n_pair) {
+    rng = p.pair_rng[q];
+    row = p.pair_row[q].x;
+    GramStream64<NO>::first_blocks(a, rng.x, rng.y, lane, f0, f1);
+  }
+  for (; q < p.n_pair; q += gridDim.x) {
+    const int qn = q + gridDim.x;
+    int4 rng_n = make_int4(0, 0, 0, 0);
+    int row_n = -1;
+    if (qn < p.n_pair) {
+      rng_n = p.pair_rng[qn];
+      row_n = p.pair_row[qn].x;
+    }
     Gram64 g;
     zero_gram(g);
-    GramStream64<NO>(a, F, blk, a.rows.row_ptr[row], a.rows.row_ptr[row + 1]).run(g);
-    sse += (double)finish64(a, row, g, it, s);
+    GramStream64<NO>(a, F, blk, rng.x, rng.y).run(f0, f1, g);
+    if (qn < p.n_pair) GramStream64<NO>::first_blocks(a, rng_n.x, rng_n.y, lane, f0, f1);
+    sse += (double)finish64(a, row, g, it, alpha, s);
+    rng = rng_n;
+    row = row_n;
   }
   write_sse(a, sse, blockIdx.x);
 }
@@ -364,7 +382,9 @@ __global__ void __launch_bounds__(64, 2) k_seg64(SampleArgs a, BatchPlan p) {
   for (int sg = blockIdx.x; sg < p.n_seg; sg += gridDim.x) {
     Gram64 g;
     zero_gram(g);
-    GramStream64<NO>(a, F, blk, p.seg_beg[sg], p.seg_end[sg]).run(g);
+    BlkReg<NO> f0, f1;
+    GramStream64<NO>::first_blocks(a, p.seg_beg[sg], p.seg_end[sg], lane, f0, f1);
+    GramStream64<NO>(a, F, blk, p.seg_beg[sg], p.seg_end[sg]).run(f0, f1, g);
     float* out = p.partial + (size_t)sg * kPartial64;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -381,6 +401,7 @@ __global__ void __launch_bounds__(64, 2) k_heavy64(SampleArgs a, BatchPlan p, in
   __shared__ Fin64Lds s;
   const int lane = threadIdx.x;
   const int it = *a.iter;
+  const float alpha = (float)(*a.alpha);
   double sse = 0.0;
   for (int q = blockIdx.x; q < p.n_heavy; q += gridDim.x) {
     Gram64 g;
@@ -396,12 +417,12 @@ __global__ void __launch_bounds__(64, 2) k_heavy64(SampleArgs a, BatchPlan p, in
       g.b0 += in[3072 + lane];
       g.b1 += in[3136 + lane];
     }
-    sse += (double)finish64(a, p.heavy[q], g, it, s);
+    sse += (double)finish64(a, p.heavy[q], g, it, alpha, s);
   }
   write_sse(a, sse, sse_slot0 + blockIdx.x);
 }
 
-int batched64_light_grid(const BatchPlan& p) { return p.n_light < 8192 ? p.n_light : 8192; }
+int batched64_light_grid(const BatchPlan& p) { return p.n_pair < 8192 ? p.n_pair : 8192; }
 int batched64_heavy_grid(const BatchPlan& p) { return p.n_heavy < 2048 ? p.n_heavy : 2048; }
 
 template <int NO>
