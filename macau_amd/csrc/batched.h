@@ -81,6 +81,37 @@ __device__ __forceinline__ void load_blk(const SampleArgs& a, int e0, int end, i
   }
 }
 
+// Sums each heavy slot's segment partials (W floats per segment, stored W
+// apart) in segment order into the slot's first segment.  Thread (slot, q)
+// owns entry q of the slot, so each segment's loads are coalesced across the
+// work-group and the slots' sums run in parallel.
+template <int W>
+__global__ void __launch_bounds__(256) k_seg_sum(float* partial, const int* seg0, int nslot) {
+  constexpr int CH = (W + 255) / 256;  // work-groups per slot
+  const int slot = blockIdx.x / CH, q = (blockIdx.x % CH) * 256 + threadIdx.x;
+  if (slot >= nslot || q >= W) return;
+  const int s0 = seg0[slot], s1 = seg0[slot + 1];
+  float* base = partial + (size_t)s0 * W + q;
+  float acc = base[0];
+  int sg = s0 + 1;
+  for (; sg + 8 <= s1; sg += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = partial[(size_t)(sg + u) * W + q];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u];
+  }
+  for (; sg < s1; sg++) acc += partial[(size_t)sg * W + q];
+  base[0] = acc;
+}
+
+template <int W>
+inline void launch_seg_sum(const BatchPlan& p, hipStream_t s) {
+  constexpr int CH = (W + 255) / 256;
+  if (p.n_heavy > 0)
+    hipLaunchKernelGGL((k_seg_sum<W>), dim3(p.n_heavy * CH), dim3(256), 0, s, p.partial, p.heavy_seg0, p.n_heavy);
+}
+
 // wave-sum of the SSE shares into sse_partial[slot]
 __device__ inline void write_sse(const SampleArgs& a, double acc, int slot) {
   if (!a.sse_partial) return;

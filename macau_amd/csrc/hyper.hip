@@ -147,32 +147,42 @@ __global__ void __launch_bounds__(StatShape<KP>::THREADS)
   }
 }
 
-// partial[0][q] = sum_b partial[b][q] in a fixed order (four interleaved
-// strands of blocks combined in strand order).  Each work-group owns 64
-// entries q and reads every block's copy of them before it writes row 0.
+// partial[0][q] = sum_b partial[b][q] in a fixed order: 16 interleaved
+// strands of blocks (strand g takes b = g, g + 16, ..., eight loads in
+// flight), combined in strand order.  A work-group owns 16 entries q, so
+// each load instruction reads 16 consecutive entries of 4 blocks.
+constexpr int kRedStrands = 16, kRedEntries = 256 / kRedStrands;
+
 __global__ void __launch_bounds__(256) k_stats_reduce(double* partial, int nblk, int width) {
-  __shared__ double red[4][64];
-  const int e = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int q = blockIdx.x * 64 + e;
+  __shared__ double red[kRedStrands][kRedEntries];
+  const int e = threadIdx.x % kRedEntries, g = threadIdx.x / kRedEntries;
+  const int q = blockIdx.x * kRedEntries + e;
   double s = 0.0;
   if (q < width) {
     int b = g;
-    for (; b + 28 < nblk; b += 32) {
+    for (; b + 7 * kRedStrands < nblk; b += 8 * kRedStrands) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = partial[(size_t)(b + 4 * u) * width + q];
+      for (int u = 0; u < 8; u++) v[u] = partial[(size_t)(b + kRedStrands * u) * width + q];
 #pragma unroll
       for (int u = 0; u < 8; u++) s += v[u];
     }
-    for (; b < nblk; b += 4) s += partial[(size_t)b * width + q];
+    for (; b < nblk; b += kRedStrands) s += partial[(size_t)b * width + q];
   }
   red[g][e] = s;
   __syncthreads();
-  if (g == 0 && q < width) partial[q] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+  if (g == 0 && q < width) {
+    double t = red[0][e];
+#pragma unroll
+    for (int h = 1; h < kRedStrands; h++) t += red[h][e];
+    partial[q] = t;
+  }
 }
 
 void launch_partial_reduce(double* partial, int nblk, int width, hipStream_t s) {
-  if (nblk > 1) hipLaunchKernelGGL(k_stats_reduce, dim3((width + 63) / 64), dim3(256), 0, s, partial, nblk, width);
+  if (nblk > 1)
+    hipLaunchKernelGGL(k_stats_reduce, dim3((width + kRedEntries - 1) / kRedEntries), dim3(256), 0, s, partial, nblk,
+                       width);
 }
 
 template <typename T>
@@ -194,10 +204,7 @@ static void launch_stats(const T* U, const float* sub, int ld, int N, int K, dou
 #undef MGPU_STATS_CASE
     default: break;
   }
-  if (nblk > 1) {
-    const int width = K + K * K;
-    hipLaunchKernelGGL(k_stats_reduce, dim3((width + 63) / 64), dim3(256), 0, s, partial, nblk, width);
-  }
+  launch_partial_reduce(partial, nblk, K + K * K, s);
 }
 
 void launch_hyper_stats_f32(const float* U, const float* sub, int ld, int N, int K, double* partial, int nblk,

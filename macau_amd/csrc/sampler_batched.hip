@@ -9,8 +9,8 @@
 // Work decomposition of one half-sweep (BatchPlan, built on the host):
 //   * rows with more than 1024 nonzeros are split into 1024-nnz segments;
 //     k_seg32 computes each segment's partial Gram and RHS (one wave per
-//     segment), k_heavy32 sums the partials of two rows in segment order and
-//     finishes them;
+//     segment), k_seg_sum adds each row's partials in segment order (one
+//     thread per partial entry), and k_heavy32 finishes two rows per wave;
 //   * the other rows, sorted by nnz (descending), are taken two per wave by
 //     k_light32 (similar nnz in a pair keeps both halves busy).
 // Gram: each MFMA step takes 16 nonzeros (lanes 0-31 the first 8, lanes
@@ -417,14 +417,13 @@ __global__ void __launch_bounds__(64) k_seg32(SampleArgs a, BatchPlan p) {
   }
 }
 
-// sum of the segment partials of one heavy slot (fixed segment order)
-__device__ __forceinline__ void sum_segments(const BatchPlan& p, int slot, int lane, f32x16& acc, float& bp) {
-  for (int sg = p.heavy_seg0[slot]; sg < p.heavy_seg0[slot + 1]; sg++) {
-    const float* in = p.partial + (size_t)sg * kPartial;
+// the summed partial of one heavy slot (k_seg_sum leaves it in the slot's
+// first segment)
+__device__ __forceinline__ void slot_partial(const BatchPlan& p, int slot, int lane, f32x16& acc, float& bp) {
+  const float* in = p.partial + (size_t)p.heavy_seg0[slot] * kPartial;
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] += in[r * 64 + lane];
-    bp += in[1024 + lane];
-  }
+  for (int r = 0; r < 16; r++) acc[r] = in[r * 64 + lane];
+  bp = in[1024 + lane];
 }
 
 __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, int sse_slot0) {
@@ -444,8 +443,8 @@ __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, in
     float bpA = 0.f, bpB = 0.f;
     const int rowA = p.heavy[2 * q];
     const int rowB = (2 * q + 1 < p.n_heavy) ? p.heavy[2 * q + 1] : -1;
-    sum_segments(p, 2 * q, lane, accA, bpA);
-    if (rowB >= 0) sum_segments(p, 2 * q + 1, lane, accB, bpB);
+    slot_partial(p, 2 * q, lane, accA, bpA);
+    if (rowB >= 0) slot_partial(p, 2 * q + 1, lane, accB, bpB);
     sse += (double)finish_pair(a, rowA, rowB, accA, accB, bpA, bpB, it, alpha, s);
   }
   write_sse(a, sse, sse_slot0 + blockIdx.x);
@@ -467,6 +466,7 @@ static void launch_batched_t(const SampleArgs& a, const BatchPlan& p, hipStream_
   if (p.n_seg > 0) {
     const int gs = p.n_seg < 16384 ? p.n_seg : 16384;
     hipLaunchKernelGGL((k_seg32<NO>), dim3(gs), dim3(64), 0, s, a, p);
+    launch_seg_sum<kPartial>(p, s);
   }
   if (gl > 0) hipLaunchKernelGGL((k_light32<NO>), dim3(gl), dim3(64), 0, s, a, p);
   if (gh > 0) hipLaunchKernelGGL(k_heavy32, dim3(gh), dim3(64), 0, s, a, p, gl);

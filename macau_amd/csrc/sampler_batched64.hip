@@ -404,19 +404,17 @@ __global__ void __launch_bounds__(64, 2) k_heavy64(SampleArgs a, BatchPlan p, in
   const float alpha = (float)(*a.alpha);
   double sse = 0.0;
   for (int q = blockIdx.x; q < p.n_heavy; q += gridDim.x) {
+    // the slot's summed partial (k_seg_sum leaves it in the first segment)
     Gram64 g;
-    zero_gram(g);
-    for (int sg = p.heavy_seg0[q]; sg < p.heavy_seg0[q + 1]; sg++) {
-      const float* in = p.partial + (size_t)sg * kPartial64;
+    const float* in = p.partial + (size_t)p.heavy_seg0[q] * kPartial64;
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        g.g00[r] += in[r * 64 + lane];
-        g.g10[r] += in[1024 + r * 64 + lane];
-        g.g11[r] += in[2048 + r * 64 + lane];
-      }
-      g.b0 += in[3072 + lane];
-      g.b1 += in[3136 + lane];
+    for (int r = 0; r < 16; r++) {
+      g.g00[r] = in[r * 64 + lane];
+      g.g10[r] = in[1024 + r * 64 + lane];
+      g.g11[r] = in[2048 + r * 64 + lane];
     }
+    g.b0 = in[3072 + lane];
+    g.b1 = in[3136 + lane];
     sse += (double)finish64(a, p.heavy[q], g, it, alpha, s);
   }
   write_sse(a, sse, sse_slot0 + blockIdx.x);
@@ -431,6 +429,7 @@ static void launch64_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s) {
   if (p.n_seg > 0) {
     const int gs = p.n_seg < 16384 ? p.n_seg : 16384;
     hipLaunchKernelGGL((k_seg64<NO>), dim3(gs), dim3(64), 0, s, a, p);
+    launch_seg_sum<kPartial64>(p, s);
   }
   if (gl > 0) hipLaunchKernelGGL((k_light64<NO>), dim3(gl), dim3(64), 0, s, a, p);
   if (gh > 0) hipLaunchKernelGGL(k_heavy64, dim3(gh), dim3(64), 0, s, a, p, gl);
