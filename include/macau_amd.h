@@ -48,6 +48,15 @@ void macau_free(macau_model* m); /* replaces `del macau` (macau.pyx:449) */
 int macau_add_bpmf_prior(macau_model* m);
 int macau_add_macau_prior(macau_model* m, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
                           const double* vals /* NULL = binary */, double lambda_beta, double tol);
+/* make_prior(side, num_latent, max_ff_size, lambda_beta, tol) (macau.pyx:154-192):
+ * the same prior, and the link matrix is solved through the Cholesky factor of
+ * F^T F + lambda_beta I (compute_ff, latentprior.cpp:199-203, 280-289) when
+ * ncol <= max_ff_size, by block CG otherwise; macau_add_macau_prior always
+ * uses the block CG.  The Python layer passes max_ff_size = 10000 as
+ * setSidePriors does (macau.pyx:334). */
+int macau_add_macau_prior_ff(macau_model* m, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
+                             const double* vals /* NULL = binary */, double lambda_beta, double tol,
+                             int max_ff_size);
 
 /* ---- Macau API (macau.h:35-53; macau.pxd:63-84) ---- */
 int macau_set_samples(macau_model* m, int burnin, int nsamples);

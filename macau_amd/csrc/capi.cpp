@@ -87,6 +87,18 @@ int macau_add_macau_prior(macau_model* h, int nrow, int ncol, long long nnz, con
   API_END_INT
 }
 
+int macau_add_macau_prior_ff(macau_model* h, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
+                             const double* vals, double lambda_beta, double tol, int max_ff_size) {
+  API_BEGIN
+  std::vector<int> r(rows, rows + nnz), c(cols, cols + nnz);
+  std::vector<double> v;
+  if (vals) v.assign(vals, vals + nnz);
+  auto* p = new MacauPrior(nrow, ncol, r, c, v, vals == nullptr, lambda_beta, tol);
+  p->use_ftf = ncol <= max_ff_size;  // compute_ff (macau.pyx:160)
+  M(h).addPrior(std::unique_ptr<ILatentPrior>(p));
+  API_END_INT
+}
+
 int macau_set_samples(macau_model* h, int burnin, int nsamples) {
   API_BEGIN
   M(h).setSamples(burnin, nsamples);

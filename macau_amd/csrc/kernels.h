@@ -184,6 +184,18 @@ struct CsrDev {
   const int* multi_slot0 = nullptr;
   double* partial = nullptr;  // slots x nr
 };
+
+// --- F^T F branch of MacauPrior::sample_beta (latentprior.cpp:280-289) ---
+// fail-flag value set when F^T F + lambda_beta I is not positive definite
+constexpr int kFailDenseChol = 1 << 29;
+// F^T F (ncol x ncol, col-major, lower triangle; the upper is zeroed) of the
+// CSR F and its transpose Ft, in the reference's accumulation order
+void launch_ftf(const CsrDev& F, const CsrDev& Ft, double* FtF, hipStream_t s);
+// X (n x nrhs row-major, holding B on entry) <- (FtF + reg I)^{-1} X through
+// a blocked Cholesky in the n x n work matrix W; *reg is a device scalar;
+// a non-positive pivot sets *fail to kFailDenseChol
+void launch_ftf_solve(const double* FtF, int n, const double* reg, double* W, double* X, int nrhs, int* fail,
+                      hipStream_t s);
 constexpr int kSpmmSegNnz = 256;
 // out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr;
 // reg is a device scalar, read only when add is non-null; nothing is done

@@ -532,6 +532,9 @@ void Model::check_fail() {
   const int f = download_scalar<int>(d_fail, stream);
   if (f) {
     upload_scalar<int>(d_fail, 0, stream);
+    // chol_decomp's message for the F^T F branch (chol.cpp:21-28), the per-row
+    // sampler's otherwise (latentprior.cpp:375)
+    if (f == kFailDenseChol) throw std::runtime_error("c++ error: Cholesky decomp failed");
     throw std::runtime_error("Cholesky Decomposition failed!");
   }
 }
@@ -852,6 +855,12 @@ void MacauPrior::init(Model& m, int mode) {
   partial_sumsq.alloc(256 * sizeof(double));
   // block CG buffers sized for the largest block (<= 40 right-hand sides)
   cg.alloc(std::min(K, 40), nfeat, N);
+  if (use_ftf) {  // FtF = At_mul_A(F) once (latentprior.cpp:199-203)
+    const size_t nn = (size_t)nfeat * nfeat * sizeof(double);
+    FtF.alloc(nn);
+    Wchol.alloc(nn);
+    launch_ftf(F(), Ft(), FtF.as<double>(), m.stream);
+  }
   HIPCHK(hipStreamSynchronize(m.stream));
 }
 
@@ -940,17 +949,27 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   launch_beta_tmp(b, tmp.as<double>(), s);
   launch_spmm(Ft(), tmp.as<double>(), K, Fty.as<double>(), nullptr, nullptr, s);
   launch_beta_noise_add(b, Fty.as<double>(), s);
-  // solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (linop.h:206-226)
-  const int blocksize = 32, excess = 8;
-  if (K <= excess + blocksize) {
-    last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, 0, K, tol,
-                                   beta.as<double>(), cg, s);
+  if (use_ftf) {
+    // K = FtF + lambda_beta I; chol_decomp(K); chol_solve_t(K, Ft_y);
+    // beta = Ft_y (latentprior.cpp:280-288)
+    HIPCHK(hipMemcpyAsync(beta.as<void>(), Fty.as<void>(), (size_t)K * nfeat * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+    launch_ftf_solve(FtF.as<double>(), nfeat, lambda_beta_d.as<double>(), Wchol.as<double>(), beta.as<double>(), K,
+                     m.d_fail.as<int>(), s);
+    last_cg_iters = 0;
   } else {
-    for (int i = 0; i < K; i += blocksize) {
-      int nr = blocksize;
-      if (i + blocksize + excess >= K) nr = K - i;
-      last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, i, nr, tol,
+    // solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (linop.h:206-226)
+    const int blocksize = 32, excess = 8;
+    if (K <= excess + blocksize) {
+      last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, 0, K, tol,
                                      beta.as<double>(), cg, s);
+    } else {
+      for (int i = 0; i < K; i += blocksize) {
+        int nr = blocksize;
+        if (i + blocksize + excess >= K) nr = K - i;
+        last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, i, nr, tol,
+                                       beta.as<double>(), cg, s);
+      }
     }
   }
   // compute_uhat and the per-row prior term for the next half-sweep

@@ -112,6 +112,10 @@ class MacauPrior : public ILatentPrior {
   CgBuffers cg;
   int bbt_nblk = 0;
   int last_cg_iters = 0;
+  // F^T F branch of sample_beta (latentprior.cpp:280-289): set for side
+  // information with at most max_ff_size features (macau.pyx:160)
+  bool use_ftf = false;
+  DevBuf FtF, Wchol;  // F^T F (lower, FP64) and the per-iteration Cholesky work matrix
   CsrDev F() const;
   CsrDev Ft() const;
   // uploads F and F^T and builds their SpMM work lists (nrmax right-hand sides)

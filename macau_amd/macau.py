@@ -217,8 +217,10 @@ class Model:
             check(_lib.lib().macau_set_relation_data_test(self.h, ptr(te_idx), len(dims), ptr(te_vals),
                                                           te_idx.shape[0], ptr(dims)))
 
-    def add_prior(self, side, lambda_beta, tol):
-        """make_prior (macau.pyx:154-192)."""
+    def add_prior(self, side, lambda_beta, tol, max_ff_size=10000):
+        """make_prior (macau.pyx:154-192); side information with at most
+        max_ff_size columns solves the link matrix through F^T F (compute_ff,
+        macau.pyx:160), like setSidePriors (macau.pyx:334)."""
         L = _lib.lib()
         if side is None:
             check(L.macau_add_bpmf_prior(self.h))
@@ -237,8 +239,8 @@ class Model:
         r = np.ascontiguousarray(r, dtype=np.int32)
         c = np.ascontiguousarray(c, dtype=np.int32)
         v = np.ascontiguousarray(v, dtype=np.float64)
-        check(L.macau_add_macau_prior(self.h, side.shape[0], side.shape[1], len(v), ptr(r), ptr(c), ptr(v),
-                                      float(lambda_beta), float(tol)))
+        check(L.macau_add_macau_prior_ff(self.h, side.shape[0], side.shape[1], len(v), ptr(r), ptr(c), ptr(v),
+                                         float(lambda_beta), float(tol), int(max_ff_size)))
 
     def get_dims(self):
         d = np.zeros(self.nmodes, dtype=np.int32)

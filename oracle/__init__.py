@@ -58,7 +58,7 @@ class OrcRunCfg(ctypes.Structure):
         ("seed", _u64), ("max_iters", ctypes.c_int),
         ("pred", _p), ("pred_var", _p), ("U_out", _p * 8), ("rmse", _dbl), ("rmse1", _dbl),
         ("alpha", _dbl), ("alpha_trace", _p), ("rmse_trace", _p), ("lambda_beta_out", _p),
-        ("cg_iters_last", ctypes.c_int),
+        ("cg_iters_last", ctypes.c_int), ("use_ftf", ctypes.c_int * 8),
     ]
 
 
@@ -98,6 +98,9 @@ def lib():
                                           _dbl, ctypes.c_int, ctypes.c_int, _p]
         L.orc_posterior_lambda_beta.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _dbl, ctypes.POINTER(_dbl),
                                                 ctypes.POINTER(_dbl)]
+        L.orc_at_mul_a.argtypes = [ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _p]
+        L.orc_ftf_solve.restype = ctypes.c_int
+        L.orc_ftf_solve.argtypes = [ctypes.c_int, _i64, _p, _dbl, _p, _p]
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [ctypes.POINTER(OrcRunCfg)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -316,6 +319,26 @@ def blockcg(A, At, reg, B, tol, blocksize=None, excess=None):
     return X, it
 
 
+def at_mul_a(A, At):
+    """F^T F of a Csr (At_mul_A, linop.cpp:93-118): (ncol, ncol) with the lower
+    triangle filled and the upper triangle zero."""
+    out = np.zeros((A.ncol, A.ncol), order="F")
+    lib().orc_at_mul_a(ctypes.byref(A.c()), ctypes.byref(At.c()), _ptr(out))
+    return out
+
+
+def ftf_solve(FtF, reg, B):
+    """X (FtF + reg I) = B through Cholesky (latentprior.cpp:280-288); FtF is
+    (n, n) with the lower triangle used, B is (n, nrhs) C-order (= nrhs x n
+    col-major).  Raises when FtF + reg I is not positive definite."""
+    FtF = np.asfortranarray(FtF, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    X = np.zeros_like(B)
+    if lib().orc_ftf_solve(B.shape[1], B.shape[0], _ptr(FtF), reg, _ptr(B), _ptr(X)):
+        raise RuntimeError("Cholesky decomp failed")
+    return X
+
+
 def posterior_lambda_beta(beta, Lambda, nu, mu):
     """beta is (nfeat, K) C-order (K x nfeat col-major)."""
     beta = np.ascontiguousarray(beta, dtype=np.float64)
@@ -340,8 +363,9 @@ class RunResult:
 
 def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False, adaptive=False,
         precision=1.0, sn_init=1.0, sn_max=10.0, side=None, lambda_beta=5.0, tol=1e-6, burnin=10,
-        nsamples=10, seed=0, max_iters=0, want_U=False):
-    """Full Gibbs run (MacauX::run).  side: per-mode (F, Ft) Csr pairs or None."""
+        nsamples=10, seed=0, max_iters=0, want_U=False, use_ftf=None):
+    """Full Gibbs run (MacauX::run).  side: per-mode (F, Ft) Csr pairs or None;
+    use_ftf: per-mode flags selecting the F^T F branch of sample_beta."""
     M = len(dims)
     cfg = OrcRunCfg()
     cfg.nmodes, cfg.K, cfg.is_tensor = M, K, int(is_tensor)
@@ -361,6 +385,7 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
             cfg.macau[m] = 1
             cfg.F[m] = side[m][0].c()
             cfg.Ft[m] = side[m][1].c()
+            cfg.use_ftf[m] = int(bool(use_ftf and use_ftf[m]))
     cfg.lambda_beta0, cfg.tol = lambda_beta, tol
     cfg.burnin, cfg.nsamples, cfg.seed, cfg.max_iters = burnin, nsamples, seed, max_iters
     pred = np.zeros(len(tv))

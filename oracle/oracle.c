@@ -627,11 +627,52 @@ static void mvnormal_prec_cols(int K, const double* Lchol, int64_t n, uint64_t s
   }
 }
 
+/* At_mul_A(SparseDoubleFeat) (linop.cpp:93-118): for every feature f1, the
+ * rows of F holding it, and for each such row every feature f2 >= f1 of it */
+void orc_at_mul_a(const orc_csr* A, const orc_csr* At, double* out) {
+  const int64_t n = A->ncol;
+  memset(out, 0, sizeof(double) * (size_t)n * (size_t)n);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t f1 = 0; f1 < n; f1++) {
+    for (int64_t i = At->row_ptr[f1]; i < At->row_ptr[f1 + 1]; i++) {
+      const int64_t r = At->cols[i];
+      const double v1 = At->vals ? At->vals[i] : 1.0;
+      for (int64_t j = A->row_ptr[r]; j < A->row_ptr[r + 1]; j++) {
+        const int64_t f2 = A->cols[j];
+        if (f1 <= f2) out[f2 + (size_t)f1 * n] += (A->vals ? A->vals[j] : 1.0) * v1;
+      }
+    }
+  }
+}
+
+/* K = FtF (lower) + reg I; chol_decomp(K); chol_solve_t(K, B): every row of
+ * B (a latent component over the features) is solved against K */
+int orc_ftf_solve(int nrhs, int64_t n, const double* FtF, double reg, const double* B, double* X) {
+  double* L = (double*)malloc(sizeof(double) * (size_t)n * (size_t)n);
+  for (int64_t j = 0; j < n; j++)
+    for (int64_t i = 0; i < n; i++) L[i + (size_t)j * n] = i >= j ? FtF[i + (size_t)j * n] : 0.0;
+  for (int64_t i = 0; i < n; i++) L[i + (size_t)i * n] += reg;
+  if (orc_chol_lower((int)n, L)) {
+    free(L);
+    return -1;
+  }
+  double* x = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int k = 0; k < nrhs; k++) {
+    for (int64_t f = 0; f < n; f++) x[f] = B[k + (size_t)f * nrhs];
+    orc_trsv_lower((int)n, L, x);
+    orc_trsv_upper_t((int)n, L, x);
+    for (int64_t f = 0; f < n; f++) X[k + (size_t)f * nrhs] = x[f];
+  }
+  free(x);
+  free(L);
+  return 0;
+}
+
 /* MacauPrior::update_prior (latentprior.cpp:254-264) with sample_beta
- * (:272-293, block-CG branch) */
+ * (:272-293); FtF (non-null) selects the F^T F branch, else the block CG */
 static int macau_update_prior_t(int K, int mode, const double* U, int64_t N, const orc_csr* F,
-                                const orc_csr* Ft, const double* WI, int df, double* beta, double* Uhat,
-                                double* lambda_beta, double tol, double* mu, double* Lambda,
+                                const orc_csr* Ft, const double* FtF, const double* WI, int df, double* beta,
+                                double* Uhat, double* lambda_beta, double tol, double* mu, double* Lambda,
                                 uint64_t seed, uint32_t it, int* cg_iters) {
   const int64_t nfeat = F->ncol;
   const size_t KK = (size_t)K * K;
@@ -661,8 +702,14 @@ static int macau_update_prior_t(int K, int mode, const double* U, int64_t N, con
     const double sl = sqrt(*lambda_beta);
     for (size_t q = 0; q < (size_t)K * nfeat; q++) Fty[q] += sl * noise[q];
   }
-  /* solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (:291) */
-  *cg_iters = orc_blockcg_blocked(K, F, Ft, *lambda_beta, Fty, tol, 32, 8, beta);
+  if (FtF) {
+    /* K = FtF + lambda_beta I, chol_decomp, chol_solve_t (:280-288) */
+    if (orc_ftf_solve(K, nfeat, FtF, *lambda_beta, Fty, beta)) { rc = -11; goto out; }
+    *cg_iters = 0;
+  } else {
+    /* solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (:291) */
+    *cg_iters = orc_blockcg_blocked(K, F, Ft, *lambda_beta, Fty, tol, 32, 8, beta);
+  }
   /* compute_uhat: Uhat = beta F^T (linop.h:196-198) */
   orc_csr_A_mul_Bn(K, F, beta, Uhat);
   {
@@ -687,6 +734,7 @@ int orc_run(orc_run_cfg* c) {
   double* beta[8] = {0};
   double* Uhat[8] = {0};
   double* murows[8] = {0};
+  double* FtF[8] = {0};
   double lambda_beta[8];
   double* WI = (double*)calloc(KK, sizeof(double));
   int rc = 0;
@@ -702,6 +750,10 @@ int orc_run(orc_run_cfg* c) {
       Uhat[m] = (double*)calloc((size_t)K * c->dims[m], sizeof(double));
       murows[m] = (double*)malloc(sizeof(double) * (size_t)K * c->dims[m]);
       lambda_beta[m] = c->lambda_beta0;
+      if (c->use_ftf[m]) {  /* MacauPrior::init: At_mul_A(FtF, *F) (latentprior.cpp:200-203) */
+        FtF[m] = (double*)malloc(sizeof(double) * (size_t)c->F[m].ncol * (size_t)c->F[m].ncol);
+        orc_at_mul_a(&c->F[m], &c->Ft[m], FtF[m]);
+      }
     }
   }
   /* noise.init (noisemodels.cpp:14-53) */
@@ -737,7 +789,7 @@ int orc_run(orc_run_cfg* c) {
     for (int m = 0; m < M; m++) {
       if (c->macau[m]) {
         int cgit = 0;
-        rc = macau_update_prior_t(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], WI, K, beta[m], Uhat[m],
+        rc = macau_update_prior_t(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], FtF[m], WI, K, beta[m], Uhat[m],
                                   &lambda_beta[m], c->tol, mu[m], Lambda[m], c->seed, (uint32_t)i, &cgit);
         c->cg_iters_last = cgit;
       } else {
@@ -770,7 +822,7 @@ int orc_run(orc_run_cfg* c) {
   }
 done:
   for (int m = 0; m < M; m++) {
-    free(U[m]); free(mu[m]); free(Lambda[m]); free(beta[m]); free(Uhat[m]); free(murows[m]);
+    free(U[m]); free(mu[m]); free(Lambda[m]); free(beta[m]); free(Uhat[m]); free(murows[m]); free(FtF[m]);
   }
   free(WI);
   return rc;

@@ -119,6 +119,14 @@ int orc_blockcg_blocked(int nrhs, const orc_csr* A, const orc_csr* At, double re
                         double tol, int blocksize, int excess, double* X);
 void orc_posterior_lambda_beta(int K, int64_t nfeat, const double* beta, const double* Lambda, double nu,
                                double mu, double* shape, double* scale);
+/* F^T F of a CSR F (At_mul_A(SparseDoubleFeat), linop.cpp:93-118): out is
+ * ncol x ncol col-major, lower triangle (the upper stays zero) */
+void orc_at_mul_a(const orc_csr* A, const orc_csr* At, double* out);
+/* FtF branch of sample_beta (latentprior.cpp:280-289): X (nrhs x n,
+ * col-major K x n) = B (FtF + reg I)^{-1}, through the Cholesky factor of
+ * FtF (lower) + reg I and chol_solve_t (chol.cpp:47-53).  Returns -1 when
+ * the matrix is not positive definite (chol_decomp throws). */
+int orc_ftf_solve(int nrhs, int64_t n, const double* FtF, double reg, const double* B, double* X);
 
 /* ---- a4: full Gibbs run (macau.cpp:72-86, 94-148) ---- */
 typedef struct {
@@ -147,6 +155,8 @@ typedef struct {
   double*  rmse_trace;     /* per-iteration posterior-mean RMSE, or NULL     */
   double*  lambda_beta_out;/* per mode, or NULL                              */
   int      cg_iters_last;
+  /* input: per mode, solve beta through F^T F (1) or the block CG (0) */
+  int      use_ftf[8];
 } orc_run_cfg;
 int orc_run(orc_run_cfg* cfg);
 

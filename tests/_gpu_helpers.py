@@ -124,7 +124,9 @@ class GpuModel:
                                                  ptr(np.ascontiguousarray(test_vals, dtype=np.float64)),
                                                  ti.shape[0], ptr(d)))
 
-    def prior(self, side=None, lambda_beta=5.0, tol=1e-6):
+    def prior(self, side=None, lambda_beta=5.0, tol=1e-6, max_ff_size=None):
+        """max_ff_size None: block CG always (macau_add_macau_prior); else the
+        F^T F branch for side information with at most that many columns."""
         L = _lib.lib()
         if side is None:
             check(L.macau_add_bpmf_prior(self.h))
@@ -133,7 +135,11 @@ class GpuModel:
             r = np.ascontiguousarray(r, dtype=np.int32)
             c = np.ascontiguousarray(c, dtype=np.int32)
             v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
-            check(L.macau_add_macau_prior(self.h, nr, nc, len(r), ptr(r), ptr(c), ptr(v), lambda_beta, tol))
+            if max_ff_size is None:
+                check(L.macau_add_macau_prior(self.h, nr, nc, len(r), ptr(r), ptr(c), ptr(v), lambda_beta, tol))
+            else:
+                check(L.macau_add_macau_prior_ff(self.h, nr, nc, len(r), ptr(r), ptr(c), ptr(v), lambda_beta, tol,
+                                                 max_ff_size))
 
     def setup(self, burnin, nsamples, seed, sampler, graph=True):
         L = _lib.lib()

@@ -47,6 +47,22 @@ def reference_kat():
               [1.69333333, -0.12709677, -1.94666667, 0.49483871],
               [0.66, -0.04064516, -0.78, 0.65225806]],
     }
+    # tests.cpp:16-29  At_mul_A on the binary CSR (lower triangle entries)
+    k["At_mul_A_bin"] = {"lower": {"0,0": 2, "1,1": 3, "2,2": 2, "3,3": 2, "1,0": 0, "2,0": 2, "3,0": 0,
+                                   "2,1": 0, "3,1": 1, "3,2": 0}}
+    # tests.cpp:31-58  At_mul_A on the valued CSR (Approx entries)
+    k["At_mul_A_csr"] = {"lower": {"0,0": 4.3801, "1,1": 2.4485, "2,2": 8.6420, "3,3": 5.9572, "1,0": 0,
+                                   "2,0": 3.8282, "3,0": 0, "2,1": 0, "3,1": 0.0714, "3,2": 0}}
+    # tests.cpp:209-233  chol_decomp + chol_solve_t on a 3 x 3 lower triangle, 5 rhs
+    k["chol_solve_t"] = {
+        "m": [[7, 0, 0], [2, 5, 0], [6, 1, 6]],
+        "rhs": [[-1.227, -0.890, 0.293], [0.356, -0.733, -1.201], [-0.003, -0.091, -1.467],
+                [0.819, 0.725, -0.719], [-0.485, 0.955, 1.707]],
+        "xopt": [[-1.67161, 0.151609, 1.69517], [2.10217, -0.545174, -2.21148], [1.80587, -0.34187, -1.99339],
+                 [1.71883, -0.180826, -1.80852], [-2.93874, 0.746739, 3.09878]],
+    }
+    # tests.cpp:859-885  make_dense_prior: FtF = lower(F^T F) of a 3 x 2 F
+    k["dense_prior_ftf"] = {"F": [[0.1, 0.3], [0.4, 0.11], [-0.7, 0.23]]}
     # tests.cpp:241-254  posterior_lambda_beta(beta, Lambda_u, nu = 0.01, mu = 0.05)
     k["posterior_lambda_beta"] = {"beta": [[3.0, -2.0, 0.5], [1.0, 0.91, -0.2]],
                                   "Lambda": [[0.5, 0.1], [0.1, 0.3]], "nu": 0.01, "mu": 0.05,

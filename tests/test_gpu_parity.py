@@ -251,6 +251,46 @@ def test_model_macau_matches_oracle():
     assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
 
 
+@pytest.mark.parametrize("nfeat,binary", [(70, True), (130, False), (33, False)])
+def test_model_macau_ftf_matches_oracle(nfeat, binary):
+    """The F^T F branch of sample_beta (latentprior.cpp:280-289): F^T F on the
+    GPU, blocked Cholesky of F^T F + lambda_beta I, two blocked solves; feature
+    counts that leave a partial last 32-column block."""
+    announce("model macau F^T F branch nfeat=%d binary=%d" % (nfeat, binary))
+    rng = np.random.default_rng(nfeat)
+    nrow, ncol, K = 300, 90, 8
+    Fm = rng.random((nrow, nfeat)) < 0.08
+    fr, fc = np.nonzero(Fm)
+    fv = np.ones(len(fr)) if binary else rng.standard_normal(len(fr))
+    Ut = rng.standard_normal((nrow, 3))
+    Vt = rng.standard_normal((ncol, 3))
+    r, c = np.nonzero(rng.random((nrow, ncol)) < 0.2)
+    y = np.sum(Ut[r] * Vt[c], 1) + 0.3 * rng.standard_normal(len(r))
+    perm = rng.permutation(len(r))
+    te, tr = perm[:len(r) // 5], perm[len(r) // 5:]
+    m = GpuModel(2, K, 10.0)
+    m.data(np.stack([r[tr], c[tr]], 1), y[tr], [nrow, ncol], np.stack([r[te], c[te]], 1), y[te])
+    m.prior(side=(fr, fc, None if binary else fv, nrow, nfeat), lambda_beta=5.0, tol=1e-6, max_ff_size=10000)
+    m.prior()
+    m.setup(4, 6, SEED, PLAIN, graph=False)
+    modes, mean, nnz = O.matrix_rows(r[tr], c[tr], y[tr], nrow, ncol)
+    ti, tv = O.matrix_test(r[te], c[te], y[te], nrow, ncol)
+    side = [O.csr_pair(fr, fc, fv, nrow, nfeat), None]
+    kw = dict(precision=10.0, side=side, lambda_beta=5.0, tol=1e-6, burnin=4, nsamples=6, seed=SEED,
+              use_ftf=[True, False])
+    m.iterate(2)
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, max_iters=2, want_U=True, **kw)
+    e0, e1 = row_rel_err(m.latents(0, nrow), ref.U[0]), row_rel_err(m.latents(1, ncol), ref.U[1])
+    lb = _lib.lib().macau_get_link_lambda(m.h, 0)
+    print("F^T F model iter 2: row err %.3e %.3e, lambda_beta %.6f vs %.6f" % (e0, e1, lb, ref.lambda_beta[0]))
+    assert e0 < TOL_FP32 and e1 < TOL_FP32
+    assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < TOL_FP32
+    assert _lib.lib().macau_get_last_cg_iters(m.h, 0) == 0
+    m.iterate(8)
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, **kw)
+    assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
+
+
 # ------------------------------------------------------------ batched sampler
 BATCHED = _lib.SAMPLER_BATCHED
 # Split-BF16 Gram (hi*hi + hi*lo + lo*hi, FP32 accumulate) keeps ~16 bits per

@@ -135,3 +135,34 @@ def test_empty_test_set_rmse_is_nan():
     r1, ra = O.eval_rmse(2, np.zeros((0, 2), dtype=np.int32), np.zeros(0), 0.0,
                          [np.ones((1, 2)), np.ones((1, 2))], 0, pred, var)
     assert np.isnan(r1) and np.isnan(ra)
+
+
+def _check_lower(AA, entries):
+    for key, v in entries.items():
+        i, j = (int(x) for x in key.split(","))
+        assert AA[i, j] == pytest.approx(v, abs=1e-4), key
+
+
+def test_at_mul_a_binary(kat):  # tests.cpp:16-29
+    F, Ft = _feat(kat, True)
+    _check_lower(O.at_mul_a(F, Ft), kat["At_mul_A_bin"]["lower"])
+
+
+def test_at_mul_a_csr(kat):  # tests.cpp:31-58
+    F, Ft = _feat(kat, False)
+    AA = O.at_mul_a(F, Ft)
+    _check_lower(AA, kat["At_mul_A_csr"]["lower"])
+    assert np.all(np.triu(AA, 1) == 0.0)
+
+
+def test_chol_solve_t(kat):  # tests.cpp:209-233
+    c = kat["chol_solve_t"]
+    X = O.ftf_solve(np.array(c["m"], dtype=float), 0.0, np.array(c["rhs"]).T)
+    np.testing.assert_allclose(X.T, np.array(c["xopt"]), rtol=1e-4, atol=1e-5)
+
+
+def test_dense_prior_ftf(kat):  # tests.cpp:859-885
+    F = np.array(kat["dense_prior_ftf"]["F"])
+    r, c = np.nonzero(np.ones_like(F))
+    A, At = O.csr_pair(r, c, F[r, c], F.shape[0], F.shape[1])
+    np.testing.assert_allclose(O.at_mul_a(A, At), np.tril(F.T @ F), atol=1e-15)

@@ -292,3 +292,44 @@ def test_full_run_quality_macau():
                 seed=4)
     assert res.rmse < 0.6
     assert res.lambda_beta[0] > 0
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 3), (45, 32), (130, 40)])
+def test_ftf_solve_matches_dense_solve(n, nrhs):
+    """F^T F branch of sample_beta: Cholesky of the lower triangle + reg I."""
+    rng = np.random.default_rng(n)
+    F = scipy.sparse.random(3 * n + 5, n, density=0.2, random_state=n, format="coo")
+    A, At = O.csr_pair(F.row, F.col, F.data, F.shape[0], n)
+    FtF = O.at_mul_a(A, At)
+    Fd = F.toarray()
+    np.testing.assert_allclose(FtF, np.tril(Fd.T @ Fd), rtol=1e-13, atol=1e-13)
+    B = rng.standard_normal((n, nrhs))
+    reg = 0.7
+    X = O.ftf_solve(FtF, reg, B)
+    np.testing.assert_allclose(X, np.linalg.solve(Fd.T @ Fd + reg * np.eye(n), B), rtol=1e-9, atol=1e-11)
+
+
+def test_ftf_solve_not_positive_definite():
+    FtF = np.array([[1.0, 0.0], [3.0, 1.0]])  # lower triangle of an indefinite matrix
+    with pytest.raises(RuntimeError):
+        O.ftf_solve(FtF, 0.0, np.ones((2, 1)))
+
+
+def test_full_run_ftf_branch_matches_cg_branch():
+    """Both branches of sample_beta solve the same system; with a tight CG
+    tolerance the first iterations agree."""
+    rng = np.random.default_rng(12)
+    nrow, ncol, K, nfeat = 80, 30, 4, 25
+    Fd = (rng.random((nrow, nfeat)) < 0.2).astype(float) * rng.standard_normal((nrow, nfeat))
+    r, c = np.nonzero(rng.random((nrow, ncol)) < 0.3)
+    y = rng.standard_normal(len(r))
+    modes, mean, nnz = O.matrix_rows(r, c, y, nrow, ncol)
+    ti, tv = O.matrix_test(r[:10], c[:10], y[:10], nrow, ncol)
+    fr, fc = np.nonzero(Fd)
+    side = [O.csr_pair(fr, fc, Fd[fr, fc], nrow, nfeat), None]
+    kw = dict(precision=5.0, side=side, burnin=2, nsamples=2, seed=5, max_iters=3, want_U=True, tol=1e-12)
+    a = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, use_ftf=[True, False], **kw)
+    b = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, **kw)
+    assert a.cg_iters_last == 0 and b.cg_iters_last > 0
+    np.testing.assert_allclose(a.U[0], b.U[0], rtol=1e-6, atol=1e-8)
+    assert a.lambda_beta[0] == pytest.approx(b.lambda_beta[0], rel=1e-6)
