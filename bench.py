@@ -23,6 +23,10 @@ Other workloads (`--workload`, one JSON line each, not the headline):
   c3  Macau with 100,000-dim sparse binary row features (block CG for the
       link matrix every iteration); reports the CG iterations per solve and
       the achieved HBM bandwidth of the row prior update;
+  c3f the same data with 4,000-dim row features, for which the link matrix
+      takes the F^T F Cholesky branch (side information with at most 10,000
+      features, as the Python API chooses); reports the FP64 rate of the
+      per-iteration factorisation;
   c4  3-mode tensor 50,000 x 2,000 x 100, 10M cells, K = 64.
 """
 import argparse
@@ -43,6 +47,8 @@ METRIC = "Gibbs iters/sec, batched-mixed vs plain FP32, 20M nnz K=32, RMSE≤+0.
 PEAK_TFLOPS = {"plain": 157.3, "batched": 2500.0}  # FP32 vector/MFMA; BF16 dense MFMA
 UNIT_OF = {"plain": "f32", "batched": "bf16x3-f32acc"}
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+MAX_FF_SIZE = 10000  # side information with at most this many features uses F^T F (macau.pyx:334)
 
 
 def load_workload(name, seed):
@@ -56,11 +62,13 @@ def load_workload(name, seed):
                     side=[None, None],
                     label="BPMF 138000x27000, 20M observed (18M train / 2M test), Zipf-1.1 degrees, K=32, "
                           "adaptive noise")
-    if name == "c3":
-        (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c3(seed)
+    if name in ("c3", "c3f"):
+        nfeat = 100000 if name == "c3" else 4000
+        (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c3(seed, nfeat=nfeat)
+        how = "CG tol 1e-6" if nfeat > MAX_FF_SIZE else "F^T F Cholesky"
         return dict(idx=idx, vals=vals, tidx=tidx, tvals=tvals, dims=dims, settings=settings, side=side,
-                    label="Macau 15000x350, 60k observed, 100000-dim binary row features (~%d nnz), K=32, "
-                          "lambda_beta 5, CG tol 1e-6" % side[0].nnz)
+                    label="Macau 15000x350, 60k observed, %d-dim binary row features (~%d nnz), K=32, "
+                          "lambda_beta 5, %s" % (nfeat, side[0].nnz, how))
     if name == "c4":
         (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c4(seed)
         return dict(idx=idx, vals=vals, tidx=tidx, tvals=tvals, dims=dims, settings=settings, side=side,
@@ -94,8 +102,8 @@ def build_model(w, sampler_kind, seed, graph=True):
             r = np.ascontiguousarray(s.row, dtype=np.int32)
             c = np.ascontiguousarray(s.col, dtype=np.int32)
             v = np.ascontiguousarray(s.data, dtype=np.float64)
-            check(L.macau_add_macau_prior(h, s.shape[0], s.shape[1], len(v), ptr(r), ptr(c), ptr(v),
-                                          float(st["lambda_beta"]), float(st["tol"])))
+            check(L.macau_add_macau_prior_ff(h, s.shape[0], s.shape[1], len(v), ptr(r), ptr(c), ptr(v),
+                                             float(st["lambda_beta"]), float(st["tol"]), MAX_FF_SIZE))
     check(L.macau_set_samples(h, st["burnin"], st["nsamples"]))
     check(L.macau_set_seed(h, seed))
     check(L.macau_set_sampler(h, sampler_kind))
@@ -203,7 +211,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c4"], default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
@@ -274,7 +282,15 @@ def main():
         "phase_ms": phase_ms,
         "roofline": roof,
     }
-    if w["side"][0] is not None:
+    if w["side"][0] is not None and w["side"][0].shape[1] <= MAX_FF_SIZE:
+        # the row prior update is dominated by the Cholesky of F^T F + lambda_beta I
+        nf = w["side"][0].shape[1]
+        tf = (nf ** 3 / 3.0) / (ph[nm] * 1e-3) / 1e12
+        out["ftf_cholesky"] = {"features": nf, "prior_rows_ms": float(ph[nm]), "factor_flops": nf ** 3 / 3.0,
+                               "achieved_TFLOPs_lower_bound": tf, "frac_of_fp64_peak": tf / PEAK_FP64_TFLOPS,
+                               "note": "time includes F^T F + lambda I, the two K-column solves, the beta noise "
+                                       "draw, Uhat and lambda_beta updates"}
+    elif w["side"][0] is not None:
         # the row prior update is dominated by the block CG for the link matrix
         iters = res[head]["cg"][0]
         cgb = cg_bytes_per_iteration(w) * max(iters, 1)
