@@ -38,7 +38,9 @@ const char* macau_version(void);
  *      (macau.h:83-85, macau.cpp:207-228; macau.pxd:82-84) ---- */
 macau_model* macau_make_fixed(int nmodes, int num_latent, double precision);
 macau_model* macau_make_adaptive(int nmodes, int num_latent, double sn_init, double sn_max);
-/* probit noise is not provided by this library: returns NULL with an error */
+/* ProbitNoise (noisemodels.h:88-103): matrix data (nmodes = 2); with more
+ * modes macau_init / macau_run fail like the reference's sampler and
+ * evaluation (latentprior.cpp:112-116, noisemodels.cpp:177-181) */
 macau_model* macau_make_probit(int nmodes, int num_latent);
 void macau_free(macau_model* m); /* replaces `del macau` (macau.pyx:449) */
 
@@ -88,8 +90,13 @@ int macau_get_iteration(macau_model* m);
  * stream; phase_ms[2 * nmodes + 2] receives the mean ms of: sampling of each
  * mode, the prior update of each mode, the noise update, the evaluation */
 int macau_profile(macau_model* m, int n, double* phase_ms);
+/* RMSE of the one-sample / averaged predictions; for probit models the
+ * predictions are probabilities and these are root Brier scores */
 double macau_get_rmse_onesample(macau_model* m);
 double macau_get_rmse_avg(macau_model* m);
+/* probit models: test AUC of the one-sample and averaged predictions
+ * (ProbitNoise::auc_test_onesample / auc_test, noisemodels.cpp:156-157) */
+int macau_get_auc(macau_model* m, double* auc_onesample, double* auc_avg);
 double macau_get_noise_precision(macau_model* m);
 double macau_get_mean_value(macau_model* m);
 int macau_get_dims(macau_model* m, int* dims);
@@ -132,6 +139,10 @@ int macau_rng_probe(unsigned long long seed, unsigned c0, unsigned iter, unsigne
                     float* z32, double* z64);
 int macau_gamma_probe(unsigned long long seed, const double* shapes, int n, unsigned iter, unsigned tag,
                       double* out);
+/* out[j] = rand_truncnorm(mean[j], 1, 0) (truncnorm.cpp:9-37) keyed by
+ * (c0 + j, iter, tag); inv[j] = inv_norm_cdf(p[j]) */
+int macau_truncnorm_probe(unsigned long long seed, unsigned c0, unsigned iter, unsigned tag, int n,
+                          const double* mean, const double* p, double* out, double* inv);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,7 @@ SIGNATURES = {
     "macau_profile": (_i, [_p, _i, _p]),
     "macau_get_rmse_onesample": (_d, [_p]),
     "macau_get_rmse_avg": (_d, [_p]),
+    "macau_get_auc": (_i, [_p, _p, _p]),
     "macau_get_noise_precision": (_d, [_p]),
     "macau_get_mean_value": (_d, [_p]),
     "macau_get_dims": (_i, [_p, _p]),
@@ -68,6 +69,7 @@ SIGNATURES = {
     "macau_solve_blockcg": (_i, [_i, _i, _i, _ll, _p, _p, _p, _d, _p, _d, _p, _p]),
     "macau_rng_probe": (_i, [_ull, _u, _u, _u, _i, _p, _p, _p]),
     "macau_gamma_probe": (_i, [_ull, _p, _i, _u, _u, _p]),
+    "macau_truncnorm_probe": (_i, [_ull, _u, _u, _u, _i, _p, _p, _p, _p]),
 }
 
 

@@ -64,9 +64,12 @@ macau_model* macau_make_adaptive(int nmodes, int K, double sn_init, double sn_ma
   API_END_VAL(nullptr)
 }
 
-macau_model* macau_make_probit(int, int) {
-  g_err = "probit noise is not supported by macau_amd";
-  return nullptr;
+macau_model* macau_make_probit(int nmodes, int K) {
+  API_BEGIN
+  auto* h = new macau_model;
+  h->m.reset(new Model(nmodes, K, std::unique_ptr<INoiseModel>(new ProbitNoise())));
+  return h;
+  API_END_VAL(nullptr)
 }
 
 void macau_free(macau_model* h) { delete h; }
@@ -205,6 +208,19 @@ double macau_get_rmse_avg(macau_model* h) {
   return M(h).host_eval_out(1);
   API_END_VAL(NAN)
 }
+int macau_get_auc(macau_model* h, double* auc_onesample, double* auc_avg) {
+  API_BEGIN
+  auto* p = dynamic_cast<ProbitNoise*>(M(h).noise.get());
+  if (!p) throw std::runtime_error("AUC is reported by probit models only");
+  if (!M(h).initialized) throw std::runtime_error("model not initialized");
+  M(h).sync();
+  M(h).check_fail();
+  p->sync_host(M(h));
+  if (auc_onesample) *auc_onesample = p->auc_test_onesample;
+  if (auc_avg) *auc_avg = p->auc_test;
+  API_END_INT
+}
+
 double macau_get_noise_precision(macau_model* h) {
   API_BEGIN
   return M(h).noise_precision();
@@ -490,6 +506,20 @@ int macau_rng_probe(unsigned long long seed, unsigned c0, unsigned iter, unsigne
   HIPCHK(hipMemcpy(bits, db.as<void>(), (size_t)n * 4, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(z32, d32.as<void>(), (size_t)n * 4, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(z64, d64.as<void>(), (size_t)n * 8, hipMemcpyDeviceToHost));
+  API_END_INT
+}
+
+int macau_truncnorm_probe(unsigned long long seed, unsigned c0, unsigned iter, unsigned tag, int n,
+                          const double* mean, const double* p, double* out, double* inv) {
+  API_BEGIN
+  DevBuf dm((size_t)n * 8), dp((size_t)n * 8), d((size_t)n * 8), di((size_t)n * 8);
+  HIPCHK(hipMemcpy(dm.as<void>(), mean, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dp.as<void>(), p, (size_t)n * 8, hipMemcpyHostToDevice));
+  launch_truncnorm_probe(n, dm.as<double>(), dp.as<double>(), seed, c0, iter, tag, d.as<double>(),
+                         di.as<double>(), nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, d.as<void>(), (size_t)n * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(inv, di.as<void>(), (size_t)n * 8, hipMemcpyDeviceToHost));
   API_END_INT
 }
 

@@ -768,7 +768,8 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
 #pragma unroll
     for (int off = 1; off < G; off <<= 1) dot += __shfl_xor(dot, off, G);
     if (sub == 0 && q < a.ntest) {
-      const double p = a.mean + (double)dot;
+      const double p = a.probit ? 0.5 * erfc(-(double)dot * M_SQRT1_2) : a.mean + (double)dot;
+      if (a.probit) a.pred1[q] = p;
       double pa;
       if (n == 0) {
         pa = p;

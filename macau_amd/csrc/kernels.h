@@ -153,11 +153,33 @@ struct EvalArgs {
   const int* iter = nullptr;
   int burnin = 0;
   double* out = nullptr;         // (rmse1, rmse_avg)
+  // probit (ProbitNoise::evalModel, noisemodels.cpp:129-159): predictions
+  // Phi(u.v) without the mean; pred1 receives the one-sample predictions
+  int probit = 0;
+  double* pred1 = nullptr;
 };
 int eval_blocks(int ntest);
 void launch_eval(const EvalArgs& a, hipStream_t s);
 
 void launch_advance_iter(int* iter, hipStream_t s);
+
+// --- probit augmentation (latentprior.cpp:387-414) ---
+struct ProbitZArgs {
+  long long nnz = 0;
+  int KP = 0;
+  const int* row_of = nullptr;  // nnz: row of each observation (mode order)
+  const int* idx = nullptr;     // nnz: other-mode index
+  const double* y = nullptr;    // nnz: raw values
+  const float* U = nullptr;     // this mode's factor, KP x N
+  const float* V = nullptr;     // other mode's factor
+  float* z = nullptr;           // nnz out
+  uint64_t seed = 0;
+  const int* iter = nullptr;
+  int mode = 0;
+};
+void launch_probit_z(const ProbitZArgs& a, hipStream_t s);
+void launch_truncnorm_probe(int n, const double* mean, const double* p, uint64_t seed, uint32_t c0, uint32_t it,
+                            uint32_t tag, double* out, double* inv, hipStream_t s);
 
 // --- RNG probes (parity tests) ---
 void launch_rng_probe(uint64_t seed, uint32_t c0, uint32_t it, uint32_t tag, int n, uint32_t* bits, float* z32,

@@ -456,6 +456,7 @@ void Model::step_launch() {
   // predictions are updated).  The streams join before the iteration
   // counter advances, so the next iteration sees every update.
   for (int m = 0; m < nmodes; m++) {
+    noise->augment(*this, m);
     priors[m]->sample_latents(*this, m);
     HIPCHK(hipEventRecord(ev_fork[m], stream));
     HIPCHK(hipStreamWaitEvent(side, ev_fork[m], 0));
@@ -500,6 +501,7 @@ std::vector<double> Model::profile(int n) {
     int q = 0;
     HIPCHK(hipEventRecord(E[q++], stream));
     for (int m = 0; m < nmodes; m++) {
+      noise->augment(*this, m);
       priors[m]->sample_latents(*this, m);
       HIPCHK(hipEventRecord(E[q++], stream));
     }
@@ -607,6 +609,7 @@ void Model::run() {
   check_fail();
   noise->rmse_test_onesample = host_eval_out(0);
   noise->rmse_test = host_eval_out(1);
+  noise->sync_host(*this);
   rmse_test = noise->getEvalMetric();
 }
 
@@ -1009,7 +1012,7 @@ std::string INoiseModel::getEvalString() {  // noisemodels.h:46
          to_string_with_precision(rmse_test_onesample, 5) + ")";
 }
 
-void INoiseModel::evalModel(Model& m) {
+static EvalArgs eval_args(Model& m) {
   EvalArgs a;
   a.ntest = (int)m.ntest;
   a.nmodes = m.nmodes;
@@ -1026,8 +1029,10 @@ void INoiseModel::evalModel(Model& m) {
   a.iter = m.d_iter.as<int>();
   a.burnin = m.burnin;
   a.out = m.d_eval_out.as<double>();
-  launch_eval(a, m.stream);
+  return a;
 }
+
+void INoiseModel::evalModel(Model& m) { launch_eval(eval_args(m), m.stream); }
 
 void FixedGaussianNoise::init(Model& m) { upload_scalar<double>(m.d_alpha, alpha, m.stream); }
 
@@ -1083,5 +1088,100 @@ std::string AdaptiveGaussianNoise::getInitStatus() {
 std::string AdaptiveGaussianNoise::getStatus() { return std::string("Prec:") + to_string_with_precision(alpha, 2); }
 
 void AdaptiveGaussianNoise::sync_host(Model& m) { alpha = m.noise_precision(); }
+
+// ------------------------------------------------------------------
+// ProbitNoise
+// ------------------------------------------------------------------
+void ProbitNoise::init(Model& m) {
+  if (m.nmodes != 2) {
+    // BPMFPrior::sample_latents(ProbitNoise&, TensorData&) throws
+    // (latentprior.cpp:112-116); with side information the sampler does
+    // nothing and ProbitNoise::evalModel(TensorData) throws
+    // (noisemodels.cpp:177-181)
+    for (auto& p : m.priors)
+      if (!dynamic_cast<MacauPrior*>(p.get())) throw std::runtime_error("Unimplemented: sample_latents");
+    throw std::runtime_error("ProbitNoise::evalModel unimplemented.");
+  }
+  upload_scalar<double>(m.d_alpha, 1.0, m.stream);
+  d_y.clear();
+  d_row_of.clear();
+  d_z.clear();
+  for (int q = 0; q < m.nmodes; q++) {
+    const HostRows& h = m.rows[q];
+    std::vector<int> row_of(h.vals.size());
+    for (int n = 0; n < m.dims[q]; n++)
+      for (int e = h.row_ptr[n]; e < h.row_ptr[n + 1]; e++) row_of[e] = n;
+    d_y.emplace_back();
+    d_row_of.emplace_back();
+    upload(d_y.back(), h.vals, m.stream);
+    upload(d_row_of.back(), row_of, m.stream);
+    d_z.emplace_back(std::max<size_t>(h.vals.size(), 1) * sizeof(float));
+    m.rdev[q].r = d_z.back().as<float>();  // the samplers read z in place of y - mean
+  }
+  d_pred1.alloc(std::max<long long>(m.ntest, 1) * sizeof(double));
+  HIPCHK(hipStreamSynchronize(m.stream));
+}
+
+void ProbitNoise::augment(Model& m, int mode) {
+  ProbitZArgs a;
+  a.nnz = (long long)m.rows[mode].vals.size();
+  a.KP = m.KP;
+  a.row_of = d_row_of[mode].as<int>();
+  a.idx = m.d_idx[mode].as<int>();
+  a.y = d_y[mode].as<double>();
+  a.U = m.d_U[mode].as<float>();
+  a.V = m.d_U[1 - mode].as<float>();
+  a.z = d_z[mode].as<float>();
+  a.seed = m.seed;
+  a.iter = m.d_iter.as<int>();
+  a.mode = mode;
+  launch_probit_z(a, m.stream);
+}
+
+void ProbitNoise::evalModel(Model& m) {
+  EvalArgs a = eval_args(m);
+  a.probit = 1;
+  a.pred1 = d_pred1.as<double>();
+  launch_eval(a, m.stream);
+}
+
+std::string ProbitNoise::getEvalString() {  // noisemodels.h:100
+  return std::string("AUC: ") + to_string_with_precision(auc_test, 5) + " (1samp: " +
+         to_string_with_precision(auc_test_onesample, 5) + ")";
+}
+
+void ProbitNoise::sync_host(Model& m) {
+  std::vector<double> pred(m.ntest), pred1(m.ntest);
+  if (m.ntest) {
+    HIPCHK(hipMemcpyAsync(pred.data(), m.d_pred.as<void>(), m.ntest * sizeof(double), hipMemcpyDeviceToHost,
+                          m.stream));
+    HIPCHK(hipMemcpyAsync(pred1.data(), d_pred1.as<void>(), m.ntest * sizeof(double), hipMemcpyDeviceToHost,
+                          m.stream));
+    HIPCHK(hipStreamSynchronize(m.stream));
+  }
+  auc_test_onesample = auc(pred1, m.test_vals);
+  auc_test = auc(pred, m.test_vals);
+}
+
+double auc(const std::vector<double>& pred, const std::vector<double>& test) {
+  const size_t n = pred.size();
+  if (n == 0) return NAN;
+  std::vector<size_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](size_t i, size_t j) { return pred[i] < pred[j]; });
+  double np = 0.0;
+  for (double t : test) np += t;
+  const double nn = (double)n - np;
+  // x: positives so far, y: negatives so far, along ascending predictions;
+  // each step adds (x_{i+1} - x_i) y_{i+1}
+  double x = test[perm[0]], y = 1.0 - x, area = 0.0;
+  for (size_t i = 1; i < n; i++) {
+    const double x1 = x + test[perm[i]], y1 = y + 1.0 - test[perm[i]];
+    area += (x1 - x) * y1;
+    x = x1;
+    y = y1;
+  }
+  return area / (np * nn);
+}
 
 }  // namespace mgpu

@@ -136,6 +136,8 @@ class INoiseModel {
   virtual std::string getInitStatus() = 0;
   virtual std::string getStatus() = 0;
   virtual void sync_host(Model&) {}
+  // enqueued before mode `mode` is sampled (probit data augmentation)
+  virtual void augment(Model&, int) {}
   double rmse_test = NAN, rmse_test_onesample = NAN;
 };
 
@@ -161,6 +163,28 @@ class AdaptiveGaussianNoise : public INoiseModel {
   double sn_init, sn_max;
   double alpha = NAN, alpha_max = NAN, var_total = NAN;
 };
+
+// ProbitNoise (noisemodels.h:88-103, noisemodels.cpp:129-159, matrix data):
+// precision 1, observations replaced by z before each mode is sampled;
+// the test AUCs are computed on the host from the device predictions when
+// sync_host is called.
+class ProbitNoise : public INoiseModel {
+ public:
+  void init(Model& m) override;
+  void augment(Model& m, int mode) override;
+  void evalModel(Model& m) override;
+  double getEvalMetric() override { return auc_test; }
+  std::string getEvalString() override;
+  std::string getInitStatus() override { return "Probit noise model"; }
+  std::string getStatus() override { return ""; }
+  void sync_host(Model& m) override;
+  double auc_test = NAN, auc_test_onesample = NAN;
+  std::vector<DevBuf> d_y, d_row_of, d_z;  // per mode
+  DevBuf d_pred1;
+};
+
+// auc (bpmfutils.h:182-213): pred sorted ascending, ties by index
+double auc(const std::vector<double>& pred, const std::vector<double>& test);
 
 // MacauX<DType, NType> (macau.h:56-81, macau.cpp)
 class Model {

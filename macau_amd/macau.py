@@ -11,10 +11,11 @@ Differences from the reference, all additive:
     (macau.cpp:73); ``seed=None`` keeps that behaviour.
   * ``sampler=`` keyword: ``"auto"`` (default), ``"batched"`` (split-BF16
     MFMA Gram, nnz-bucketed) or ``"plain"`` (FP32, one work-group per row).
-  * sparse side information with <= 10000 features is solved with the block
-    CG instead of the dense F'F Cholesky branch (latentprior.cpp:280-289);
-    both solve the same system (F'F + lambda I) beta' = Ft_y.
-Not provided: ``univariate=True`` (MacauOnePrior) and ``precision="probit"``.
+Side information with at most 10000 features solves the link matrix through
+the dense F'F Cholesky branch (latentprior.cpp:280-289), larger side
+information through the block CG, as in the reference.  ``precision="probit"``
+runs ProbitNoise on matrix data.
+Not provided: ``univariate=True`` (MacauOnePrior).
 """
 import ctypes
 import numbers
@@ -188,7 +189,7 @@ class Model:
             if precision == "adaptive" or precision == "sample":
                 h = L.macau_make_adaptive(nmodes, num_latent, 1.0, float(sn_max))
             elif precision == "probit":
-                raise NotImplementedError("probit noise is not provided by macau_amd")
+                h = L.macau_make_probit(nmodes, num_latent)
             else:
                 raise ValueError("Parameter 'precision' has to be either a number or \"adaptive\" for adaptive "
                                  "precision, or \"probit\" for binary matrices.")
@@ -364,8 +365,12 @@ def macau(Y,
     df["y_pred_std"] = pd.Series(yhat_sd)
 
     result = MacauResult()
-    result.rmse_test = L.macau_get_rmse_test(model.h)
-    result.auc_test = np.nan
+    if isinstance(precision, str) and precision == "probit":
+        result.rmse_test = np.nan
+        result.auc_test = L.macau_get_rmse_test(model.h)
+    else:
+        result.rmse_test = L.macau_get_rmse_test(model.h)
+        result.auc_test = np.nan
     result.Yshape = data.shape
     result.ntrain = data.valTrain.shape[0]
     result.ntest = data.valTest.shape[0] if Ytest is not None else 0

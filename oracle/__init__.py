@@ -26,6 +26,7 @@ _LIB = None
 
 TAG_LATENT, TAG_WISHART_Z, TAG_WISHART_CHI, TAG_NW_MU = 1, 2, 3, 4
 TAG_NOISE, TAG_BETA_U, TAG_BETA_F, TAG_LAMBDA_BETA = 5, 6, 7, 8
+TAG_PROBIT = 9
 
 
 def tag(stream, mode=0):
@@ -59,6 +60,7 @@ class OrcRunCfg(ctypes.Structure):
         ("pred", _p), ("pred_var", _p), ("U_out", _p * 8), ("rmse", _dbl), ("rmse1", _dbl),
         ("alpha", _dbl), ("alpha_trace", _p), ("rmse_trace", _p), ("lambda_beta_out", _p),
         ("cg_iters_last", ctypes.c_int), ("use_ftf", ctypes.c_int * 8),
+        ("noise_probit", ctypes.c_int), ("auc", _dbl), ("auc1", _dbl),
     ]
 
 
@@ -101,6 +103,17 @@ def lib():
         L.orc_at_mul_a.argtypes = [ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _p]
         L.orc_ftf_solve.restype = ctypes.c_int
         L.orc_ftf_solve.argtypes = [ctypes.c_int, _i64, _p, _dbl, _p, _p]
+        L.orc_norm_cdf.restype = _dbl
+        L.orc_norm_cdf.argtypes = [_dbl]
+        L.orc_inv_norm_cdf.restype = _dbl
+        L.orc_inv_norm_cdf.argtypes = [_dbl]
+        L.orc_truncnorm0.restype = _dbl
+        L.orc_truncnorm0.argtypes = [_dbl, _u64, _u32, _u32, _u32]
+        L.orc_probit_z.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _p, _p, _u64, _u32, ctypes.c_int, _p]
+        L.orc_auc.restype = _dbl
+        L.orc_auc.argtypes = [_i64, _p, _p]
+        L.orc_eval_probit.argtypes = [ctypes.c_int, _i64, _p, _p, _p, ctypes.c_int, _p, _p, _p,
+                                      ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [ctypes.POINTER(OrcRunCfg)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -292,6 +305,55 @@ def eval_rmse(K, test_idx, test_vals, mean, factors, n, pred, pred_var):
     return r1.value, ra.value
 
 
+def norm_cdf(x):
+    """Phi(x) (truncnorm.cpp:5-7)."""
+    return lib().orc_norm_cdf(float(x))
+
+
+def inv_norm_cdf(p):
+    """Phi^{-1}(p) (the reference's inv_norm_cdf, truncnorm.cpp / tests.cpp:560-568)."""
+    return lib().orc_inv_norm_cdf(float(p))
+
+
+def truncnorm0(mean, seed, c0, it, tg):
+    """One draw of rand_truncnorm(mean, 1, 0) (truncnorm.cpp:9-37) keyed by
+    Philox block (c0, attempt, it, tg)."""
+    return lib().orc_truncnorm0(float(mean), seed, c0, it, tg)
+
+
+def probit_z(K, rows, others, U, seed, it, mode):
+    """Augmented observations z (mode order) for probit sampling of one mode
+    (latentprior.cpp:393-398); U is this mode's current factor (N, K)."""
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in others]
+    arr = (ctypes.c_void_p * len(fs))(*[_ptr(f) for f in fs])
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    z = np.zeros(int(rows.row_ptr[-1]) + 1)
+    lib().orc_probit_z(K, ctypes.byref(rows.c()), ctypes.cast(arr, ctypes.c_void_p), _ptr(U), seed, it, mode,
+                       _ptr(z))
+    return z[:-1]
+
+
+def auc(pred, test):
+    """auc (bpmfutils.h:182-213)."""
+    pred = np.ascontiguousarray(pred, dtype=np.float64)
+    test = np.ascontiguousarray(test, dtype=np.float64)
+    return lib().orc_auc(len(pred), _ptr(pred), _ptr(test))
+
+
+def eval_probit(K, test_idx, test_vals, factors, n, pred, pred_var):
+    """ProbitNoise::evalModel (noisemodels.cpp:129-159); returns (one-sample
+    predictions, auc of them, auc of the running mean)."""
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    arr = (ctypes.c_void_p * len(fs))(*[_ptr(f) for f in fs])
+    ti = np.ascontiguousarray(test_idx, dtype=np.int32)
+    tv = np.ascontiguousarray(test_vals, dtype=np.float64)
+    p1 = np.zeros(len(tv) + 1)
+    a1, aa = _dbl(), _dbl()
+    lib().orc_eval_probit(K, len(tv), _ptr(ti), _ptr(tv), ctypes.cast(arr, ctypes.c_void_p), n, _ptr(pred),
+                          _ptr(pred_var), _ptr(p1), ctypes.byref(a1), ctypes.byref(aa))
+    return p1[:-1], a1.value, aa.value
+
+
 def csr_A_mul_Bn(A, B):
     """out[r] = sum_c A[r,c] B[c]; B is (ncol, nrhs) C-order."""
     B = np.ascontiguousarray(B, dtype=np.float64)
@@ -363,9 +425,10 @@ class RunResult:
 
 def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False, adaptive=False,
         precision=1.0, sn_init=1.0, sn_max=10.0, side=None, lambda_beta=5.0, tol=1e-6, burnin=10,
-        nsamples=10, seed=0, max_iters=0, want_U=False, use_ftf=None):
+        nsamples=10, seed=0, max_iters=0, want_U=False, use_ftf=None, probit=False):
     """Full Gibbs run (MacauX::run).  side: per-mode (F, Ft) Csr pairs or None;
-    use_ftf: per-mode flags selecting the F^T F branch of sample_beta."""
+    use_ftf: per-mode flags selecting the F^T F branch of sample_beta;
+    probit: ProbitNoise (matrix data), reporting auc / auc1."""
     M = len(dims)
     cfg = OrcRunCfg()
     cfg.nmodes, cfg.K, cfg.is_tensor = M, K, int(is_tensor)
@@ -379,6 +442,7 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
     keep += [ti, tv]
     cfg.ntest, cfg.test_idx, cfg.test_vals = len(tv), _ptr(ti), _ptr(tv)
     cfg.noise_adaptive, cfg.precision, cfg.sn_init, cfg.sn_max = int(adaptive), precision, sn_init, sn_max
+    cfg.noise_probit = int(bool(probit))
     side = side or [None] * M
     for m in range(M):
         if side[m] is not None:
@@ -407,6 +471,7 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
         raise RuntimeError("oracle run failed (%d)" % rc)
     res = RunResult()
     res.rmse, res.rmse1, res.alpha = cfg.rmse, cfg.rmse1, cfg.alpha
+    res.auc, res.auc1 = cfg.auc, cfg.auc1
     res.pred, res.pred_var = pred, pvar
     res.alpha_trace, res.rmse_trace = at[:total], rt[:total]
     res.U = Us

@@ -723,6 +723,130 @@ out:
 }
 
 /* ------------------------------------------------------------------ */
+/* probit noise                                                             */
+/* ------------------------------------------------------------------ */
+double orc_norm_cdf(double x) { return 0.5 * erfc(-x * M_SQRT1_2); }
+
+double orc_inv_norm_cdf(double p) {
+  if (!(p > 0.0)) return -INFINITY;
+  if (!(p < 1.0)) return INFINITY;
+  const double q = p < 0.5 ? p : 1.0 - p;  /* exact for p >= 0.5 */
+  /* A&S 26.2.23: upper-tail quantile of q, |error| < 4.5e-4 */
+  const double t = sqrt(-2.0 * log(q));
+  double y = -(t - (2.515517 + t * (0.802853 + t * 0.010328)) / (1.0 + t * (1.432788 + t * (0.189269 + t * 0.001308))));
+  /* Halley on f(y) = Phi(y) - q (y <= 0); for subnormal q, 1 / phi(y)
+   * would overflow and the starting value is kept */
+  for (int k = 0; k < 3 && q >= 2.2250738585072014e-308; k++) {
+    const double e = orc_norm_cdf(y) - q;
+    const double u = e * 2.5066282746310002 * exp(0.5 * y * y);  /* e / phi(y) */
+    y -= u / (1.0 + 0.5 * y * u);
+  }
+  return p < 0.5 ? y : -y;
+}
+
+/* (k + 0.5) / 2^53 for the 53-bit integer k of two Philox words */
+static double ud53(uint32_t a, uint32_t b) {
+  const uint64_t k = ((uint64_t)(a >> 6) << 27) | (uint64_t)(b >> 5);
+  return ((double)k + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+double orc_truncnorm0(double mean, uint64_t seed, uint32_t c0, uint32_t it, uint32_t tag) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  const double abar = -mean;  /* (low_cut - mean) / std with low_cut 0, std 1 */
+  uint32_t w[4];
+  if (abar > 3.0) {
+    /* rand_truncnorm_rej (truncnorm.cpp:14-24).  Its acceptance test
+     * v <= xbar / abar holds for every draw (xbar >= abar), so the first
+     * proposal is returned: xbar has the law P(xbar > x) = exp(-(x^2 -
+     * abar^2) / 2), as in the reference */
+    for (uint32_t t = 0; t < 64; t++) {
+      const uint32_t ctr[4] = {c0, t, it, tag};
+      orc_philox4x32_10(ctr, key, w);
+      const double u = ud53(w[0], w[1]), v = ud53(w[2], w[3]);
+      const double xbar = sqrt(abar * abar - 2.0 * log(1.0 - u));
+      if (v <= xbar / abar) return xbar + mean;
+    }
+    return abar + mean;
+  }
+  /* rand_truncnorm_icdf (truncnorm.cpp:9-12): U(Phi(abar), 1) through Phi^{-1} */
+  const uint32_t ctr[4] = {c0, 0u, it, tag};
+  orc_philox4x32_10(ctr, key, w);
+  const double lo = orc_norm_cdf(abar);
+  return orc_inv_norm_cdf(lo + (1.0 - lo) * ud53(w[0], w[1])) + mean;
+}
+
+void orc_probit_z(int K, const orc_rows* rows, const double* const* others, const double* U, uint64_t seed,
+                  uint32_t it, int mode, double* z) {
+  const int no = rows->nother;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t n = 0; n < rows->nrows; n++) {
+    const double* u = U + (size_t)n * K;
+    for (int64_t e = rows->row_ptr[n]; e < rows->row_ptr[n + 1]; e++) {
+      double dot = 0.0;
+      for (int k = 0; k < K; k++) {
+        double c = others[0][(size_t)rows->idx[e * no] * K + k];
+        for (int t = 1; t < no; t++) c *= others[t][(size_t)rows->idx[e * no + t] * K + k];
+        dot += c * u[k];
+      }
+      const double y = 2.0 * rows->vals[e] - 1.0;
+      z[e] = y * orc_truncnorm0(y * dot, seed, (uint32_t)e, it, ORC_TAG(ORC_TAG_PROBIT, mode));
+    }
+  }
+}
+
+static const double* auc_key;
+static int auc_cmp(const void* a, const void* b) {
+  const int64_t i = *(const int64_t*)a, j = *(const int64_t*)b;
+  if (auc_key[i] < auc_key[j]) return -1;
+  if (auc_key[i] > auc_key[j]) return 1;
+  return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+double orc_auc(int64_t n, const double* pred, const double* test) {
+  if (n == 0) return NAN;
+  int64_t* perm = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+  for (int64_t i = 0; i < n; i++) perm[i] = i;
+  auc_key = pred;
+  qsort(perm, (size_t)n, sizeof(int64_t), auc_cmp);
+  double NP = 0.0;
+  for (int64_t i = 0; i < n; i++) NP += test[i];
+  const double NN = (double)n - NP;
+  /* stack_x = running positives, stack_y = running negatives along the sort */
+  double sx = test[perm[0]], sy = 1.0 - sx, auc = 0.0;
+  for (int64_t i = 1; i < n; i++) {
+    const double x1 = sx + test[perm[i]], y1 = sy + 1.0 - test[perm[i]];
+    auc += (x1 - sx) * y1;
+    sx = x1;
+    sy = y1;
+  }
+  free(perm);
+  return auc / (NP * NN);
+}
+
+void orc_eval_probit(int K, int64_t ntest, const int32_t* test_idx, const double* test_vals, const double* const* U,
+                     int n, double* pred, double* pred_var, double* pred1, double* auc1, double* auc_avg) {
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < ntest; t++) {
+    const int32_t* ix = test_idx + t * 2;
+    double d = 0.0;
+    for (int k = 0; k < K; k++) d += U[0][(size_t)ix[0] * K + k] * U[1][(size_t)ix[1] * K + k];
+    const double p = orc_norm_cdf(d);
+    double pa;
+    if (n == 0) {
+      pa = p;
+    } else {
+      const double delta = p - pred[t];
+      pa = pred[t] + delta / (n + 1);
+      pred_var[t] += delta * (p - pa);
+    }
+    pred1[t] = p;
+    pred[t] = pa;
+  }
+  *auc1 = orc_auc(ntest, pred1, test_vals);
+  *auc_avg = orc_auc(ntest, pred, test_vals);
+}
+
+/* ------------------------------------------------------------------ */
 /* a4 Gibbs driver (MacauX::init / run, macau.cpp:72-86, 94-148)            */
 /* ------------------------------------------------------------------ */
 int orc_run(orc_run_cfg* c) {
@@ -770,6 +894,20 @@ int orc_run(orc_run_cfg* c) {
   for (int64_t t = 0; t < c->ntest; t++) { c->pred[t] = 0.0; c->pred_var[t] = 0.0; }
   c->rmse = NAN;
   c->rmse1 = NAN;
+  c->auc = NAN;
+  c->auc1 = NAN;
+  /* probit: the samplers see precision 1 and the augmented z as the
+   * observations, with no mean (latentprior.cpp:387-414) */
+  double* zbuf = NULL;
+  double* pred1 = NULL;
+  if (c->noise_probit) {
+    if (M != 2) { rc = -3; goto done; }  /* "Unimplemented: sample_latents" (latentprior.cpp:112-116) */
+    int64_t mx = 0;
+    for (int m = 0; m < M; m++) mx = c->train[m].row_ptr[c->train[m].nrows] > mx ? c->train[m].row_ptr[c->train[m].nrows] : mx;
+    zbuf = (double*)malloc(sizeof(double) * ((size_t)mx + 1));
+    pred1 = (double*)malloc(sizeof(double) * ((size_t)c->ntest + 1));
+    alpha = 1.0;
+  }
   int total = c->burnin + c->nsamples;
   if (c->max_iters > 0 && c->max_iters < total) total = c->max_iters;
   for (int i = 0; i < total; i++) {
@@ -782,8 +920,17 @@ int orc_run(orc_run_cfg* c) {
         for (int64_t n = 0; n < c->dims[m]; n++)
           for (int k = 0; k < K; k++) murows[m][(size_t)n * K + k] = mu[m][k] + Uhat[m][(size_t)n * K + k];
       }
-      int64_t f = orc_sample_latents(K, &c->train[m], c->mean, others, alpha, mu[m], Lambda[m],
-                                     c->macau[m] ? murows[m] : NULL, c->seed, (uint32_t)i, m, U[m]);
+      int64_t f;
+      if (c->noise_probit) {
+        orc_probit_z(K, &c->train[m], others, U[m], c->seed, (uint32_t)i, m, zbuf);
+        orc_rows zr = c->train[m];
+        zr.vals = zbuf;
+        f = orc_sample_latents(K, &zr, 0.0, others, 1.0, mu[m], Lambda[m], c->macau[m] ? murows[m] : NULL, c->seed,
+                               (uint32_t)i, m, U[m]);
+      } else {
+        f = orc_sample_latents(K, &c->train[m], c->mean, others, alpha, mu[m], Lambda[m],
+                               c->macau[m] ? murows[m] : NULL, c->seed, (uint32_t)i, m, U[m]);
+      }
       if (f) { rc = -1; goto done; }
     }
     for (int m = 0; m < M; m++) {
@@ -800,7 +947,7 @@ int orc_run(orc_run_cfg* c) {
       }
       if (rc) { rc = -2; goto done; }
     }
-    if (c->noise_adaptive) {
+    if (c->noise_adaptive && !c->noise_probit) {
       const double* Us[8];
       for (int m = 0; m < M; m++) Us[m] = U[m];
       double sse = orc_train_sse(K, M, &c->train[0], c->mean, Us);
@@ -809,11 +956,15 @@ int orc_run(orc_run_cfg* c) {
     {
       const double* Us[8];
       for (int m = 0; m < M; m++) Us[m] = U[m];
-      orc_eval_rmse(K, M, c->ntest, c->test_idx, c->test_vals, c->mean, Us, i < c->burnin ? 0 : i - c->burnin,
-                    c->pred, c->pred_var, &c->rmse1, &c->rmse);
+      if (c->noise_probit)
+        orc_eval_probit(K, c->ntest, c->test_idx, c->test_vals, Us, i < c->burnin ? 0 : i - c->burnin, c->pred,
+                        c->pred_var, pred1, &c->auc1, &c->auc);
+      else
+        orc_eval_rmse(K, M, c->ntest, c->test_idx, c->test_vals, c->mean, Us, i < c->burnin ? 0 : i - c->burnin,
+                      c->pred, c->pred_var, &c->rmse1, &c->rmse);
     }
     if (c->alpha_trace) c->alpha_trace[i] = alpha;
-    if (c->rmse_trace) c->rmse_trace[i] = c->rmse;
+    if (c->rmse_trace) c->rmse_trace[i] = c->noise_probit ? c->auc : c->rmse;
   }
   c->alpha = alpha;
   for (int m = 0; m < M; m++) {
@@ -825,6 +976,8 @@ done:
     free(U[m]); free(mu[m]); free(Lambda[m]); free(beta[m]); free(Uhat[m]); free(murows[m]); free(FtF[m]);
   }
   free(WI);
+  free(zbuf);
+  free(pred1);
   return rc;
 }
 

@@ -40,6 +40,7 @@ extern "C" {
 #define ORC_TAG_BETA_U      6u  /* c0 = col,  j = component (sample_beta)   */
 #define ORC_TAG_BETA_F      7u  /* c0 = feat, j = component (sample_beta)   */
 #define ORC_TAG_LAMBDA_BETA 8u  /* c0 = 0,    gamma for lambda_beta         */
+#define ORC_TAG_PROBIT      9u  /* c0 = observation (mode order), c1 = attempt */
 #define ORC_TAG(stream, mode) (((uint32_t)(stream) << 24) | (uint32_t)(mode))
 
 /* ---- RNG (mvnormal.cpp:59-166 replaced by counter-based draws) ---- */
@@ -128,6 +129,30 @@ void orc_at_mul_a(const orc_csr* A, const orc_csr* At, double* out);
  * the matrix is not positive definite (chol_decomp throws). */
 int orc_ftf_solve(int nrhs, int64_t n, const double* FtF, double reg, const double* B, double* X);
 
+/* ---- probit noise (latentprior.cpp:387-414, truncnorm.cpp, noisemodels.cpp:129-159) ---- */
+/* Phi(x) = 0.5 erfc(-x / sqrt 2) (truncnorm.cpp:5-7) */
+double orc_norm_cdf(double x);
+/* Phi^{-1}(p): Abramowitz & Stegun 26.2.23 as the start, three Halley steps
+ * on the lower-tail form Phi(y) = min(p, 1 - p); -inf / +inf at 0 / 1 */
+double orc_inv_norm_cdf(double p);
+/* rand_truncnorm(mean, 1, 0) (truncnorm.cpp:9-37): inverse-CDF sampling
+ * for a cut abar = -mean <= 3, the reference's tail sampler above (whose
+ * acceptance test always passes, so its proposal law is kept), with 53-bit
+ * uniforms from Philox block (c0, attempt, it, tag) */
+double orc_truncnorm0(double mean, uint64_t seed, uint32_t c0, uint32_t it, uint32_t tag);
+/* z_e = y (rand_truncnorm(y u_n . v_e, 1, 0)), y = 2 value - 1, for every
+ * observation e of the rows (mode-order index e) given the current factors
+ * U (K x N, this mode) and the other mode's (latentprior.cpp:393-398) */
+void orc_probit_z(int K, const orc_rows* rows, const double* const* others, const double* U, uint64_t seed,
+                  uint32_t it, int mode, double* z);
+/* auc (bpmfutils.h:182-213); ties in pred ordered by index */
+double orc_auc(int64_t n, const double* pred, const double* test);
+/* ProbitNoise::evalModel (noisemodels.cpp:129-159): Phi(u . v), Welford
+ * running mean over samples, AUC of both; pred1 receives the one-sample
+ * predictions */
+void orc_eval_probit(int K, int64_t ntest, const int32_t* test_idx, const double* test_vals, const double* const* U,
+                     int n, double* pred, double* pred_var, double* pred1, double* auc1, double* auc_avg);
+
 /* ---- a4: full Gibbs run (macau.cpp:72-86, 94-148) ---- */
 typedef struct {
   int      nmodes, K, is_tensor;
@@ -157,6 +182,10 @@ typedef struct {
   int      cg_iters_last;
   /* input: per mode, solve beta through F^T F (1) or the block CG (0) */
   int      use_ftf[8];
+  /* input: probit noise (matrix data; overrides precision / adaptive) */
+  int      noise_probit;
+  /* outputs of probit runs (rmse / rmse1 stay NaN) */
+  double   auc, auc1;
 } orc_run_cfg;
 int orc_run(orc_run_cfg* cfg);
 

@@ -103,6 +103,8 @@ class GpuModel:
         L = _lib.lib()
         if precision == "adaptive":
             self.h = L.macau_make_adaptive(nmodes, K, 1.0, sn_max)
+        elif precision == "probit":
+            self.h = L.macau_make_probit(nmodes, K)
         else:
             self.h = L.macau_make_fixed(nmodes, K, float(precision))
         assert self.h, _lib.last_error()
@@ -171,6 +173,12 @@ class GpuModel:
 
     def rmse(self):
         return _lib.lib().macau_get_rmse_avg(self.h)
+
+    def auc(self):
+        """(one-sample AUC, AUC of the averaged predictions) of a probit model."""
+        a1, aa = ctypes.c_double(), ctypes.c_double()
+        check(_lib.lib().macau_get_auc(self.h, ctypes.byref(a1), ctypes.byref(aa)))
+        return a1.value, aa.value
 
     def link(self, mode, nfeat):
         b = np.zeros((nfeat, self.K))

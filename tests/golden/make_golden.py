@@ -63,6 +63,17 @@ def reference_kat():
     }
     # tests.cpp:859-885  make_dense_prior: FtF = lower(F^T F) of a 3 x 2 F
     k["dense_prior_ftf"] = {"F": [[0.1, 0.3], [0.4, 0.11], [-0.7, 0.23]]}
+    # tests.cpp:606-614  auc of descending predictions
+    k["auc"] = {"test": [1, 0, 1, 0, 1, 0, 0, 1, 0, 1] + [0] * 10,
+                "pred": [20.0 - i for i in range(20)], "auc": 0.84}
+    # tests.cpp:887-892  inv_norm_cdf (p, expected); p = 0 gives -inf
+    k["inv_norm_cdf"] = [[0.5, 0.0], [0.9, 1.2815515655446004], [0.01, -2.3263478740408408]]
+    # tests.cpp:894-899  norm_cdf (x, expected)
+    k["norm_cdf"] = [[0.0, 0.5], [-1.0, 0.15865525393145707], [-3.0, 0.0013498980316300933],
+                     [4.0, 0.99996832875816688]]
+    # tests.cpp:901-910  rand_truncnorm(mean, std, low) >= low, 10 draws each
+    k["rand_truncnorm_bounds"] = [[0.0, 1.0, 2.0], [0.0, 1.0, 3.0], [0.0, 1.0, 5.0], [0.0, 1.0, 50.0],
+                                  [30.0, 2.0, 50.0]]
     # tests.cpp:241-254  posterior_lambda_beta(beta, Lambda_u, nu = 0.01, mu = 0.05)
     k["posterior_lambda_beta"] = {"beta": [[3.0, -2.0, 0.5], [1.0, 0.91, -0.2]],
                                   "Lambda": [[0.5, 0.1], [0.1, 0.3]], "nu": 0.01, "mu": 0.05,

@@ -38,7 +38,12 @@ def test_version_and_error_string():
     assert isinstance(_lib.last_error(), str)
 
 
-def test_probit_factory_reports_error():
+def test_probit_factory():
+    """make_macau_probit (macau.cpp:207-211): a handle for valid sizes (the
+    constructor makes no device calls), NULL with an error otherwise."""
     L = _lib.lib()
-    assert not L.macau_make_probit(2, 4)
-    assert "probit" in _lib.last_error()
+    h = L.macau_make_probit(2, 4)
+    assert h
+    L.macau_free(h)
+    assert not L.macau_make_probit(2, 0)
+    assert "num_latent" in _lib.last_error()

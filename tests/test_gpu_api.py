@@ -1,9 +1,9 @@
 """The Python entry points on the GPU, mirroring the reference's own tests
 (python/macau/test_macau.py).  Every case runs the full Gibbs chain through
 the HIP library; quality bounds are the reference's (tensor RMSE < 0.5 at
-precision 50, test_macau.py:132-207).  Cases the reference runs with probit
-noise or the univariate sampler check that those raise NotImplementedError,
-since this library does not provide them."""
+precision 50, test_macau.py:132-207; probit AUC > 0.55, test_macau.py:47-62).
+The case the reference runs with the univariate sampler checks that it
+raises NotImplementedError, since this library does not provide it."""
 import itertools
 import os
 
@@ -74,10 +74,39 @@ def test_macau_dense():  # test_macau.py:41-45
     assert np.isfinite(res.rmse_test)
 
 
-def test_probit_and_univariate_not_provided():  # test_macau.py:47-73
+def test_macau_dense_probit():  # test_macau.py:47-62
+    announce("api macau dense probit")
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((25, 2))
+    B = rng.standard_normal((3, 2))
+    idx = list(itertools.product(np.arange(A.shape[0]), np.arange(B.shape[0])))
+    df = pd.DataFrame(np.asarray(idx), columns=["A", "B"])
+    df["value"] = (np.array([np.sum(A[i[0], :] * B[i[1], :]) for i in idx]) > 0.0).astype(np.float64)
+    np.random.seed(21)
+    Ytrain, Ytest = macau_amd.make_train_test_df(df, 0.2)
+    results = macau_amd.macau(Y=Ytrain, Ytest=Ytest, side=[A, None], num_latent=4, verbose=False, burnin=20,
+                              nsamples=20, univariate=False, precision="probit", seed=21)
+    assert (results.prediction.columns[0:2] == ["A", "B"]).all()
+    assert results.auc_test > 0.55, "AUC %f" % results.auc_test
+    assert np.isnan(results.rmse_test)
+    p = results.prediction["y_pred"].values
+    assert np.all((p > 0) & (p < 1))
+
+
+def test_bpmf_emptytest_probit():  # test_macau.py:84-88
+    announce("api bpmf probit empty test set")
+    X = _rand(15, 10, 0.2, 22)
+    X.data = (X.data > 0.5).astype(np.float64)
+    r0 = macau_amd.bpmf(X, Ytest=0, num_latent=10, burnin=10, nsamples=15, precision="probit", verbose=False,
+                        seed=22)
+    r1 = macau_amd.bpmf(X, Ytest=None, num_latent=10, burnin=10, nsamples=15, precision="probit", verbose=False,
+                        seed=22)
+    assert np.isnan(r0.auc_test) and np.isnan(r1.auc_test)
+    assert r0.prediction.shape[0] == 0
+
+
+def test_univariate_not_provided():  # test_macau.py:64-73
     Y = _rand(10, 20, 0.2, 9)
-    with pytest.raises(NotImplementedError):
-        macau_amd.macau(Y, verbose=False, precision="probit", burnin=2, nsamples=2)
     with pytest.raises(NotImplementedError):
         macau_amd.bpmf(Y, verbose=False, univariate=True, burnin=2, nsamples=2)
 
