@@ -27,9 +27,13 @@ Other workloads (`--workload`, one JSON line each, not the headline):
       takes the F^T F Cholesky branch (side information with at most 10,000
       features, as the Python API chooses); reports the FP64 rate of the
       per-iteration factorisation;
-  c4  3-mode tensor 50,000 x 2,000 x 100, 10M cells, K = 64.
+  c4  3-mode tensor 50,000 x 2,000 x 100, 10M cells, K = 64;
+  c2p the c2 cells binarised at the train mean, probit noise (the
+      augmentation kernel draws one truncated normal per observation before
+      each half-sweep); the full-chain line reports test AUC instead of RMSE.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -54,14 +58,22 @@ MAX_FF_SIZE = 10000  # side information with at most this many features uses F^T
 def load_workload(name, seed):
     """Generic N-mode form: idx [nnz x nmodes], vals, test idx/vals, dims,
     settings, side (per mode: scipy COO or None), label."""
-    if name == "c2":
+    if name in ("c2", "c2p"):
         train, test, dims, settings = synthetic.c2(seed)
         idx = np.stack([train[0], train[1]], 1)
         tidx = np.stack([test[0], test[1]], 1)
-        return dict(idx=idx, vals=train[2], tidx=tidx, tvals=test[2], dims=dims, settings=settings,
-                    side=[None, None],
-                    label="BPMF 138000x27000, 20M observed (18M train / 2M test), Zipf-1.1 degrees, K=32, "
-                          "adaptive noise")
+        vals, tvals = train[2], test[2]
+        if name == "c2":
+            return dict(idx=idx, vals=vals, tidx=tidx, tvals=tvals, dims=dims, settings=settings,
+                        side=[None, None],
+                        label="BPMF 138000x27000, 20M observed (18M train / 2M test), Zipf-1.1 degrees, K=32, "
+                              "adaptive noise")
+        cut = float(np.mean(vals))
+        settings = dict(settings, precision="probit")
+        return dict(idx=idx, vals=(vals > cut).astype(np.float64), tidx=tidx,
+                    tvals=(tvals > cut).astype(np.float64), dims=dims, settings=settings, side=[None, None],
+                    label="BPMF probit 138000x27000, 20M observed (18M train / 2M test) binarised at the train "
+                          "mean, Zipf-1.1 degrees, K=32")
     if name in ("c3", "c3f"):
         nfeat = 100000 if name == "c3" else 4000
         (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c3(seed, nfeat=nfeat)
@@ -84,6 +96,8 @@ def build_model(w, sampler_kind, seed, graph=True):
     nm = len(w["dims"])
     if st["precision"] == "adaptive":
         h = L.macau_make_adaptive(nm, K, 1.0, 10.0)
+    elif st["precision"] == "probit":
+        h = L.macau_make_probit(nm, K)
     else:
         h = L.macau_make_fixed(nm, K, float(st["precision"]))
     if not h:
@@ -157,6 +171,10 @@ def full_chain_rmse(name, w, seed):
         check(L.macau_iterate(h, n))
         check(L.macau_synchronize(h))
         t1 = time.perf_counter()
+        if w["settings"]["precision"] == "probit":
+            a1, aa = ctypes.c_double(), ctypes.c_double()
+            check(L.macau_get_auc(h, ctypes.byref(a1), ctypes.byref(aa)))
+            return aa.value, n / (t1 - t0)
         return L.macau_get_rmse_avg(h), n / (t1 - t0)
     finally:
         L.macau_free(h)
@@ -211,7 +229,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c4"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c4", "c2p"], default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
@@ -242,8 +260,8 @@ def main():
         if not args.no_chain:
             rmse, ips = full_chain_rmse(name, w, args.seed)
             res[name]["rmse"] = rmse
-            log("%s full chain (%d iterations): posterior-mean RMSE %.6f" %
-                (name, st["burnin"] + st["nsamples"], rmse))
+            log("%s full chain (%d iterations): posterior-mean %s %.6f" %
+                (name, st["burnin"] + st["nsamples"], "AUC" if st["precision"] == "probit" else "RMSE", rmse))
 
     flops = sampler_flops(w)
     ph = res[head]["phase"]
@@ -303,9 +321,10 @@ def main():
         out["plain_iters_per_sec"] = 1.0 / res["plain"]["sec"]
         out["speedup_vs_plain"] = res["plain"]["sec"] / sec
     if not args.no_chain:
-        out["rmse"] = {k: v.get("rmse") for k, v in res.items()}
+        key = "auc" if st["precision"] == "probit" else "rmse"
+        out[key] = {k: v.get("rmse") for k, v in res.items()}
         if "plain" in res and head != "plain":
-            out["rmse"]["rel_diff_vs_plain"] = res[head]["rmse"] / res["plain"]["rmse"] - 1.0
+            out[key]["rel_diff_vs_plain"] = res[head]["rmse"] / res["plain"]["rmse"] - 1.0
     if not args.no_cpu and args.workload == "c2":
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_iters, args.seed, log)
     print(json.dumps(out), flush=True)

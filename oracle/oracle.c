@@ -727,21 +727,53 @@ out:
 /* ------------------------------------------------------------------ */
 double orc_norm_cdf(double x) { return 0.5 * erfc(-x * M_SQRT1_2); }
 
+/* Wichura, "Algorithm AS 241: the percentage points of the normal
+ * distribution", Applied Statistics 37 (1988) 477-484, PPND16: rational
+ * minimax approximations in q = p - 1/2 (|q| <= 0.425) and in
+ * r = sqrt(-log(min(p, 1 - p))) (r <= 5, r > 5); about 1e-16 relative. */
+static const double AS241_A[8] = {3.3871328727963666080e0, 1.3314166789178437745e+2, 1.9715909503065514427e+3,
+                                  1.3731693765509461125e+4, 4.5921953931549871457e+4, 6.7265770927008700853e+4,
+                                  3.3430575583588128105e+4, 2.5090809287301226727e+3};
+static const double AS241_B[8] = {1.0, 4.2313330701600911252e+1, 6.8718700749205790830e+2, 5.3941960214247511077e+3,
+                                  2.1213794301586595867e+4, 3.9307895800092710610e+4, 2.8729085735721942674e+4,
+                                  5.2264952788528545610e+3};
+static const double AS241_C[8] = {1.42343711074968357734e0, 4.63033784615654529590e0, 5.76949722146069140550e0,
+                                  3.64784832476320460504e0, 1.27045825245236838258e0, 2.41780725177450611770e-1,
+                                  2.27238449892691845833e-2, 7.74545014278341407640e-4};
+static const double AS241_D[8] = {1.0, 2.05319162663775882187e0, 1.67638483018380384940e0, 6.89767334985100004550e-1,
+                                  1.48103976427480074590e-1, 1.51986665636164571966e-2, 5.47593808499534494600e-4,
+                                  1.05075007164441684324e-9};
+static const double AS241_E[8] = {6.65790464350110377720e0, 5.46378491116411436990e0, 1.78482653991729133580e0,
+                                  2.96560571828504891230e-1, 2.65321895265761230930e-2, 1.24266094738807843860e-3,
+                                  2.71155556874348757815e-5, 2.01033439929228813265e-7};
+static const double AS241_F[8] = {1.0, 5.99832206555887937690e-1, 1.36929880922735805310e-1, 1.48753612908506148525e-2,
+                                  7.86869131145613259100e-4, 1.84631831751005468180e-5, 1.42151175831644588870e-7,
+                                  2.04426310338993978564e-15};
+
+static double horner8(const double* c, double x) {
+  double v = c[7];
+  for (int i = 6; i >= 0; i--) v = v * x + c[i];
+  return v;
+}
+
 double orc_inv_norm_cdf(double p) {
   if (!(p > 0.0)) return -INFINITY;
   if (!(p < 1.0)) return INFINITY;
-  const double q = p < 0.5 ? p : 1.0 - p;  /* exact for p >= 0.5 */
-  /* A&S 26.2.23: upper-tail quantile of q, |error| < 4.5e-4 */
-  const double t = sqrt(-2.0 * log(q));
-  double y = -(t - (2.515517 + t * (0.802853 + t * 0.010328)) / (1.0 + t * (1.432788 + t * (0.189269 + t * 0.001308))));
-  /* Halley on f(y) = Phi(y) - q (y <= 0); for subnormal q, 1 / phi(y)
-   * would overflow and the starting value is kept */
-  for (int k = 0; k < 3 && q >= 2.2250738585072014e-308; k++) {
-    const double e = orc_norm_cdf(y) - q;
-    const double u = e * 2.5066282746310002 * exp(0.5 * y * y);  /* e / phi(y) */
-    y -= u / (1.0 + 0.5 * y * u);
+  const double q = p - 0.5;
+  if (fabs(q) <= 0.425) {
+    const double r = 0.180625 - q * q;
+    return q * horner8(AS241_A, r) / horner8(AS241_B, r);
   }
-  return p < 0.5 ? y : -y;
+  double r = sqrt(-log(q < 0.0 ? p : 1.0 - p));
+  double x;
+  if (r <= 5.0) {
+    r -= 1.6;
+    x = horner8(AS241_C, r) / horner8(AS241_D, r);
+  } else {
+    r -= 5.0;
+    x = horner8(AS241_E, r) / horner8(AS241_F, r);
+  }
+  return q < 0.0 ? -x : x;
 }
 
 /* (k + 0.5) / 2^53 for the 53-bit integer k of two Philox words */

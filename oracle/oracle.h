@@ -132,8 +132,7 @@ int orc_ftf_solve(int nrhs, int64_t n, const double* FtF, double reg, const doub
 /* ---- probit noise (latentprior.cpp:387-414, truncnorm.cpp, noisemodels.cpp:129-159) ---- */
 /* Phi(x) = 0.5 erfc(-x / sqrt 2) (truncnorm.cpp:5-7) */
 double orc_norm_cdf(double x);
-/* Phi^{-1}(p): Abramowitz & Stegun 26.2.23 as the start, three Halley steps
- * on the lower-tail form Phi(y) = min(p, 1 - p); -inf / +inf at 0 / 1 */
+/* Phi^{-1}(p): Wichura's AS 241 (PPND16); -inf / +inf at 0 / 1 */
 double orc_inv_norm_cdf(double p);
 /* rand_truncnorm(mean, 1, 0) (truncnorm.cpp:9-37): inverse-CDF sampling
  * for a cut abar = -mean <= 3, the reference's tail sampler above (whose
