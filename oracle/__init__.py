@@ -27,6 +27,7 @@ _LIB = None
 TAG_LATENT, TAG_WISHART_Z, TAG_WISHART_CHI, TAG_NW_MU = 1, 2, 3, 4
 TAG_NOISE, TAG_BETA_U, TAG_BETA_F, TAG_LAMBDA_BETA = 5, 6, 7, 8
 TAG_PROBIT = 9
+TAG_BETA_ONE = 10
 
 
 def tag(stream, mode=0):
@@ -60,7 +61,7 @@ class OrcRunCfg(ctypes.Structure):
         ("pred", _p), ("pred_var", _p), ("U_out", _p * 8), ("rmse", _dbl), ("rmse1", _dbl),
         ("alpha", _dbl), ("alpha_trace", _p), ("rmse_trace", _p), ("lambda_beta_out", _p),
         ("cg_iters_last", ctypes.c_int), ("use_ftf", ctypes.c_int * 8),
-        ("noise_probit", ctypes.c_int), ("auc", _dbl), ("auc1", _dbl),
+        ("noise_probit", ctypes.c_int), ("auc", _dbl), ("auc1", _dbl), ("uni", ctypes.c_int * 8),
     ]
 
 
@@ -114,6 +115,16 @@ def lib():
         L.orc_auc.argtypes = [_i64, _p, _p]
         L.orc_eval_probit.argtypes = [ctypes.c_int, _i64, _p, _p, _p, ctypes.c_int, _p, _p, _p,
                                       ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
+        L.orc_sample_latents_uni.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p, _dbl, _p, _p, _u64,
+                                             _u32, ctypes.c_int, _p]
+        L.orc_at_mul_bt_col.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _i64, _p, _p]
+        L.orc_add_acol_mul_bt.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _i64, _p, _p]
+        L.orc_beta_sweep_uni.argtypes = [ctypes.c_int, _i64, ctypes.POINTER(OrcCsr), _p, _p, _p, _p, _p, _p, _p,
+                                         _u64, _u32, ctypes.c_int]
+        L.orc_lambda_beta_uni.argtypes = [ctypes.c_int, _i64, _p, _u64, _u32, ctypes.c_int, _p]
+        L.orc_macau_one_update_prior.restype = ctypes.c_int
+        L.orc_macau_one_update_prior.argtypes = [ctypes.c_int, ctypes.c_int, _p, _i64, ctypes.POINTER(OrcCsr),
+                                                 ctypes.POINTER(OrcCsr), _p, _p, _p, _p, _p, _p, _u64, _u32]
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [ctypes.POINTER(OrcRunCfg)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -354,6 +365,73 @@ def eval_probit(K, test_idx, test_vals, factors, n, pred, pred_var):
     return p1[:-1], a1.value, aa.value
 
 
+def sample_latents_uni(K, rows, mean, others, alpha, lam, mu_rows, U, seed, it, mode):
+    """MacauOnePrior::sample_latents (macauoneprior.cpp:42-95, 108-187):
+    coordinate-wise update of U (N x K, modified copy returned); lam is the
+    diagonal prior precision, mu_rows (N x K) the per-row prior mean."""
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in others]
+    arr = (ctypes.c_void_p * len(fs))(*[_ptr(f) for f in fs])
+    U = np.array(U, dtype=np.float64, order="C", copy=True)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    mr = np.ascontiguousarray(mu_rows, dtype=np.float64)
+    lib().orc_sample_latents_uni(K, ctypes.byref(rows.c()), mean, ctypes.cast(arr, ctypes.c_void_p), alpha,
+                                 _ptr(lam), _ptr(mr), seed, it, mode, _ptr(U))
+    return U
+
+
+def at_mul_bt_col(Ft, f, Z):
+    """At_mul_Bt (linop.h:121-149): Z^T F[:, f] for Z (N x nr)."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    y = np.zeros(Z.shape[1])
+    lib().orc_at_mul_bt_col(Z.shape[1], ctypes.byref(Ft.c()), f, _ptr(Z), _ptr(y))
+    return y
+
+
+def add_acol_mul_bt(Ft, f, b, Z):
+    """add_Acol_mul_bt (linop.h:151-178): returns Z + F[:, f] b^T."""
+    Z = np.array(Z, dtype=np.float64, order="C", copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    lib().orc_add_acol_mul_bt(Z.shape[1], ctypes.byref(Ft.c()), f, _ptr(b), _ptr(Z))
+    return Z
+
+
+def beta_sweep_uni(Ft, colsq, lam, lambda_beta, mu, U, Uhat, beta, seed, it, mode):
+    """One feature sweep of MacauOnePrior::sample_beta (macauoneprior.cpp:215-252);
+    returns the new beta (nfeat x K)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N, K = U.shape
+    beta = np.array(beta, dtype=np.float64, order="C", copy=True)
+    args = [np.ascontiguousarray(x, dtype=np.float64) for x in (colsq, lam, lambda_beta, mu, Uhat)]
+    lib().orc_beta_sweep_uni(K, N, ctypes.byref(Ft.c()), _ptr(args[0]), _ptr(args[1]), _ptr(args[2]), _ptr(args[3]),
+                             _ptr(U), _ptr(args[4]), _ptr(beta), seed, it, mode)
+    return beta
+
+
+def lambda_beta_uni(beta, seed, it, mode):
+    """sample_lambda_beta (macauoneprior.cpp:263-289)."""
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    out = np.zeros(beta.shape[1])
+    lib().orc_lambda_beta_uni(beta.shape[1], beta.shape[0], _ptr(beta), seed, it, mode, _ptr(out))
+    return out
+
+
+def macau_one_update_prior(U, F, Ft, colsq, mu, lam, beta, Uhat, lambda_beta, seed, it, mode):
+    """MacauOnePrior::update_prior (macauoneprior.cpp:189-289); updates and
+    returns copies of (mu, lam, beta (nfeat x K), Uhat (N x K), lambda_beta)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N, K = U.shape
+    mu, lam = np.array(mu, dtype=np.float64), np.array(lam, dtype=np.float64)
+    beta = np.array(beta, dtype=np.float64, order="C")
+    Uhat = np.array(Uhat, dtype=np.float64, order="C")
+    lb = np.array(lambda_beta, dtype=np.float64)
+    cs = np.ascontiguousarray(colsq, dtype=np.float64)
+    rc = lib().orc_macau_one_update_prior(K, mode, _ptr(U), N, ctypes.byref(F.c()), ctypes.byref(Ft.c()), _ptr(cs),
+                                          _ptr(mu), _ptr(lam), _ptr(beta), _ptr(Uhat), _ptr(lb), seed, it)
+    if rc:
+        raise RuntimeError("MacauOnePrior update failed (%d)" % rc)
+    return mu, lam, beta, Uhat, lb
+
+
 def csr_A_mul_Bn(A, B):
     """out[r] = sum_c A[r,c] B[c]; B is (ncol, nrhs) C-order."""
     B = np.ascontiguousarray(B, dtype=np.float64)
@@ -425,10 +503,11 @@ class RunResult:
 
 def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False, adaptive=False,
         precision=1.0, sn_init=1.0, sn_max=10.0, side=None, lambda_beta=5.0, tol=1e-6, burnin=10,
-        nsamples=10, seed=0, max_iters=0, want_U=False, use_ftf=None, probit=False):
+        nsamples=10, seed=0, max_iters=0, want_U=False, use_ftf=None, probit=False, uni=None):
     """Full Gibbs run (MacauX::run).  side: per-mode (F, Ft) Csr pairs or None;
     use_ftf: per-mode flags selecting the F^T F branch of sample_beta;
-    probit: ProbitNoise (matrix data), reporting auc / auc1."""
+    probit: ProbitNoise (matrix data), reporting auc / auc1;
+    uni: per-mode flags selecting MacauOnePrior for modes with side information."""
     M = len(dims)
     cfg = OrcRunCfg()
     cfg.nmodes, cfg.K, cfg.is_tensor = M, K, int(is_tensor)
@@ -450,6 +529,7 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
             cfg.F[m] = side[m][0].c()
             cfg.Ft[m] = side[m][1].c()
             cfg.use_ftf[m] = int(bool(use_ftf and use_ftf[m]))
+            cfg.uni[m] = int(bool(uni and uni[m]))
     cfg.lambda_beta0, cfg.tol = lambda_beta, tol
     cfg.burnin, cfg.nsamples, cfg.seed, cfg.max_iters = burnin, nsamples, seed, max_iters
     pred = np.zeros(len(tv))

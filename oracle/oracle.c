@@ -723,6 +723,150 @@ out:
 }
 
 /* ------------------------------------------------------------------ */
+/* univariate prior MacauOnePrior                                           */
+/* ------------------------------------------------------------------ */
+void orc_sample_latents_uni(int K, const orc_rows* rows, double mean, const double* const* others, double alpha,
+                            const double* lambda, const double* mu_rows, uint64_t seed, uint32_t it, int mode,
+                            double* U) {
+  const int64_t N = rows->nrows;
+  const int no = rows->nother;
+#pragma omp parallel
+  {
+    double* Q = (double*)malloc(sizeof(double) * K);
+    double* z = (double*)malloc(sizeof(double) * (K + 4));
+    int64_t cap = 0;
+    double *yhat = NULL, *prod = NULL;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t n = 0; n < N; n++) {
+      const int64_t beg = rows->row_ptr[n], nnz = rows->row_ptr[n + 1] - beg;
+      if (nnz > cap) {
+        cap = nnz;
+        yhat = (double*)realloc(yhat, sizeof(double) * (size_t)cap);
+        prod = (double*)realloc(prod, sizeof(double) * (size_t)cap * K);
+      }
+      double* u = U + (size_t)n * K;
+      /* Qi = lambda + alpha sum v^2, Yhat = mean + u . v (:60-66, :137-148) */
+      for (int d = 0; d < K; d++) Q[d] = lambda[d];
+      for (int64_t j = 0; j < nnz; j++) {
+        const int64_t e = beg + j;
+        double* v = prod + (size_t)j * K;
+        const double* c0 = others[0] + (size_t)rows->idx[e * no] * K;
+        for (int d = 0; d < K; d++) v[d] = c0[d];
+        for (int t = 1; t < no; t++) {
+          const double* ct = others[t] + (size_t)rows->idx[e * no + t] * K;
+          for (int d = 0; d < K; d++) v[d] *= ct[d];
+        }
+        double dot = 0.0;
+        for (int d = 0; d < K; d++) {
+          Q[d] += alpha * v[d] * v[d];
+          dot += u[d] * v[d];
+        }
+        yhat[j] = mean + dot;
+      }
+      for (int d = 0; d < K; d += 4) orc_normal_block(seed, (uint32_t)n, (uint32_t)(d >> 2), it,
+                                                      ORC_TAG(ORC_TAG_LATENT, mode), z + d);
+      /* coordinate updates (:70-93, :154-185) */
+      for (int d = 0; d < K; d++) {
+        const double uid = u[d];
+        double L = lambda[d] * mu_rows[(size_t)n * K + d];
+        for (int64_t j = 0; j < nnz; j++) {
+          const double vjd = prod[(size_t)j * K + d];
+          L += alpha * (rows->vals[beg + j] - (yhat[j] - uid * vjd)) * vjd;
+        }
+        const double var = 1.0 / Q[d];
+        u[d] = L * var + sqrt(var) * z[d];
+        const double delta = u[d] - uid;
+        for (int64_t j = 0; j < nnz; j++) yhat[j] += delta * prod[(size_t)j * K + d];
+      }
+    }
+    free(Q);
+    free(z);
+    free(yhat);
+    free(prod);
+  }
+}
+
+void orc_at_mul_bt_col(int nr, const orc_csr* Ft, int64_t f, const double* Z, double* y) {
+  for (int d = 0; d < nr; d++) y[d] = 0.0;
+  for (int64_t e = Ft->row_ptr[f]; e < Ft->row_ptr[f + 1]; e++) {
+    const double v = Ft->vals ? Ft->vals[e] : 1.0;
+    const double* z = Z + (size_t)Ft->cols[e] * nr;
+    for (int d = 0; d < nr; d++) y[d] += v * z[d];
+  }
+}
+
+void orc_add_acol_mul_bt(int nr, const orc_csr* Ft, int64_t f, const double* b, double* Z) {
+  for (int64_t e = Ft->row_ptr[f]; e < Ft->row_ptr[f + 1]; e++) {
+    const double v = Ft->vals ? Ft->vals[e] : 1.0;
+    double* z = Z + (size_t)Ft->cols[e] * nr;
+    for (int d = 0; d < nr; d++) z[d] += v * b[d];
+  }
+}
+
+void orc_beta_sweep_uni(int K, int64_t N, const orc_csr* Ft, const double* colsq, const double* lambda,
+                        const double* lambda_beta, const double* mu, const double* U, const double* Uhat,
+                        double* beta, uint64_t seed, uint32_t it, int mode) {
+  const int64_t nfeat = Ft->nrow;
+  double* Z = (double*)malloc(sizeof(double) * (size_t)K * (size_t)(N > 0 ? N : 1));
+  double* zx = (double*)malloc(sizeof(double) * K);
+  double* delta = (double*)malloc(sizeof(double) * K);
+  /* Z = U - mu - Uhat (macauoneprior.cpp:227-232) */
+  for (int64_t i = 0; i < N; i++)
+    for (int d = 0; d < K; d++) Z[(size_t)i * K + d] = U[(size_t)i * K + d] - mu[d] - Uhat[(size_t)i * K + d];
+  /* features in order; the latents are independent chains, so all K are
+   * advanced together (the reference runs them in blocks of 4, :220-251) */
+  for (int64_t f = 0; f < nfeat; f++) {
+    orc_at_mul_bt_col(K, Ft, f, Z, zx);
+    for (int d = 0; d < K; d++) {
+      const double A = lambda_beta[d] + lambda[d] * colsq[f];
+      const double B = lambda[d] * (zx[d] + beta[(size_t)f * K + d] * colsq[f]);
+      const double Ainv = 1.0 / A;
+      const double bnew =
+          B * Ainv + sqrt(Ainv) * orc_normal(seed, (uint32_t)f, (uint32_t)d, it, ORC_TAG(ORC_TAG_BETA_ONE, mode));
+      delta[d] = beta[(size_t)f * K + d] - bnew;
+      beta[(size_t)f * K + d] = bnew;
+    }
+    orc_add_acol_mul_bt(K, Ft, f, delta, Z);
+  }
+  free(Z);
+  free(zx);
+  free(delta);
+}
+
+void orc_lambda_beta_uni(int K, int64_t nfeat, const double* beta, uint64_t seed, uint32_t it, int mode,
+                         double* lambda_beta) {
+  for (int d = 0; d < K; d++) {
+    double ss = 0.0;
+    for (int64_t f = 0; f < nfeat; f++) ss += beta[(size_t)f * K + d] * beta[(size_t)f * K + d];
+    const double a = 0.1 + (double)nfeat / 2.0, b = 0.1 + ss / 2.0;
+    lambda_beta[d] = orc_gamma(seed, a, (uint32_t)d, it, ORC_TAG(ORC_TAG_LAMBDA_BETA, mode)) / b;
+  }
+}
+
+int orc_macau_one_update_prior(int K, int mode, const double* U, int64_t N, const orc_csr* F, const orc_csr* Ft,
+                               const double* colsq, double* mu, double* lambda, double* beta, double* Uhat,
+                               double* lambda_beta, uint64_t seed, uint32_t it) {
+  const size_t KK = (size_t)K * K;
+  double* Ud = (double*)malloc(sizeof(double) * (size_t)K * N);
+  double* WI = (double*)calloc(KK, sizeof(double));
+  double* mu0 = (double*)calloc(K, sizeof(double));
+  double* Lam = (double*)malloc(sizeof(double) * KK);
+  int rc = 0;
+  /* sample_mu_lambda (:198-213): the Normal-Wishart draw, diagonal kept */
+  for (size_t q = 0; q < (size_t)K * N; q++) Ud[q] = U[q] - Uhat[q];
+  for (int i = 0; i < K; i++) WI[i + (size_t)i * K] = 1.0;
+  rc = orc_cond_normal_wishart(K, N, Ud, mu0, 2.0, WI, (double)K, seed, it, mode, mu, Lam);
+  if (rc) goto out;
+  for (int d = 0; d < K; d++) lambda[d] = Lam[d + (size_t)d * K];
+  orc_beta_sweep_uni(K, N, Ft, colsq, lambda, lambda_beta, mu, U, Uhat, beta, seed, it, mode);  /* :215-252 */
+  orc_csr_A_mul_Bn(K, F, beta, Uhat);                                                          /* compute_uhat */
+  orc_lambda_beta_uni(K, F->ncol, beta, seed, it, mode, lambda_beta);                           /* :263-289 */
+out:
+  free(Ud); free(WI); free(mu0); free(Lam);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
 /* probit noise                                                             */
 /* ------------------------------------------------------------------ */
 double orc_norm_cdf(double x) { return 0.5 * erfc(-x * M_SQRT1_2); }
@@ -891,6 +1035,9 @@ int orc_run(orc_run_cfg* c) {
   double* Uhat[8] = {0};
   double* murows[8] = {0};
   double* FtF[8] = {0};
+  double* lam_uni[8] = {0};   /* MacauOnePrior: diagonal prior precision */
+  double* lb_uni[8] = {0};    /* MacauOnePrior: per-latent lambda_beta */
+  double* colsq[8] = {0};     /* MacauOnePrior: squared column norms of F */
   double lambda_beta[8];
   double* WI = (double*)calloc(KK, sizeof(double));
   int rc = 0;
@@ -906,7 +1053,18 @@ int orc_run(orc_run_cfg* c) {
       Uhat[m] = (double*)calloc((size_t)K * c->dims[m], sizeof(double));
       murows[m] = (double*)malloc(sizeof(double) * (size_t)K * c->dims[m]);
       lambda_beta[m] = c->lambda_beta0;
-      if (c->use_ftf[m]) {  /* MacauPrior::init: At_mul_A(FtF, *F) (latentprior.cpp:200-203) */
+      if (c->uni[m]) {  /* MacauOnePrior::init (macauoneprior.cpp:16-38), setLambdaBeta */
+        lam_uni[m] = (double*)malloc(sizeof(double) * K);
+        lb_uni[m] = (double*)malloc(sizeof(double) * K);
+        colsq[m] = (double*)calloc((size_t)c->F[m].ncol + 1, sizeof(double));
+        for (int k = 0; k < K; k++) {
+          lam_uni[m][k] = 10.0;
+          lb_uni[m][k] = c->lambda_beta0;
+        }
+        for (int64_t f = 0; f < c->F[m].ncol; f++)
+          for (int64_t e = c->Ft[m].row_ptr[f]; e < c->Ft[m].row_ptr[f + 1]; e++)
+            colsq[m][f] += c->Ft[m].vals ? c->Ft[m].vals[e] * c->Ft[m].vals[e] : 1.0;
+      } else if (c->use_ftf[m]) {  /* MacauPrior::init: At_mul_A(FtF, *F) (latentprior.cpp:200-203) */
         FtF[m] = (double*)malloc(sizeof(double) * (size_t)c->F[m].ncol * (size_t)c->F[m].ncol);
         orc_at_mul_a(&c->F[m], &c->Ft[m], FtF[m]);
       }
@@ -934,6 +1092,8 @@ int orc_run(orc_run_cfg* c) {
   double* pred1 = NULL;
   if (c->noise_probit) {
     if (M != 2) { rc = -3; goto done; }  /* "Unimplemented: sample_latents" (latentprior.cpp:112-116) */
+    for (int m = 0; m < M; m++)
+      if (c->macau[m] && c->uni[m]) { rc = -4; goto done; }  /* "Not implemented!" (macauoneprior.cpp:260-264) */
     int64_t mx = 0;
     for (int m = 0; m < M; m++) mx = c->train[m].row_ptr[c->train[m].nrows] > mx ? c->train[m].row_ptr[c->train[m].nrows] : mx;
     zbuf = (double*)malloc(sizeof(double) * ((size_t)mx + 1));
@@ -953,7 +1113,11 @@ int orc_run(orc_run_cfg* c) {
           for (int k = 0; k < K; k++) murows[m][(size_t)n * K + k] = mu[m][k] + Uhat[m][(size_t)n * K + k];
       }
       int64_t f;
-      if (c->noise_probit) {
+      if (c->macau[m] && c->uni[m]) {
+        orc_sample_latents_uni(K, &c->train[m], c->mean, others, alpha, lam_uni[m], murows[m], c->seed, (uint32_t)i,
+                               m, U[m]);
+        f = 0;
+      } else if (c->noise_probit) {
         orc_probit_z(K, &c->train[m], others, U[m], c->seed, (uint32_t)i, m, zbuf);
         orc_rows zr = c->train[m];
         zr.vals = zbuf;
@@ -966,7 +1130,10 @@ int orc_run(orc_run_cfg* c) {
       if (f) { rc = -1; goto done; }
     }
     for (int m = 0; m < M; m++) {
-      if (c->macau[m]) {
+      if (c->macau[m] && c->uni[m]) {
+        rc = orc_macau_one_update_prior(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], colsq[m], mu[m], lam_uni[m],
+                                        beta[m], Uhat[m], lb_uni[m], c->seed, (uint32_t)i);
+      } else if (c->macau[m]) {
         int cgit = 0;
         rc = macau_update_prior_t(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], FtF[m], WI, K, beta[m], Uhat[m],
                                   &lambda_beta[m], c->tol, mu[m], Lambda[m], c->seed, (uint32_t)i, &cgit);
@@ -1001,11 +1168,18 @@ int orc_run(orc_run_cfg* c) {
   c->alpha = alpha;
   for (int m = 0; m < M; m++) {
     if (c->U_out[m]) memcpy(c->U_out[m], U[m], sizeof(double) * (size_t)K * c->dims[m]);
-    if (c->lambda_beta_out && c->macau[m]) c->lambda_beta_out[m] = lambda_beta[m];
+    if (c->lambda_beta_out && c->macau[m] && c->uni[m]) {
+      double s = 0.0;
+      for (int k = 0; k < K; k++) s += lb_uni[m][k];
+      c->lambda_beta_out[m] = s / K;  /* getLinkLambda: lambda_beta.mean() (macauoneprior.h:45) */
+    } else if (c->lambda_beta_out && c->macau[m]) {
+      c->lambda_beta_out[m] = lambda_beta[m];
+    }
   }
 done:
   for (int m = 0; m < M; m++) {
     free(U[m]); free(mu[m]); free(Lambda[m]); free(beta[m]); free(Uhat[m]); free(murows[m]); free(FtF[m]);
+    free(lam_uni[m]); free(lb_uni[m]); free(colsq[m]);
   }
   free(WI);
   free(zbuf);

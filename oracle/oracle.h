@@ -41,6 +41,7 @@ extern "C" {
 #define ORC_TAG_BETA_F      7u  /* c0 = feat, j = component (sample_beta)   */
 #define ORC_TAG_LAMBDA_BETA 8u  /* c0 = 0,    gamma for lambda_beta         */
 #define ORC_TAG_PROBIT      9u  /* c0 = observation (mode order), c1 = attempt */
+#define ORC_TAG_BETA_ONE   10u  /* c0 = feature, j = latent (univariate sample_beta) */
 #define ORC_TAG(stream, mode) (((uint32_t)(stream) << 24) | (uint32_t)(mode))
 
 /* ---- RNG (mvnormal.cpp:59-166 replaced by counter-based draws) ---- */
@@ -152,6 +153,38 @@ double orc_auc(int64_t n, const double* pred, const double* test);
 void orc_eval_probit(int K, int64_t ntest, const int32_t* test_idx, const double* test_vals, const double* const* U,
                      int n, double* pred, double* pred_var, double* pred1, double* auc1, double* auc_avg);
 
+/* ---- univariate prior MacauOnePrior (macauoneprior.cpp) ---- */
+/* sample_latents (macauoneprior.cpp:42-95 matrix, :108-187 tensor): per row,
+ * coordinate-wise u_d ~ N(L_d / Q_d, 1 / Q_d) in order d = 0..K-1 with
+ * running predictions; lambda is the diagonal prior precision and
+ * mu_rows (N x K) the per-row prior mean mu + Uhat_n; the row's normals are
+ * the latent stream (row, d, it, tag(LATENT, mode)) */
+void orc_sample_latents_uni(int K, const orc_rows* rows, double mean, const double* const* others, double alpha,
+                            const double* lambda, const double* mu_rows, uint64_t seed, uint32_t it, int mode,
+                            double* U);
+/* MacauOnePrior::update_prior (macauoneprior.cpp:189-252): sample_mu_lambda
+ * (CondNormalWishart of U - Uhat with mu0 = 0, kappa 2, WI = I, nu = K; the
+ * diagonal is kept), sample_beta (for every latent d, features f in order:
+ * A = lambda_beta_d + lambda_d colsq_f, B = lambda_d (z_d . F_f + beta_df
+ * colsq_f), beta_df ~ N(B / A, 1 / A), z_d updated), compute_uhat and
+ * per-latent lambda_beta_d ~ Gamma(0.1 + nfeat / 2, 1 / (0.1 + |beta_d|^2 / 2)).
+ * U, Uhat: N x K; beta: nfeat x K; lambda, lambda_beta, mu: K; colsq: nfeat */
+/* y = Z^T F[:, f] for Z (N x nr): At_mul_Bt (linop.h:121-149) */
+void orc_at_mul_bt_col(int nr, const orc_csr* Ft, int64_t f, const double* Z, double* y);
+/* Z += F[:, f] b^T: add_Acol_mul_bt (linop.h:151-178) */
+void orc_add_acol_mul_bt(int nr, const orc_csr* Ft, int64_t f, const double* b, double* Z);
+/* the feature sweep of sample_beta (macauoneprior.cpp:215-252) from
+ * Z = U - mu - Uhat; normals keyed (feature, latent, it, tag(BETA_ONE, mode)) */
+void orc_beta_sweep_uni(int K, int64_t N, const orc_csr* Ft, const double* colsq, const double* lambda,
+                        const double* lambda_beta, const double* mu, const double* U, const double* Uhat,
+                        double* beta, uint64_t seed, uint32_t it, int mode);
+/* sample_lambda_beta (macauoneprior.cpp:263-289), gamma keyed (d, it, tag(LAMBDA_BETA, mode)) */
+void orc_lambda_beta_uni(int K, int64_t nfeat, const double* beta, uint64_t seed, uint32_t it, int mode,
+                         double* lambda_beta);
+int orc_macau_one_update_prior(int K, int mode, const double* U, int64_t N, const orc_csr* F, const orc_csr* Ft,
+                               const double* colsq, double* mu, double* lambda, double* beta, double* Uhat,
+                               double* lambda_beta, uint64_t seed, uint32_t it);
+
 /* ---- a4: full Gibbs run (macau.cpp:72-86, 94-148) ---- */
 typedef struct {
   int      nmodes, K, is_tensor;
@@ -185,6 +218,10 @@ typedef struct {
   int      noise_probit;
   /* outputs of probit runs (rmse / rmse1 stay NaN) */
   double   auc, auc1;
+  /* input: per mode with side information, the univariate prior
+   * MacauOnePrior (1) instead of MacauPrior (0); lambda_beta_out then
+   * receives the mean of its per-latent lambda_beta */
+  int      uni[8];
 } orc_run_cfg;
 int orc_run(orc_run_cfg* cfg);
 

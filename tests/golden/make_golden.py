@@ -63,6 +63,15 @@ def reference_kat():
     }
     # tests.cpp:859-885  make_dense_prior: FtF = lower(F^T F) of a 3 x 2 F
     k["dense_prior_ftf"] = {"F": [[0.1, 0.3], [0.4, 0.11], [-0.7, 0.23]]}
+    # tests.cpp:338-368  At_mul_Bt: B (2 x 6) times column 1 of the feature matrix
+    k["At_mul_Bt"] = {"B": [[-0.23, -2.89, -1.04, -0.52, -1.45, -1.42], [-0.16, -0.62, 1.19, 1.12, 0.11, 0.61]],
+                      "col": 1, "Y_bin": [-4.16, 0.41], "Y_csr": [0.9942, 1.8285]}
+    # tests.cpp:370-403  add_Acol_mul_bt: Z (2 x 6) += b * column 1^T
+    k["add_Acol_mul_bt"] = {
+        "Z": [[-0.23, -2.89, -1.04, -0.52, -1.45, -1.42], [-0.16, -0.62, 1.19, 1.12, 0.11, 0.61]],
+        "b": [-4.16, 0.41], "col": 1,
+        "Z_bin": [[-4.39, -7.05, -5.2, -0.52, -1.45, -1.42], [0.25, -0.21, 1.6, 1.12, 0.11, 0.61]],
+        "Z_csr": [[-2.726, 0.5212, -5.9904, -0.52, -1.45, -1.42], [0.086, -0.9562, 1.6779, 1.12, 0.11, 0.61]]}
     # tests.cpp:606-614  auc of descending predictions
     k["auc"] = {"test": [1, 0, 1, 0, 1, 0, 0, 1, 0, 1] + [0] * 10,
                 "pred": [20.0 - i for i in range(20)], "auc": 0.84}
