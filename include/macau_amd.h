@@ -59,6 +59,12 @@ int macau_add_macau_prior(macau_model* m, int nrow, int ncol, long long nnz, con
 int macau_add_macau_prior_ff(macau_model* m, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
                              const double* vals /* NULL = binary */, double lambda_beta, double tol,
                              int max_ff_size);
+/* make_one_prior(side, num_latent, lambda_beta) (macau.pyx:194-214):
+ * MacauOnePrior<SparseFeat / SparseDoubleFeat> (macauoneprior.h:8-51) with
+ * setLambdaBeta(lambda_beta); the univariate=True path of setSidePriors
+ * (macau.pyx:328-333).  Not available with probit noise. */
+int macau_add_macau_one_prior(macau_model* m, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
+                              const double* vals /* NULL = binary */, double lambda_beta);
 
 /* ---- Macau API (macau.h:35-53; macau.pxd:63-84) ---- */
 int macau_set_samples(macau_model* m, int burnin, int nsamples);

@@ -32,6 +32,7 @@ SIGNATURES = {
     "macau_add_bpmf_prior": (_i, [_p]),
     "macau_add_macau_prior": (_i, [_p, _i, _i, _ll, _p, _p, _p, _d, _d]),
     "macau_add_macau_prior_ff": (_i, [_p, _i, _i, _ll, _p, _p, _p, _d, _d, _i]),
+    "macau_add_macau_one_prior": (_i, [_p, _i, _i, _ll, _p, _p, _p, _d]),
     "macau_set_samples": (_i, [_p, _i, _i]),
     "macau_set_relation_data": (_i, [_p, _p, _i, _p, _i, _p]),
     "macau_set_relation_data_test": (_i, [_p, _p, _i, _p, _i, _p]),

@@ -102,6 +102,16 @@ int macau_add_macau_prior_ff(macau_model* h, int nrow, int ncol, long long nnz, 
   API_END_INT
 }
 
+int macau_add_macau_one_prior(macau_model* h, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
+                              const double* vals, double lambda_beta) {
+  API_BEGIN
+  std::vector<int> r(rows, rows + nnz), c(cols, cols + nnz);
+  std::vector<double> v;
+  if (vals) v.assign(vals, vals + nnz);
+  M(h).addPrior(std::unique_ptr<ILatentPrior>(new MacauOnePrior(nrow, ncol, r, c, v, vals == nullptr, lambda_beta)));
+  API_END_INT
+}
+
 int macau_set_samples(macau_model* h, int burnin, int nsamples) {
   API_BEGIN
   M(h).setSamples(burnin, nsamples);

@@ -31,6 +31,7 @@ enum : uint32_t {
   TAG_BETA_F = 7u,       // c0 = feat, j = component (link-matrix prior)
   TAG_LAMBDA_BETA = 8u,  // c0 = 0,    gamma for lambda_beta
   TAG_PROBIT = 9u,       // c0 = observation (mode order), c1 = attempt
+  TAG_BETA_ONE = 10u,    // c0 = feature, j = latent (univariate link matrix)
 };
 MGPU_HD uint32_t make_tag(uint32_t stream, uint32_t mode) { return (stream << 24) | mode; }
 

@@ -58,6 +58,10 @@ struct SampleArgs {
   // algebraic train SSE of this half-sweep (batched sampler only), or null
   double* sse_partial = nullptr;
   const float* r2 = nullptr;       // per-row sum of squared residuals
+  // univariate prior (MacauOnePrior, plain sampler only): Lambda_f is
+  // diagonal and each row takes one coordinate-wise (Gauss-Seidel) sweep
+  // from its current value instead of an exact draw
+  int univariate = 0;
 };
 
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
@@ -219,7 +223,8 @@ void launch_ftf(const CsrDev& F, const CsrDev& Ft, double* FtF, hipStream_t s);
 void launch_ftf_solve(const double* FtF, int n, const double* reg, double* W, double* X, int nrhs, int* fail,
                       hipStream_t s);
 constexpr int kSpmmSegNnz = 256;
-// out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr;
+// out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr
+// (any nr; beyond 64 the columns are taken in further passes of 64);
 // reg is a device scalar, read only when add is non-null; nothing is done
 // when skip is non-null and *skip != 0 (a converged block CG)
 void launch_spmm(const CsrDev& A, const double* in, int nr, double* out, const double* reg, const double* add,
@@ -273,5 +278,32 @@ void launch_uhat_finish(const CsrDev& F, const double* beta, int K, int KP, cons
                         float* Uhat, float* Lmu_rows, double* tmp, hipStream_t s);
 void launch_lambda_beta(int K, double nfeat, const double* bbt_partial, int nblk, const double* Lambda,
                         double* lambda_beta, uint64_t seed, const int* iter, int mode, hipStream_t s);
+
+// --- univariate prior MacauOnePrior (macauoneprior.cpp:189-289) ---
+struct BetaUniArgs {
+  int K = 0, KP = 0, N = 0, nfeat = 0;
+  const float* U = nullptr;          // KP x N
+  const double* uhat = nullptr;      // K x N (previous Uhat, FP64)
+  const double* mu = nullptr;        // K
+  const double* Lambda = nullptr;    // K x K, diagonal
+  double* lambda_beta = nullptr;     // K (read by the sweep, written by launch_lambda_beta_uni)
+  const double* colsq = nullptr;     // nfeat: sum of squares of each column of F
+  const int* Ft_rowptr = nullptr;    // F^T as CSR (nfeat rows), no repeated entries in a row
+  const int* Ft_cols = nullptr;
+  const double* Ft_vals = nullptr;   // null = binary
+  double* beta = nullptr;            // K x nfeat
+  double* ainv = nullptr;            // K x nfeat scratch
+  double* nz = nullptr;              // K x nfeat scratch
+  double* Zw = nullptr;              // K x N scratch (used when N exceeds the LDS rows)
+  uint64_t seed = 0;
+  const int* iter = nullptr;
+  int mode = 0;
+};
+// Lambda <- diag(Lambda), Lambda_f <- its diagonal, Lmu_f = lambda mu
+void launch_uni_diag(int K, int KP, double* Lambda, float* Lambda_f, float* Lmu_f, const double* mu, hipStream_t s);
+// sample_beta: the per-latent feature sweeps
+void launch_beta_uni(const BetaUniArgs& a, hipStream_t s);
+void launch_lambda_beta_uni(const BetaUniArgs& a, hipStream_t s);
+bool beta_uni_in_lds(int N);
 
 }  // namespace mgpu

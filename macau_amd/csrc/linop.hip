@@ -23,21 +23,24 @@ __global__ void __launch_bounds__(kSpmmThreads)
   const int lane = threadIdx.x & 63;
   const int w = (blockIdx.x * kSpmmThreads + threadIdx.x) >> 6;
   const int nw = (gridDim.x * kSpmmThreads) >> 6;
-  const int comp = lane % NP, sub = lane / NP;
+  const int sub = lane / NP;
   for (int r = w; r < A.nrow; r += nw) {
-    double acc = 0.0;
     const int beg = A.row_ptr[r], end = A.row_ptr[r + 1];
-    if (comp < nr) {
-      for (int e = beg + sub; e < end; e += S) {
-        const double v = A.vals ? A.vals[e] : 1.0;
-        acc += v * in[(size_t)A.cols[e] * nr + comp];
+    for (int c0 = 0; c0 < nr; c0 += NP) {  // one pass when nr <= NP
+      const int comp = c0 + lane % NP;
+      double acc = 0.0;
+      if (comp < nr) {
+        for (int e = beg + sub; e < end; e += S) {
+          const double v = A.vals ? A.vals[e] : 1.0;
+          acc += v * in[(size_t)A.cols[e] * nr + comp];
+        }
       }
-    }
 #pragma unroll
-    for (int off = NP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
-    if (sub == 0 && comp < nr) {
-      if (add) acc += (*reg) * add[(size_t)r * nr + comp];
-      out[(size_t)r * nr + comp] = acc;
+      for (int off = NP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      if (sub == 0 && comp < nr) {
+        if (add) acc += (*reg) * add[(size_t)r * nr + comp];
+        out[(size_t)r * nr + comp] = acc;
+      }
     }
   }
 }
@@ -54,25 +57,28 @@ __global__ void __launch_bounds__(kSpmmThreads)
   const int lane = threadIdx.x & 63;
   const int w = (blockIdx.x * kSpmmThreads + threadIdx.x) >> 6;
   const int nw = (gridDim.x * kSpmmThreads) >> 6;
-  const int comp = lane % NP, sub = lane / NP;
+  const int sub = lane / NP;
   for (int sg = w; sg < A.nseg; sg += nw) {
     const int4 d = A.seg[sg];  // row, first nnz, end nnz, slot
-    double acc = 0.0;
-    if (comp < nr) {
+    for (int c0 = 0; c0 < nr; c0 += NP) {  // one pass when nr <= NP
+      const int comp = c0 + lane % NP;
+      double acc = 0.0;
+      if (comp < nr) {
 #pragma unroll 4
-      for (int e = d.y + sub; e < d.z; e += S) {
-        const double v = A.vals ? A.vals[e] : 1.0;
-        acc += v * in[(size_t)A.cols[e] * nr + comp];
+        for (int e = d.y + sub; e < d.z; e += S) {
+          const double v = A.vals ? A.vals[e] : 1.0;
+          acc += v * in[(size_t)A.cols[e] * nr + comp];
+        }
       }
-    }
 #pragma unroll
-    for (int off = NP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
-    if (sub == 0 && comp < nr) {
-      if (d.w < 0) {
-        if (add) acc += (*reg) * add[(size_t)d.x * nr + comp];
-        out[(size_t)d.x * nr + comp] = acc;
-      } else {
-        A.partial[(size_t)d.w * nr + comp] = acc;
+      for (int off = NP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      if (sub == 0 && comp < nr) {
+        if (d.w < 0) {
+          if (add) acc += (*reg) * add[(size_t)d.x * nr + comp];
+          out[(size_t)d.x * nr + comp] = acc;
+        } else {
+          A.partial[(size_t)d.w * nr + comp] = acc;
+        }
       }
     }
   }

@@ -305,7 +305,9 @@ void Model::init() {
   build_plans();
   // the batched sampler computes the train SSE of the adaptive noise inside
   // the last half-sweep (u^T G u - 2 u^T c + sum r^2 per row)
-  fused_sse = sampler == SAMPLER_BATCHED && dynamic_cast<AdaptiveGaussianNoise*>(noise.get()) != nullptr;
+  // (not when the last mode is univariate: that mode runs the plain sampler)
+  fused_sse = sampler == SAMPLER_BATCHED && dynamic_cast<AdaptiveGaussianNoise*>(noise.get()) != nullptr &&
+              dynamic_cast<MacauOnePrior*>(priors[nmodes - 1].get()) == nullptr;
   if (fused_sse) {
     const int last = nmodes - 1;
     const HostRows& h = rows[last];
@@ -1002,6 +1004,120 @@ void MacauPrior::saveModel(Model& m, int mode, const std::string& prefix) {  // 
   HIPCHK(hipMemcpyAsync(h.data(), beta.as<void>(), h.size() * sizeof(double), hipMemcpyDeviceToHost, m.stream));
   HIPCHK(hipStreamSynchronize(m.stream));
   write_csv(prefix + "-link.csv", h, m.num_latent, nfeat);
+}
+
+// ---- MacauOnePrior ----
+MacauOnePrior::MacauOnePrior(int nr, int nc, const std::vector<int>& rows, const std::vector<int>& cols,
+                             const std::vector<double>& vals, bool bin, double lb)
+    : MacauPrior(nr, nc, rows, cols, vals, bin, lb, 1e-6) {
+  // col_square_sum (linop.cpp:374-401): entries counted (binary) or squared
+  colsq.assign(nc, 0.0);
+  for (int f = 0; f < nc; f++)
+    for (int e = Ft_rowptr[f]; e < Ft_rowptr[f + 1]; e++) colsq[f] += Ft_vals[e] * Ft_vals[e];
+  // F^T with the repeated entries of each row merged (values added), so the
+  // sweep's scatter has distinct rows per column; At_mul_Bt and
+  // add_Acol_mul_bt (linop.h:121-178) give the same sums
+  Fu_rowptr.assign(nc + 1, 0);
+  Fu_cols.clear();
+  Fu_vals.clear();
+  std::vector<std::pair<int, double>> ent;
+  for (int f = 0; f < nc; f++) {
+    ent.clear();
+    for (int e = Ft_rowptr[f]; e < Ft_rowptr[f + 1]; e++) ent.emplace_back(Ft_cols[e], Ft_vals[e]);
+    std::stable_sort(ent.begin(), ent.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+      return x.first < y.first;
+    });
+    for (size_t q = 0; q < ent.size(); q++) {
+      if (q > 0 && ent[q].first == ent[q - 1].first) {
+        Fu_vals.back() += ent[q].second;
+      } else {
+        Fu_cols.push_back(ent[q].first);
+        Fu_vals.push_back(ent[q].second);
+      }
+    }
+    Fu_rowptr[f + 1] = (int)Fu_cols.size();
+  }
+}
+
+void MacauOnePrior::init(Model& m, int mode) {  // macauoneprior.cpp:16-38
+  ILatentPrior::init(m, mode);  // mu = 0, Lambda = 10 I (lambda = 10)
+  if (nrow != m.dims[mode])
+    throw std::runtime_error("side information rows must equal the size of its data mode");
+  const int K = m.num_latent, KP = m.KP, N = nrow;
+  upload_side(K, m.stream);
+  const size_t kf = (size_t)K * nfeat;
+  beta.alloc(std::max<size_t>(kf, 1) * sizeof(double));
+  HIPCHK(hipMemsetAsync(beta.as<void>(), 0, beta.bytes(), m.stream));
+  Uhat.alloc((size_t)KP * N * sizeof(float));
+  HIPCHK(hipMemsetAsync(Uhat.as<void>(), 0, Uhat.bytes(), m.stream));
+  Lmu_rows.alloc((size_t)KP * N * sizeof(float));
+  HIPCHK(hipMemsetAsync(Lmu_rows.as<void>(), 0, Lmu_rows.bytes(), m.stream));
+  uhat64.alloc(std::max<size_t>((size_t)K * N, 1) * sizeof(double));
+  HIPCHK(hipMemsetAsync(uhat64.as<void>(), 0, uhat64.bytes(), m.stream));
+  upload(lambda_beta_vec, std::vector<double>(K, lambda_beta0), m.stream);  // setLambdaBeta
+  upload(dcolsq, colsq.empty() ? std::vector<double>{0.0} : colsq, m.stream);
+  upload(dFu_rowptr, Fu_rowptr, m.stream);
+  upload(dFu_cols, Fu_cols.empty() ? std::vector<int>{0} : Fu_cols, m.stream);
+  upload(dFu_vals, Fu_vals.empty() ? std::vector<double>{0.0} : Fu_vals, m.stream);
+  ainv.alloc(std::max<size_t>(kf, 1) * sizeof(double));
+  nz.alloc(std::max<size_t>(kf, 1) * sizeof(double));
+  Zw.alloc(beta_uni_in_lds(N) ? sizeof(double) : (size_t)K * N * sizeof(double));
+  partial_sumsq.alloc(256 * sizeof(double));
+  HIPCHK(hipStreamSynchronize(m.stream));
+}
+
+void MacauOnePrior::sample_latents(Model& m, int mode) {  // macauoneprior.cpp:42-95, 108-187
+  SampleArgs a = m.sample_args(mode, *this);
+  a.univariate = 1;
+  launch_sample_plain(a, m.stream);
+}
+
+void MacauOnePrior::update_prior(Model& m, int mode) {  // macauoneprior.cpp:189-196
+  const int K = m.num_latent, KP = m.KP, N = nrow;
+  hipStream_t s = m.stream;
+  // sample_mu_lambda (:198-213): CondNormalWishart(U - Uhat, 0, 2, I, K), diagonal kept
+  launch_hyper_stats_f32(m.d_U[mode].as<float>(), Uhat.as<float>(), KP, N, K, stats.as<double>(), stats_nblk, s);
+  launch_nw_draw(nw_args(m, *this, mode, (double)K), s);
+  launch_uni_diag(K, KP, Lambda.as<double>(), Lambda_f.as<float>(), Lmu_f.as<float>(), mu.as<double>(), s);
+  // sample_beta (:215-252) from the previous Uhat
+  BetaUniArgs b;
+  b.K = K;
+  b.KP = KP;
+  b.N = N;
+  b.nfeat = nfeat;
+  b.U = m.d_U[mode].as<float>();
+  b.uhat = uhat64.as<double>();
+  b.mu = mu.as<double>();
+  b.Lambda = Lambda.as<double>();
+  b.lambda_beta = lambda_beta_vec.as<double>();
+  b.colsq = dcolsq.as<double>();
+  b.Ft_rowptr = dFu_rowptr.as<int>();
+  b.Ft_cols = dFu_cols.as<int>();
+  b.Ft_vals = dFu_vals.as<double>();
+  b.beta = beta.as<double>();
+  b.ainv = ainv.as<double>();
+  b.nz = nz.as<double>();
+  b.Zw = Zw.as<double>();
+  b.seed = m.seed;
+  b.iter = m.d_iter.as<int>();
+  b.mode = mode;
+  launch_beta_uni(b, s);
+  // compute_uhat, with the per-row prior term lambda (mu + Uhat_n) for the
+  // next half-sweep; the FP64 Uhat stays in uhat64 for the next sweep
+  launch_uhat_finish(F(), beta.as<double>(), K, KP, mu.as<double>(), Lambda.as<double>(), Uhat.as<float>(),
+                     Lmu_rows.as<float>(), uhat64.as<double>(), s);
+  // sample_lambda_beta (:263-289)
+  launch_lambda_beta_uni(b, s);
+}
+
+double MacauOnePrior::getLinkLambda(Model& m) {
+  std::vector<double> h(m.num_latent);
+  HIPCHK(hipMemcpyAsync(h.data(), lambda_beta_vec.as<void>(), h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        m.stream));
+  HIPCHK(hipStreamSynchronize(m.stream));
+  double s = 0.0;
+  for (double x : h) s += x;
+  return h.empty() ? NAN : s / (double)h.size();
 }
 
 // ------------------------------------------------------------------

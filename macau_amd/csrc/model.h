@@ -124,6 +124,26 @@ class MacauPrior : public ILatentPrior {
   std::vector<DevBuf> F_plan_bufs, Ft_plan_bufs;
 };
 
+// MacauOnePrior<SparseDoubleFeat> (macauoneprior.h:8-51): diagonal prior
+// precision, coordinate-wise row updates (the plain sampler's univariate
+// variant) and a feature-sequential link-matrix sweep per latent.  Shares
+// the side-information storage, beta, Uhat and the per-row prior term of
+// MacauPrior.
+class MacauOnePrior : public MacauPrior {
+ public:
+  MacauOnePrior(int nrow, int ncol, const std::vector<int>& rows, const std::vector<int>& cols,
+                const std::vector<double>& vals, bool binary, double lambda_beta);
+  void init(Model& m, int mode) override;
+  void sample_latents(Model& m, int mode) override;
+  void update_prior(Model& m, int mode) override;
+  double getLinkLambda(Model& m) override;  // lambda_beta.mean() (macauoneprior.h:45)
+
+  std::vector<double> colsq;           // col_square_sum(F) (linop.cpp:374-401)
+  std::vector<int> Fu_rowptr, Fu_cols;  // F^T with repeated entries of a row merged
+  std::vector<double> Fu_vals;
+  DevBuf dFu_rowptr, dFu_cols, dFu_vals, dcolsq, lambda_beta_vec, uhat64, ainv, nz, Zw;
+};
+
 // noisemodels.h:13-29
 class INoiseModel {
  public:

@@ -15,6 +15,16 @@
 // along as an extra column; the back solve reads L row-major from LDS.
 // Rows are processed in natural order with no load balancing: this is the
 // plain baseline the batched sampler is measured against.
+//
+// Univariate variant (MacauOnePrior::sample_latents, macauoneprior.cpp:42-95
+// and 108-187): the reference updates u_d for d = 0..K-1 in turn,
+//   u_d ~ N(L_d / Q_d, 1 / Q_d),  Q_d = lambda_d + alpha sum_e c_ed^2,
+//   L_d = lambda_d m_d + alpha sum_e (y_e - mean - sum_{j != d} u_j c_ej) c_ed,
+// with u_j already updated for j < d.  With P and b assembled as above
+// (Lambda = diag(lambda), b = h) this is one Gauss-Seidel sweep from the
+// row's current value: u_d = (h_d - sum_{j != d} P_dj u_j) / P_dd +
+// z_d / sqrt(P_dd).  Lane j holds row j of P and its own u_j, so step d is
+// one lane-group sum of P_jd u_j and an update on lane d.
 #include "common.h"
 #include "kernels.h"
 
@@ -29,7 +39,7 @@ __device__ __forceinline__ float gather_comp(const FactorSet& F, const int* __re
   return x;
 }
 
-template <int KP, int NO>
+template <int KP, int NO, bool UNI>
 __global__ void __launch_bounds__(KP < 64 ? 64 : KP) k_sample_plain(SampleArgs a) {
   constexpr int WG = KP < 64 ? 64 : KP;
   constexpr int G = WG / KP;
@@ -91,6 +101,57 @@ __global__ void __launch_bounds__(KP < 64 ? 64 : KP) k_sample_plain(SampleArgs a
     b += a.Lmu_f[i];
   }
 
+  if constexpr (UNI) {
+    // ---- one Gauss-Seidel sweep from the current value ----
+    __shared__ float part[2][2];
+    float u = active ? a.U[(size_t)row * KP + i] : 0.f;
+    const float z =
+        i < a.K ? normal_f32(a.seed, (uint32_t)row, (uint32_t)i, (uint32_t)it, make_tag(TAG_LATENT, a.mode)) : 0.f;
+    if constexpr (KP <= 64) {
+      // the KP-lane group is inside one wave: P's row in registers
+#pragma unroll
+      for (int d = 0; d < KP; d++) {
+        if (d >= a.K) continue;  // uniform across the work-group
+        float t = acc[d] * u;  // P[i][d] u_i = P[d][i] u_i
+#pragma unroll
+        for (int off = 1; off < KP; off <<= 1) t += __shfl_xor(t, off, KP);
+        if (i == d) {
+          float pdd = acc[d];
+          if (!(pdd > 0.f)) {
+            if (active) atomicMax(a.fail, row + 1);
+            pdd = 1.f;
+          }
+          u = (b - t + pdd * u) / pdd + z / sqrtf(pdd);
+        }
+      }
+    } else {
+      // two waves per row: P's rows staged in LDS, wave partials combined
+      // through LDS (double-buffered, one barrier per step)
+#pragma unroll
+      for (int j = 0; j < KP; j++) Lrow[0][i][j] = acc[j];
+      __syncthreads();
+      for (int d = 0; d < a.K; d++) {
+        const float pid = Lrow[0][i][d];
+        float t = pid * u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) t += __shfl_xor(t, off, 64);
+        if ((tid & 63) == 0) part[d & 1][tid >> 6] = t;
+        __syncthreads();
+        t = part[d & 1][0] + part[d & 1][1];
+        if (i == d) {
+          float pdd = pid;
+          if (!(pdd > 0.f)) {
+            if (active) atomicMax(a.fail, row + 1);
+            pdd = 1.f;
+          }
+          u = (b - t + pdd * u) / pdd + z / sqrtf(pdd);
+        }
+      }
+    }
+    if (active) a.U[(size_t)row * KP + i] = u;
+    return;
+  }
+
   // ---- Cholesky with fused forward solve ----
 #pragma unroll
   for (int k = 0; k < KP; k++) {
@@ -140,7 +201,10 @@ static void launch_plain_t(const SampleArgs& a, hipStream_t s) {
   constexpr int G = WG / KP;
   const int grid = (a.rows.N + G - 1) / G;
   if (grid == 0) return;
-  hipLaunchKernelGGL((k_sample_plain<KP, NO>), dim3(grid), dim3(WG), 0, s, a);
+  if (a.univariate)
+    hipLaunchKernelGGL((k_sample_plain<KP, NO, true>), dim3(grid), dim3(WG), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sample_plain<KP, NO, false>), dim3(grid), dim3(WG), 0, s, a);
 }
 
 template <int KP>

@@ -14,8 +14,8 @@ Differences from the reference, all additive:
 Side information with at most 10000 features solves the link matrix through
 the dense F'F Cholesky branch (latentprior.cpp:280-289), larger side
 information through the block CG, as in the reference.  ``precision="probit"``
-runs ProbitNoise on matrix data.
-Not provided: ``univariate=True`` (MacauOnePrior).
+runs ProbitNoise on matrix data; ``univariate=True`` puts MacauOnePrior on
+the modes with side information, as the reference does.
 """
 import ctypes
 import numbers
@@ -243,6 +243,23 @@ class Model:
         check(L.macau_add_macau_prior_ff(self.h, side.shape[0], side.shape[1], len(v), ptr(r), ptr(c), ptr(v),
                                          float(lambda_beta), float(tol), int(max_ff_size)))
 
+    def add_one_prior(self, side, lambda_beta):
+        """make_one_prior (macau.pyx:194-214): BPMF for no side information,
+        else the univariate MacauOnePrior on sparse side information."""
+        L = _lib.lib()
+        if (side is None) or (isinstance(side, tuple) and side == ()):
+            check(L.macau_add_bpmf_prior(self.h))
+            return
+        if not _is_sparse_matrix(side):
+            raise ValueError("Unsupported side information type: '%s'" % type(side).__name__)
+        X = side.tocoo(copy=False)
+        r = np.ascontiguousarray(X.row, dtype=np.int32)
+        c = np.ascontiguousarray(X.col, dtype=np.int32)
+        v = np.ascontiguousarray(X.data, dtype=np.float64)
+        binary = bool(np.all(v == 1))
+        check(L.macau_add_macau_one_prior(self.h, side.shape[0], side.shape[1], len(v), ptr(r), ptr(c),
+                                          None if binary else ptr(v), float(lambda_beta)))
+
     def get_dims(self):
         d = np.zeros(self.nmodes, dtype=np.int32)
         check(_lib.lib().macau_get_dims(self.h, ptr(d)))
@@ -298,7 +315,8 @@ def macau(Y,
                    "adaptive" - automatically learn precision
       burnin       number of Gibbs samples to drop (burn-in)
       nsamples     number of Gibbs samples to collect
-      univariate   not provided by this implementation (raises)
+      univariate   use univariate (coordinate-wise) sampler for the modes with
+                   side information (MacauOnePrior)
       tol          error tolerance for conjugate gradient solver
       sn_max       maximum signal-to-noise ratio for observation precision
                    only used if precision="adaptive"
@@ -317,18 +335,21 @@ def macau(Y,
     if len(side) != len(data.shape):
         raise ValueError("Length of 'side' is %d but must be equal to the number of data dimensions (%d)." %
                          (len(side), len(data.shape)))
-    if univariate:
-        raise NotImplementedError("the univariate sampler (MacauOnePrior) is not provided by macau_amd")
     if sampler != "auto" and sampler not in _SAMPLERS:
         raise ValueError("sampler must be one of %s" % sorted(list(_SAMPLERS) + ["auto"]))
 
     D = int(np.int32(num_latent))
     nmodes = len(data.shape)
+    if univariate and isinstance(precision, str) and precision == "probit":
+        raise ValueError("Univariate sampler for probit model is not yet implemented.")
     model = Model(nmodes, D, precision, sn_max)
     L = _lib.lib()
     model.set_data(data)
     for s in side:
-        model.add_prior(s, np.float64(lambda_beta), np.float64(tol))
+        if univariate:
+            model.add_one_prior(s, np.float64(lambda_beta))
+        else:
+            model.add_prior(s, np.float64(lambda_beta), np.float64(tol))
 
     check(L.macau_set_samples(model.h, int(burnin), int(nsamples)))
     check(L.macau_set_verbose(model.h, int(bool(verbose))))

@@ -143,6 +143,14 @@ class GpuModel:
                 check(L.macau_add_macau_prior_ff(self.h, nr, nc, len(r), ptr(r), ptr(c), ptr(v), lambda_beta, tol,
                                                  max_ff_size))
 
+    def one_prior(self, side, lambda_beta=5.0):
+        """MacauOnePrior (univariate) on COO side information (r, c, v or None, nrow, ncol)."""
+        r, c, v, nr, nc = side
+        r = np.ascontiguousarray(r, dtype=np.int32)
+        c = np.ascontiguousarray(c, dtype=np.int32)
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        check(_lib.lib().macau_add_macau_one_prior(self.h, nr, nc, len(r), ptr(r), ptr(c), ptr(v), lambda_beta))
+
     def setup(self, burnin, nsamples, seed, sampler, graph=True):
         L = _lib.lib()
         check(L.macau_set_samples(self.h, burnin, nsamples))

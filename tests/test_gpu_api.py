@@ -1,9 +1,8 @@
 """The Python entry points on the GPU, mirroring the reference's own tests
 (python/macau/test_macau.py).  Every case runs the full Gibbs chain through
 the HIP library; quality bounds are the reference's (tensor RMSE < 0.5 at
-precision 50, test_macau.py:132-207; probit AUC > 0.55, test_macau.py:47-62).
-The case the reference runs with the univariate sampler checks that it
-raises NotImplementedError, since this library does not provide it."""
+precision 50, test_macau.py:132-207; probit AUC > 0.55, test_macau.py:47-62;
+univariate tensor RMSE < 0.5, test_macau.py:191-207)."""
 import itertools
 import os
 
@@ -105,10 +104,42 @@ def test_bpmf_emptytest_probit():  # test_macau.py:84-88
     assert r0.prediction.shape[0] == 0
 
 
-def test_univariate_not_provided():  # test_macau.py:64-73
+def test_macau_univariate():  # test_macau.py:64-73
+    announce("api univariate sampler")
     Y = _rand(10, 20, 0.2, 9)
-    with pytest.raises(NotImplementedError):
-        macau_amd.bpmf(Y, verbose=False, univariate=True, burnin=2, nsamples=2)
+    np.random.seed(9)
+    Y, Ytest = macau_amd.make_train_test(Y, 0.5)
+    side1 = scipy.sparse.coo_matrix(np.random.rand(10, 2))
+    side2 = scipy.sparse.coo_matrix(np.random.rand(20, 3))
+    results = macau_amd.bpmf(Y, Ytest=Ytest, side=[side1, side2], num_latent=4, verbose=False, burnin=50,
+                             nsamples=50, univariate=True, seed=9)
+    assert Ytest.nnz == results.prediction.shape[0]
+    assert np.isfinite(results.rmse_test)
+
+
+def test_macau_tensor_univariate():  # test_macau.py:191-207
+    announce("api univariate tensor")
+    rng = np.random.default_rng(23)
+    A = rng.standard_normal((30, 2))
+    B = rng.standard_normal((4, 2))
+    C = rng.standard_normal((2, 2))
+    idx = list(itertools.product(np.arange(A.shape[0]), np.arange(B.shape[0]), np.arange(C.shape[0])))
+    df = pd.DataFrame(np.asarray(idx), columns=["A", "B", "C"])
+    df["value"] = np.array([np.sum(A[i[0], :] * B[i[1], :] * C[i[2], :]) for i in idx])
+    np.random.seed(23)
+    Ytrain, Ytest = macau_amd.make_train_test_df(df, 0.2)
+    Acoo = scipy.sparse.coo_matrix(A)
+    results = macau_amd.macau(Y=Ytrain, Ytest=Ytest, side=[Acoo, None, None], num_latent=4, verbose=False,
+                              burnin=20, nsamples=20, univariate=True, precision=50, seed=23)
+    assert results.rmse_test < 0.5, "RMSE %f" % results.rmse_test
+
+
+def test_univariate_errors():  # macau.pyx:194-199, 393-395
+    Y = _rand(10, 20, 0.2, 9)
+    with pytest.raises(ValueError):
+        macau_amd.bpmf(Y, verbose=False, univariate=True, precision="probit", burnin=2, nsamples=2)
+    with pytest.raises(ValueError):
+        macau_amd.macau(Y, side=[np.ones((10, 2)), None], verbose=False, univariate=True, burnin=2, nsamples=2)
 
 
 def test_bpmf_emptytest():  # test_macau.py:80-82

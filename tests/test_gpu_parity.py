@@ -291,6 +291,42 @@ def test_model_macau_ftf_matches_oracle(nfeat, binary):
     assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
 
 
+@pytest.mark.parametrize("ftf", [True, False])
+def test_model_macau_wide_latent(ftf):
+    """K = 70 (more than 64 right-hand sides in the SpMMs of sample_beta and
+    compute_uhat), with the F^T F branch or the block CG (blocks 32 + 38)."""
+    announce("model macau K=70 ftf=%d" % ftf)
+    rng = np.random.default_rng(70 + ftf)
+    nrow, ncol, K, nfeat = 200, 80, 70, 40
+    Fm = rng.random((nrow, nfeat)) < 0.1
+    fr, fc = np.nonzero(Fm)
+    Ut = Fm.astype(float) @ rng.standard_normal((nfeat, 3)) / 2
+    Vt = rng.standard_normal((ncol, 3))
+    r, c = np.nonzero(rng.random((nrow, ncol)) < 0.2)
+    y = np.sum(Ut[r] * Vt[c], 1) + 0.3 * rng.standard_normal(len(r))
+    perm = rng.permutation(len(r))
+    te, tr = perm[:len(r) // 5], perm[len(r) // 5:]
+    m = GpuModel(2, K, 10.0)
+    m.data(np.stack([r[tr], c[tr]], 1), y[tr], [nrow, ncol], np.stack([r[te], c[te]], 1), y[te])
+    m.prior(side=(fr, fc, None, nrow, nfeat), lambda_beta=5.0, tol=1e-6, max_ff_size=10000 if ftf else None)
+    m.prior()
+    m.setup(3, 4, SEED, PLAIN, graph=False)
+    modes, mean, nnz = O.matrix_rows(r[tr], c[tr], y[tr], nrow, ncol)
+    ti, tv = O.matrix_test(r[te], c[te], y[te], nrow, ncol)
+    side = [O.csr_pair(fr, fc, np.ones(len(fr)), nrow, nfeat), None]
+    kw = dict(precision=10.0, side=side, lambda_beta=5.0, tol=1e-6, burnin=3, nsamples=4, seed=SEED,
+              use_ftf=[ftf, False])
+    m.iterate(2)
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, max_iters=2, want_U=True, **kw)
+    e0, e1 = row_rel_err(m.latents(0, nrow), ref.U[0]), row_rel_err(m.latents(1, ncol), ref.U[1])
+    print("macau K=70 ftf=%d iter 2: row err %.3e %.3e" % (ftf, e0, e1))
+    # the block CG stops at tolerance 1e-6: its solution error bounds the parity
+    tol = TOL_FP32 if ftf else 1e-3
+    assert e0 < tol and e1 < tol
+    lb = _lib.lib().macau_get_link_lambda(m.h, 0)
+    assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < tol
+
+
 # ------------------------------------------------------------ batched sampler
 BATCHED = _lib.SAMPLER_BATCHED
 # Split-BF16 Gram (hi*hi + hi*lo + lo*hi, FP32 accumulate) keeps ~16 bits per

@@ -54,10 +54,12 @@ def test_side_must_be_list():
         macau_amd.macau(Y, verbose=False, side="abc")
 
 
-def test_univariate_not_provided():
+def test_univariate_argument_errors():  # macau.pyx:194-199, 393-395
     Y = scipy.sparse.rand(10, 20, 0.2)
-    with pytest.raises(NotImplementedError):
-        macau_amd.macau(Y, verbose=False, univariate=True)
+    with pytest.raises(ValueError, match="probit"):
+        macau_amd.macau(Y, verbose=False, univariate=True, precision="probit")
+    with pytest.raises(ValueError, match="Unsupported side information type"):
+        macau_amd.macau(Y, side=[np.ones((10, 2)), None], verbose=False, univariate=True)
 
 
 def test_data_sparse_layout():
