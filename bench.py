@@ -28,6 +28,9 @@ Other workloads (`--workload`, one JSON line each, not the headline):
       features, as the Python API chooses); reports the FP64 rate of the
       per-iteration factorisation;
   c4  3-mode tensor 50,000 x 2,000 x 100, 10M cells, K = 64;
+  c3u the c3 data with the univariate prior (MacauOnePrior) on the rows:
+      coordinate-wise row updates and, per latent, a sequential sweep over
+      the 100,000 features for the link matrix;
   c2p the c2 cells binarised at the train mean, probit noise (the
       augmentation kernel draws one truncated normal per observation before
       each half-sweep); the full-chain line reports test AUC instead of RMSE.
@@ -74,10 +77,13 @@ def load_workload(name, seed):
                     tvals=(tvals > cut).astype(np.float64), dims=dims, settings=settings, side=[None, None],
                     label="BPMF probit 138000x27000, 20M observed (18M train / 2M test) binarised at the train "
                           "mean, Zipf-1.1 degrees, K=32")
-    if name in ("c3", "c3f"):
-        nfeat = 100000 if name == "c3" else 4000
+    if name in ("c3", "c3f", "c3u"):
+        nfeat = 4000 if name == "c3f" else 100000
         (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c3(seed, nfeat=nfeat)
         how = "CG tol 1e-6" if nfeat > MAX_FF_SIZE else "F^T F Cholesky"
+        if name == "c3u":
+            settings = dict(settings, univariate=True)
+            how = "univariate prior (coordinate-wise rows, feature-sequential link matrix)"
         return dict(idx=idx, vals=vals, tidx=tidx, tvals=tvals, dims=dims, settings=settings, side=side,
                     label="Macau 15000x350, 60k observed, %d-dim binary row features (~%d nnz), K=32, "
                           "lambda_beta 5, %s" % (nfeat, side[0].nnz, how))
@@ -116,8 +122,12 @@ def build_model(w, sampler_kind, seed, graph=True):
             r = np.ascontiguousarray(s.row, dtype=np.int32)
             c = np.ascontiguousarray(s.col, dtype=np.int32)
             v = np.ascontiguousarray(s.data, dtype=np.float64)
-            check(L.macau_add_macau_prior_ff(h, s.shape[0], s.shape[1], len(v), ptr(r), ptr(c), ptr(v),
-                                             float(st["lambda_beta"]), float(st["tol"]), MAX_FF_SIZE))
+            if st.get("univariate"):
+                check(L.macau_add_macau_one_prior(h, s.shape[0], s.shape[1], len(v), ptr(r), ptr(c), ptr(v),
+                                                  float(st["lambda_beta"])))
+            else:
+                check(L.macau_add_macau_prior_ff(h, s.shape[0], s.shape[1], len(v), ptr(r), ptr(c), ptr(v),
+                                                 float(st["lambda_beta"]), float(st["tol"]), MAX_FF_SIZE))
     check(L.macau_set_samples(h, st["burnin"], st["nsamples"]))
     check(L.macau_set_seed(h, seed))
     check(L.macau_set_sampler(h, sampler_kind))
@@ -229,7 +239,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c4", "c2p"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c3u", "c4", "c2p"], default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
@@ -300,7 +310,14 @@ def main():
         "phase_ms": phase_ms,
         "roofline": roof,
     }
-    if w["side"][0] is not None and w["side"][0].shape[1] <= MAX_FF_SIZE:
+    if w["side"][0] is not None and st.get("univariate"):
+        # the row prior update is dominated by the per-latent feature sweeps
+        F = w["side"][0]
+        out["univariate_sweep"] = {"features": F.shape[1], "latents": K, "prior_rows_ms": float(ph[nm]),
+                                   "us_per_feature_step": float(ph[nm]) * 1e3 / max(F.shape[1], 1),
+                                   "note": "one wave per latent walks the features in order; time includes the "
+                                           "Normal-Wishart draw, the parallel pre-pass, Uhat and lambda_beta"}
+    elif w["side"][0] is not None and w["side"][0].shape[1] <= MAX_FF_SIZE:
         # the row prior update is dominated by the Cholesky of F^T F + lambda_beta I
         nf = w["side"][0].shape[1]
         tf = (nf ** 3 / 3.0) / (ph[nm] * 1e-3) / 1e12

@@ -13,11 +13,12 @@
 //
 // The feature chain is sequential by construction, so it runs one wave per
 // latent with z in LDS (rows up to kUniLdsRows; a global buffer beyond).
-// Everything a step needs that does not depend on the chain -- 1 / A, the
-// scaled normal sqrt(1/A) n_df, colsq_f, the old beta_df and the first 64
-// entries of the next column -- is computed by a parallel pre-pass or
-// prefetched one step ahead, leaving the LDS gather, one wave sum, three
-// FP64 operations and the LDS scatter on the chain's critical path.
+// Everything a step needs that does not depend on the chain is off its
+// critical path: 1 / A and the scaled normal sqrt(1/A) n_df come from a
+// parallel pre-pass; the per-feature values of 64 features are loaded one
+// per lane, a chunk ahead, and read with readlane; the first 64 entries of
+// each column are loaded kUniRing features ahead.  A step is then the LDS
+// gather, one wave sum, three FP64 operations and the LDS scatter.
 #include <math.h>
 
 #include <type_traits>
@@ -56,16 +57,67 @@ __global__ void __launch_bounds__(256) k_beta_uni_pre(BetaUniArgs a) {
   }
 }
 
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
-  return x;
-}
 
 constexpr int kUniLdsRows = 19456;  // 152 KB of FP64 residuals
 
-// z in LDS, or (LDS = false) in global memory accessed as volatile so that
-// every gather sees the previous step's scatter by other lanes
+// Per-feature values of 64 consecutive features, lane j holding feature
+// f0 + j (zero past the end).
+struct UniChunk {
+  int rb, re;                  // column extent in F^T
+  double cs, bo, ai, nz;       // colsq, old beta, 1 / A, sqrt(1 / A) n
+};
+
+__device__ __forceinline__ UniChunk load_chunk(const BetaUniArgs& a, int d, int f0, int lane) {
+  UniChunk c;
+  const int f = f0 + lane;
+  const bool ok = f < a.nfeat;
+  c.rb = ok ? a.Ft_rowptr[f] : 0;
+  c.re = ok ? a.Ft_rowptr[f + 1] : 0;
+  c.cs = ok ? a.colsq[f] : 0.0;
+  c.bo = ok ? a.beta[(size_t)f * a.K + d] : 0.0;
+  c.ai = ok ? a.ainv[(size_t)d * a.nfeat + f] : 0.0;
+  c.nz = ok ? a.nz[(size_t)d * a.nfeat + f] : 0.0;
+  return c;
+}
+
+__device__ __forceinline__ int rdl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ __forceinline__ double rdl(double x, int j) {
+  const int2 v = *reinterpret_cast<int2*>(&x);
+  int2 r;
+  r.x = __builtin_amdgcn_readlane(v.x, j);
+  r.y = __builtin_amdgcn_readlane(v.y, j);
+  return *reinterpret_cast<double*>(&r);
+}
+
+// x moved across lanes by one DPP pattern (both 32-bit halves); lanes
+// outside ROWMASK receive 0
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int2 v = *reinterpret_cast<int2*>(&x);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(0, v.x, CTRL, ROWMASK, 0xF, false);
+  r.y = __builtin_amdgcn_update_dpp(0, v.y, CTRL, ROWMASK, 0xF, false);
+  return *reinterpret_cast<double*>(&r);
+}
+
+// sum over the wave in a fixed order, returned in every lane: pairs and
+// quads (quad_perm), 8 and 16 lanes (half-row and row mirror), then rows
+// combined by the row broadcasts into lane 63, which readlane distributes
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+  x += dpp_f64<0x140, 0xF>(x);  // row_mirror
+  x += dpp_f64<0x142, 0xA>(x);  // row_bcast15 into rows 1 and 3
+  x += dpp_f64<0x143, 0xC>(x);  // row_bcast31 into rows 2 and 3
+  return rdl(x, 63);
+}
+
+constexpr int kUniRing = 8;  // columns whose first 64 entries are in flight
+
+// z in LDS (one wave: its LDS operations complete in issue order, so a step
+// sees the previous step's scatter without a barrier), or (LDS = false) in
+// global memory accessed as volatile with a barrier after each scatter
 template <bool LDS>
 __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
   using ZPtr = typename std::conditional<LDS, double*, volatile double*>::type;
@@ -76,56 +128,69 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
   const double mud = a.mu[d], lamd = a.Lambda[d + d * K];
   for (int i = lane; i < N; i += 64) Z[i] = (double)a.U[(size_t)i * a.KP + d] - mud - a.uhat[(size_t)i * K + d];
   __syncthreads();
-  const int* __restrict__ rp = a.Ft_rowptr;
   const int* __restrict__ cols = a.Ft_cols;
   const double* __restrict__ vals = a.Ft_vals;
-  const int nnzF = rp[nfeat];
-  const double* ainv = a.ainv + (size_t)d * nfeat;
-  const double* nz = a.nz + (size_t)d * nfeat;
-  // state of feature f, loaded one step ahead
-  int beg = rp[0], end = nfeat > 0 ? rp[1] : 0;
-  int c0 = 0;
-  double v0 = 0.0;
-  if (beg + lane < nnzF) {
-    c0 = cols[beg + lane];
-    v0 = vals ? vals[beg + lane] : 1.0;
+
+  UniChunk cur = load_chunk(a, d, 0, lane);
+  UniChunk nxt = load_chunk(a, d, 64, lane);
+  // ring slot q holds the first 64 entries of feature f with f % kUniRing == q
+  int rc[kUniRing];
+  double rv[kUniRing];
+#pragma unroll
+  for (int q = 0; q < kUniRing; q++) {
+    const int b = rdl(cur.rb, q), e = rdl(cur.re, q);
+    rc[q] = b + lane < e ? cols[b + lane] : 0;
+    rv[q] = b + lane < e ? (vals ? vals[b + lane] : 1.0) : 0.0;
   }
-  double cs = nfeat > 0 ? a.colsq[0] : 0.0, bold = nfeat > 0 ? a.beta[d] : 0.0;
-  double ai = nfeat > 0 ? ainv[0] : 0.0, nzf = nfeat > 0 ? nz[0] : 0.0;
-  for (int f = 0; f < nfeat; f++) {
-    // prefetch feature f + 1 (its column starts where this one ends)
-    const bool more = f + 1 < nfeat;
-    const int nend = more ? rp[f + 2] : end;
-    int nc0 = 0;
-    double nv0 = 0.0;
-    if (end + lane < nnzF) {
-      nc0 = cols[end + lane];
-      nv0 = vals ? vals[end + lane] : 1.0;
+  for (int f0 = 0; f0 < nfeat; f0 += 64) {
+    double bnew_mine = cur.bo;
+    const int nloc = nfeat - f0 < 64 ? nfeat - f0 : 64;
+    for (int j0 = 0; j0 < nloc; j0 += kUniRing) {
+#pragma unroll
+      for (int q = 0; q < kUniRing; q++) {
+        const int j = j0 + q;
+        if (j < nloc) {
+          const int beg = rdl(cur.rb, j), end = rdl(cur.re, j);
+          const int c0 = rc[q];
+          const double v0 = rv[q];
+          // refill the slot with feature f0 + j + kUniRing (this chunk or the next)
+          {
+            const int jn = j + kUniRing;
+            int b, e;
+            if (jn < 64) {
+              b = rdl(cur.rb, jn);
+              e = rdl(cur.re, jn);
+            } else {
+              b = rdl(nxt.rb, jn - 64);
+              e = rdl(nxt.re, jn - 64);
+            }
+            rc[q] = b + lane < e ? cols[b + lane] : 0;
+            rv[q] = b + lane < e ? (vals ? vals[b + lane] : 1.0) : 0.0;
+          }
+          const bool live = beg + lane < end;
+          const double zc = live ? (double)Z[c0] : 0.0;
+          double zx = v0 * zc;
+          for (int e = beg + 64 + lane; e < end; e += 64) zx += (vals ? vals[e] : 1.0) * Z[cols[e]];
+          zx = wave_sum(zx);
+          const double bold = rdl(cur.bo, j), cs = rdl(cur.cs, j);
+          const double B = lamd * (zx + bold * cs);
+          const double bnew = B * rdl(cur.ai, j) + rdl(cur.nz, j);
+          const double delta = bold - bnew;
+          if (lane == j) bnew_mine = bnew;
+          // z += F_f delta (the column's rows are distinct: no two lanes collide)
+          if (live) Z[c0] = zc + v0 * delta;
+          for (int e = beg + 64 + lane; e < end; e += 64) Z[cols[e]] += (vals ? vals[e] : 1.0) * delta;
+          if (LDS) {
+            __builtin_amdgcn_wave_barrier();
+          } else {
+            __syncthreads();
+          }
+        }
+      }
     }
-    const double ncs = more ? a.colsq[f + 1] : 0.0;
-    const double nbold = more ? a.beta[(size_t)(f + 1) * K + d] : 0.0;
-    const double nai = more ? ainv[f + 1] : 0.0, nnzf = more ? nz[f + 1] : 0.0;
-    // zx = z . F_f
-    double zx = 0.0;
-    if (beg + lane < end) zx = v0 * Z[c0];
-    for (int e = beg + 64 + lane; e < end; e += 64) zx += (vals ? vals[e] : 1.0) * Z[cols[e]];
-    zx = wave_sum(zx);
-    const double B = lamd * (zx + bold * cs);
-    const double bnew = B * ai + nzf;
-    const double delta = bold - bnew;
-    if (lane == 0) a.beta[(size_t)f * K + d] = bnew;
-    // z += F_f delta (the column's rows are distinct: no two lanes collide)
-    if (beg + lane < end) Z[c0] += v0 * delta;
-    for (int e = beg + 64 + lane; e < end; e += 64) Z[cols[e]] += (vals ? vals[e] : 1.0) * delta;
-    __syncthreads();
-    beg = end;
-    end = nend;
-    c0 = nc0;
-    v0 = nv0;
-    cs = ncs;
-    bold = nbold;
-    ai = nai;
-    nzf = nnzf;
+    if (lane < nloc) a.beta[(size_t)(f0 + lane) * K + d] = bnew_mine;
+    cur = nxt;
+    nxt = load_chunk(a, d, f0 + 128, lane);
   }
 }
 
