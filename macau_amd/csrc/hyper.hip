@@ -740,7 +740,7 @@ int eval_blocks(int ntest) {
   return b;
 }
 
-template <int KP, int NM>
+template <int KP, int NM, bool PROBIT>
 __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
   constexpr int G = KP / 4;  // lanes per test cell
   __shared__ double red[2][kEvalThreads / 64];
@@ -768,8 +768,13 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
 #pragma unroll
     for (int off = 1; off < G; off <<= 1) dot += __shfl_xor(dot, off, G);
     if (sub == 0 && q < a.ntest) {
-      const double p = a.probit ? 0.5 * erfc(-(double)dot * M_SQRT1_2) : a.mean + (double)dot;
-      if (a.probit) a.pred1[q] = p;
+      double p;
+      if constexpr (PROBIT) {
+        p = 0.5 * erfc(-(double)dot * M_SQRT1_2);
+        a.pred1[q] = p;
+      } else {
+        p = a.mean + (double)dot;
+      }
       double pa;
       if (n == 0) {
         pa = p;
@@ -831,11 +836,15 @@ __global__ void __launch_bounds__(256) k_eval_finish(EvalArgs a) {
 
 template <int KP>
 static void launch_eval_k(const EvalArgs& a, hipStream_t s) {
+  if (a.probit) {  // matrix data only (ProbitNoise::evalModel(TensorData) throws)
+    hipLaunchKernelGGL((k_eval<KP, 2, true>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a);
+    return;
+  }
   switch (a.nmodes) {
-    case 2: hipLaunchKernelGGL((k_eval<KP, 2>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_eval<KP, 3>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_eval<KP, 4>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_eval<KP, 5>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_eval<KP, 2, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_eval<KP, 3, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_eval<KP, 4, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_eval<KP, 5, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
   }
 }
 
