@@ -290,7 +290,7 @@ struct BetaUniArgs {
   const double* colsq = nullptr;     // nfeat: sum of squares of each column of F
   const int* Ft_rowptr = nullptr;    // F^T as CSR (nfeat rows), no repeated entries in a row
   const int* Ft_cols = nullptr;
-  const double* Ft_vals = nullptr;   // null = binary
+  const double* Ft_vals = nullptr;   // values (1 for binary side information), never null
   double* beta = nullptr;            // K x nfeat
   double* ainv = nullptr;            // K x nfeat scratch
   double* nz = nullptr;              // K x nfeat scratch

@@ -67,16 +67,17 @@ struct UniChunk {
   double cs, bo, ai, nz;       // colsq, old beta, 1 / A, sqrt(1 / A) n
 };
 
+// unconditional loads from clamped indices (features past the end repeat
+// the last one and are never used), so no load sits under a lane mask
 __device__ __forceinline__ UniChunk load_chunk(const BetaUniArgs& a, int d, int f0, int lane) {
   UniChunk c;
-  const int f = f0 + lane;
-  const bool ok = f < a.nfeat;
-  c.rb = ok ? a.Ft_rowptr[f] : 0;
-  c.re = ok ? a.Ft_rowptr[f + 1] : 0;
-  c.cs = ok ? a.colsq[f] : 0.0;
-  c.bo = ok ? a.beta[(size_t)f * a.K + d] : 0.0;
-  c.ai = ok ? a.ainv[(size_t)d * a.nfeat + f] : 0.0;
-  c.nz = ok ? a.nz[(size_t)d * a.nfeat + f] : 0.0;
+  const int f = min(f0 + lane, a.nfeat - 1);
+  c.rb = a.Ft_rowptr[f];
+  c.re = a.Ft_rowptr[f + 1];
+  c.cs = a.colsq[f];
+  c.bo = a.beta[(size_t)f * a.K + d];
+  c.ai = a.ainv[(size_t)d * a.nfeat + f];
+  c.nz = a.nz[(size_t)d * a.nfeat + f];
   return c;
 }
 
@@ -131,16 +132,20 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
   const int* __restrict__ cols = a.Ft_cols;
   const double* __restrict__ vals = a.Ft_vals;
 
+  if (nfeat <= 0) return;
+  const int last = max(a.Ft_rowptr[nfeat] - 1, 0);  // the entry arrays hold at least one element
   UniChunk cur = load_chunk(a, d, 0, lane);
   UniChunk nxt = load_chunk(a, d, 64, lane);
-  // ring slot q holds the first 64 entries of feature f with f % kUniRing == q
+  // ring slot q holds the raw loads at min(column start + lane, last) of the
+  // feature f with f % kUniRing == q; lanes past the column's end are
+  // masked when the slot is used
   int rc[kUniRing];
   double rv[kUniRing];
 #pragma unroll
   for (int q = 0; q < kUniRing; q++) {
-    const int b = rdl(cur.rb, q), e = rdl(cur.re, q);
-    rc[q] = b + lane < e ? cols[b + lane] : 0;
-    rv[q] = b + lane < e ? (vals ? vals[b + lane] : 1.0) : 0.0;
+    const int ix = min(rdl(cur.rb, q) + lane, last);
+    rc[q] = cols[ix];
+    rv[q] = vals[ix];
   }
   for (int f0 = 0; f0 < nfeat; f0 += 64) {
     double bnew_mine = cur.bo;
@@ -149,28 +154,26 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
 #pragma unroll
       for (int q = 0; q < kUniRing; q++) {
         const int j = j0 + q;
+        const int c0 = rc[q];
+        const double v0r = rv[q];
+        // refill the slot with feature f0 + j + kUniRing (this chunk or the
+        // next), unconditionally so the load writes the slot's register
+        {
+          const int jn = j + kUniRing;
+          const int bc = rdl(cur.rb, jn & 63), bn = rdl(nxt.rb, jn & 63);
+          const int ix = min((jn < 64 ? bc : bn) + lane, last);
+          rc[q] = cols[ix];
+          rv[q] = vals[ix];
+        }
         if (j < nloc) {
           const int beg = rdl(cur.rb, j), end = rdl(cur.re, j);
-          const int c0 = rc[q];
-          const double v0 = rv[q];
-          // refill the slot with feature f0 + j + kUniRing (this chunk or the next)
-          {
-            const int jn = j + kUniRing;
-            int b, e;
-            if (jn < 64) {
-              b = rdl(cur.rb, jn);
-              e = rdl(cur.re, jn);
-            } else {
-              b = rdl(nxt.rb, jn - 64);
-              e = rdl(nxt.re, jn - 64);
-            }
-            rc[q] = b + lane < e ? cols[b + lane] : 0;
-            rv[q] = b + lane < e ? (vals ? vals[b + lane] : 1.0) : 0.0;
-          }
           const bool live = beg + lane < end;
+          const double v0 = live ? v0r : 0.0;
           const double zc = live ? (double)Z[c0] : 0.0;
           double zx = v0 * zc;
-          for (int e = beg + 64 + lane; e < end; e += 64) zx += (vals ? vals[e] : 1.0) * Z[cols[e]];
+          const bool longcol = end - beg > 64;  // uniform
+          if (longcol)
+            for (int e = beg + 64 + lane; e < end; e += 64) zx += vals[e] * Z[cols[e]];
           zx = wave_sum(zx);
           const double bold = rdl(cur.bo, j), cs = rdl(cur.cs, j);
           const double B = lamd * (zx + bold * cs);
@@ -179,7 +182,8 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
           if (lane == j) bnew_mine = bnew;
           // z += F_f delta (the column's rows are distinct: no two lanes collide)
           if (live) Z[c0] = zc + v0 * delta;
-          for (int e = beg + 64 + lane; e < end; e += 64) Z[cols[e]] += (vals ? vals[e] : 1.0) * delta;
+          if (longcol)
+            for (int e = beg + 64 + lane; e < end; e += 64) Z[cols[e]] += vals[e] * delta;
           if (LDS) {
             __builtin_amdgcn_wave_barrier();
           } else {
