@@ -160,3 +160,41 @@ def test_model_univariate_many_rows():
     assert row_rel_err(m.latents(1, ncol), ref.U[1]) < TOL_FP32
     lb = _lib.lib().macau_get_link_lambda(m.h, 0)
     assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < TOL_FP32
+
+
+def test_model_univariate_long_columns_duplicates():
+    """Feature columns with more than 64 rows (the sweep's tail loops, LDS
+    form), repeated (row, feature) entries (merged for the sweep; the oracle
+    keeps them, as the reference does: At_mul_Bt and add_Acol_mul_bt visit
+    both, col_square_sum adds both squares) and empty feature columns."""
+    announce("model univariate long columns, repeated entries, empty columns")
+    rng = np.random.default_rng(47)
+    nrow, ncol, nfeat, K = 2000, 60, 12, 8
+    fr = rng.integers(0, nrow, 2400)
+    fc = rng.integers(0, nfeat - 3, 2400)   # the last three features stay empty
+    fr = np.concatenate([fr, fr[:100]])     # 100 repeated entries
+    fc = np.concatenate([fc, fc[:100]])
+    fv = rng.standard_normal(len(fr))
+    r = rng.integers(0, nrow, 30000)
+    c = rng.integers(0, ncol, 30000)
+    key = np.unique(r.astype(np.int64) * ncol + c)
+    r, c = (key // ncol).astype(np.int32), (key % ncol).astype(np.int32)
+    Fd = np.zeros((nrow, nfeat))
+    np.add.at(Fd, (fr, fc), fv)
+    y = 0.5 * (Fd @ rng.standard_normal(nfeat))[r] + rng.standard_normal(len(r))
+    m = GpuModel(2, K, 3.0)
+    m.data(np.stack([r, c], 1), y, [nrow, ncol])
+    m.one_prior((fr, fc, fv, nrow, nfeat))
+    m.prior()
+    m.setup(1, 2, SEED, PLAIN, graph=False)
+    m.iterate(2)
+    modes, mean, nnz = O.matrix_rows(r, c, y, nrow, ncol)
+    side = [O.csr_pair(fr, fc, fv, nrow, nfeat), None]
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, np.zeros((0, 2), np.int32), np.zeros(0), precision=3.0,
+                side=side, burnin=1, nsamples=2, seed=SEED, max_iters=2, want_U=True, uni=[True, False])
+    e0, e1 = row_rel_err(m.latents(0, nrow), ref.U[0]), row_rel_err(m.latents(1, ncol), ref.U[1])
+    lb = _lib.lib().macau_get_link_lambda(m.h, 0)
+    print("univariate long/repeated/empty: row err %.3e %.3e, lambda_beta %.6f vs %.6f" %
+          (e0, e1, lb, ref.lambda_beta[0]))
+    assert e0 < TOL_FP32 and e1 < TOL_FP32
+    assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < TOL_FP32
