@@ -1,6 +1,6 @@
 """Parity of the HIP path with the FP64 oracle on identical inputs and draws.
 
-Tolerances (BASELINE north star): every sampled latent row, link matrix and
+Tolerances: every sampled latent row, link matrix and
 hyperparameter of the FP32 path within 1e-4 relative of the FP64 oracle
 (row-wise: ||u_gpu - u_ref|| / ||u_ref||); FP64 kernels within 1e-9;
 integer RNG words bit-exact; posterior-mean test RMSE within 0.5%.
