@@ -295,6 +295,12 @@ struct BetaUniArgs {
   double* ainv = nullptr;            // K x nfeat scratch
   double* nz = nullptr;              // K x nfeat scratch
   double* Zw = nullptr;              // K x N scratch (used when N exceeds the LDS rows)
+  // the features with non-empty columns in order, in batches of 4 records
+  // {feature (-1 = unused slot), column begin, column end, wide}: up to four
+  // consecutive columns of at most 16 entries with pairwise disjoint rows
+  // (wide = 0), or one longer column repeated in all four records (wide = 1)
+  const int4* batch_meta = nullptr;
+  int nbatch = 0;
   uint64_t seed = 0;
   const int* iter = nullptr;
   int mode = 0;

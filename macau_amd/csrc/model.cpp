@@ -1037,6 +1037,40 @@ MacauOnePrior::MacauOnePrior(int nr, int nc, const std::vector<int>& rows, const
     }
     Fu_rowptr[f + 1] = (int)Fu_cols.size();
   }
+  // the sweep's batches (greedy in feature order; any order-preserving
+  // grouping of commuting updates gives the sequential sweep's result):
+  // features with empty columns are left to the parallel pre-pass, columns
+  // longer than 16 entries run alone on the whole wave, and up to four
+  // consecutive shorter columns whose rows are pairwise disjoint share a step
+  std::vector<int> stamp(nr, -1), cur;
+  int nbat = 0;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    for (int q = 0; q < 4; q++) {
+      if (q < (int)cur.size())
+        batch_meta.push_back(make_int4(cur[q], Fu_rowptr[cur[q]], Fu_rowptr[cur[q] + 1], 0));
+      else
+        batch_meta.push_back(make_int4(-1, 0, 0, 0));
+    }
+    cur.clear();
+    nbat++;
+  };
+  for (int f = 0; f < nc; f++) {
+    const int len = Fu_rowptr[f + 1] - Fu_rowptr[f];
+    if (len == 0) continue;
+    if (len > 16) {
+      flush();
+      for (int q = 0; q < 4; q++) batch_meta.push_back(make_int4(f, Fu_rowptr[f], Fu_rowptr[f + 1], 1));
+      nbat++;
+      continue;
+    }
+    bool conflict = cur.size() == 4;
+    for (int e = Fu_rowptr[f]; e < Fu_rowptr[f + 1] && !conflict; e++) conflict = stamp[Fu_cols[e]] == nbat;
+    if (conflict) flush();
+    cur.push_back(f);
+    for (int e = Fu_rowptr[f]; e < Fu_rowptr[f + 1]; e++) stamp[Fu_cols[e]] = nbat;
+  }
+  flush();
 }
 
 void MacauOnePrior::init(Model& m, int mode) {  // macauoneprior.cpp:16-38
@@ -1059,6 +1093,7 @@ void MacauOnePrior::init(Model& m, int mode) {  // macauoneprior.cpp:16-38
   upload(dFu_rowptr, Fu_rowptr, m.stream);
   upload(dFu_cols, Fu_cols.empty() ? std::vector<int>{0} : Fu_cols, m.stream);
   upload(dFu_vals, Fu_vals.empty() ? std::vector<double>{0.0} : Fu_vals, m.stream);
+  upload(dbatch, batch_meta.empty() ? std::vector<int4>{make_int4(-1, 0, 0, 0)} : batch_meta, m.stream);
   ainv.alloc(std::max<size_t>(kf, 1) * sizeof(double));
   nz.alloc(std::max<size_t>(kf, 1) * sizeof(double));
   Zw.alloc(beta_uni_in_lds(N) ? sizeof(double) : (size_t)K * N * sizeof(double));
@@ -1098,6 +1133,8 @@ void MacauOnePrior::update_prior(Model& m, int mode) {  // macauoneprior.cpp:189
   b.ainv = ainv.as<double>();
   b.nz = nz.as<double>();
   b.Zw = Zw.as<double>();
+  b.batch_meta = dbatch.as<int4>();
+  b.nbatch = (int)(batch_meta.size() / 4);
   b.seed = m.seed;
   b.iter = m.d_iter.as<int>();
   b.mode = mode;

@@ -141,7 +141,8 @@ class MacauOnePrior : public MacauPrior {
   std::vector<double> colsq;           // col_square_sum(F) (linop.cpp:374-401)
   std::vector<int> Fu_rowptr, Fu_cols;  // F^T with repeated entries of a row merged
   std::vector<double> Fu_vals;
-  DevBuf dFu_rowptr, dFu_cols, dFu_vals, dcolsq, lambda_beta_vec, uhat64, ainv, nz, Zw;
+  std::vector<int4> batch_meta;  // the sweep's batches of features with disjoint columns
+  DevBuf dFu_rowptr, dFu_cols, dFu_vals, dcolsq, lambda_beta_vec, uhat64, ainv, nz, Zw, dbatch;
 };
 
 // noisemodels.h:13-29

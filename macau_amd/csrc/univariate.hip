@@ -11,14 +11,19 @@
 //   compute_uhat (the shared SpMM);
 //   sample_lambda_beta: lambda_beta_d ~ Gamma(0.1 + nfeat / 2, 1 / (0.1 + |beta_d|^2 / 2)).
 //
-// The feature chain is sequential by construction, so it runs one wave per
-// latent with z in LDS (rows up to kUniLdsRows; a global buffer beyond).
-// Everything a step needs that does not depend on the chain is off its
-// critical path: 1 / A and the scaled normal sqrt(1/A) n_df come from a
-// parallel pre-pass; the per-feature values of 64 features are loaded one
-// per lane, a chunk ahead, and read with readlane; the first 64 entries of
-// each column are loaded kUniRing features ahead.  A step is then the LDS
-// gather, one wave sum, three FP64 operations and the LDS scatter.
+// The feature chain of a latent is sequential, so it runs on one wave with
+// z in LDS (rows up to kUniLdsRows; a global buffer beyond).  Features whose
+// columns share no row commute (updating one leaves every z entry the other
+// reads unchanged), which the sweep uses twice without changing its result:
+// features with an empty column are updated by the parallel pre-pass, and
+// the host groups the others, in order, into batches: up to four
+// consecutive short columns (at most 16 entries) with pairwise disjoint
+// rows, 16 lanes each, or one long column on all 64 lanes.  Everything a
+// step needs that does not depend on the chain is off its critical path:
+// 1 / A and the scaled normal sqrt(1/A) n_df come from the pre-pass, and
+// each lane loads its batch slot's values and first column entry kUniDepth
+// batches ahead (the batch records 2 kUniDepth ahead).  A step is the LDS
+// gather, a 16- or 64-lane DPP sum, three FP64 operations and the scatter.
 #include <math.h>
 
 #include <type_traits>
@@ -52,48 +57,40 @@ __global__ void __launch_bounds__(256) k_beta_uni_pre(BetaUniArgs a) {
     const int d = (int)(q / a.nfeat), f = (int)(q % a.nfeat);
     const double A = a.lambda_beta[d] + a.Lambda[d + d * a.K] * a.colsq[f];
     const double ainv = 1.0 / A;
+    const double nzv = sqrt(ainv) * normal_f64(a.seed, (uint32_t)f, (uint32_t)d, (uint32_t)it, tag);
     a.ainv[q] = ainv;
-    a.nz[q] = sqrt(ainv) * normal_f64(a.seed, (uint32_t)f, (uint32_t)d, (uint32_t)it, tag);
+    a.nz[q] = nzv;
+    // an empty column couples to nothing: its update is B = 0, beta = nz,
+    // done here; the sequential sweep walks the other features only
+    if (a.Ft_rowptr[f + 1] == a.Ft_rowptr[f]) a.beta[(size_t)f * a.K + d] = nzv;
   }
 }
 
 
 constexpr int kUniLdsRows = 19456;  // 152 KB of FP64 residuals
 
-// Per-feature values of 64 consecutive features, lane j holding feature
-// f0 + j (zero past the end).
-struct UniChunk {
-  int rb, re;                  // column extent in F^T
-  double cs, bo, ai, nz;       // colsq, old beta, 1 / A, sqrt(1 / A) n
-};
-
-// unconditional loads from clamped indices (features past the end repeat
-// the last one and are never used), so no load sits under a lane mask
-__device__ __forceinline__ UniChunk load_chunk(const BetaUniArgs& a, int d, int f0, int lane) {
-  UniChunk c;
-  const int f = min(f0 + lane, a.nfeat - 1);
-  c.rb = a.Ft_rowptr[f];
-  c.re = a.Ft_rowptr[f + 1];
-  c.cs = a.colsq[f];
-  c.bo = a.beta[(size_t)f * a.K + d];
-  c.ai = a.ainv[(size_t)d * a.nfeat + f];
-  c.nz = a.nz[(size_t)d * a.nfeat + f];
-  return c;
-}
-
-__device__ __forceinline__ int rdl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
-__device__ __forceinline__ double rdl(double x, int j) {
+// x moved across lanes by one DPP pattern (both 32-bit halves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
   const int2 v = *reinterpret_cast<int2*>(&x);
   int2 r;
-  r.x = __builtin_amdgcn_readlane(v.x, j);
-  r.y = __builtin_amdgcn_readlane(v.y, j);
+  r.x = __builtin_amdgcn_update_dpp(0, v.x, CTRL, 0xF, 0xF, false);
+  r.y = __builtin_amdgcn_update_dpp(0, v.y, CTRL, 0xF, 0xF, false);
   return *reinterpret_cast<double*>(&r);
 }
 
-// x moved across lanes by one DPP pattern (both 32-bit halves); lanes
-// outside ROWMASK receive 0
+// sum over each 16-lane row in a fixed order (pairs, quads, half-row and
+// row mirrors), returned in every lane of the row
+__device__ __forceinline__ double row16_sum(double x) {
+  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return x;
+}
+
 template <int CTRL, int ROWMASK>
-__device__ __forceinline__ double dpp_f64(double x) {
+__device__ __forceinline__ double dpp_f64_rows(double x) {
   const int2 v = *reinterpret_cast<int2*>(&x);
   int2 r;
   r.x = __builtin_amdgcn_update_dpp(0, v.x, CTRL, ROWMASK, 0xF, false);
@@ -101,100 +98,138 @@ __device__ __forceinline__ double dpp_f64(double x) {
   return *reinterpret_cast<double*>(&r);
 }
 
-// sum over the wave in a fixed order, returned in every lane: pairs and
-// quads (quad_perm), 8 and 16 lanes (half-row and row mirror), then rows
-// combined by the row broadcasts into lane 63, which readlane distributes
-__device__ __forceinline__ double wave_sum(double x) {
-  x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
-  x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
-  x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
-  x += dpp_f64<0x140, 0xF>(x);  // row_mirror
-  x += dpp_f64<0x142, 0xA>(x);  // row_bcast15 into rows 1 and 3
-  x += dpp_f64<0x143, 0xC>(x);  // row_bcast31 into rows 2 and 3
-  return rdl(x, 63);
+// sum over the wave: the row sums, then rows combined by the row broadcasts
+// into lane 63, which readlane distributes
+__device__ __forceinline__ double wave_sum64(double x) {
+  x = row16_sum(x);
+  x += dpp_f64_rows<0x142, 0xA>(x);  // row_bcast15 into rows 1 and 3
+  x += dpp_f64_rows<0x143, 0xC>(x);  // row_bcast31 into rows 2 and 3
+  const int2 v = *reinterpret_cast<int2*>(&x);
+  int2 r;
+  r.x = __builtin_amdgcn_readlane(v.x, 63);
+  r.y = __builtin_amdgcn_readlane(v.y, 63);
+  return *reinterpret_cast<double*>(&r);
 }
 
-constexpr int kUniRing = 8;  // columns whose first 64 entries are in flight
+constexpr int kUniDepth = 4;  // batches whose loads are in flight
+
+// Loads of one batch slot for this lane: one of its feature's first 16 column
+// entries (64 for a long column; index clamped, masked at use), the
+// feature's colsq, old beta, 1 / A and scaled normal.
+struct UniSlot {
+  int f, beg, end, lng;  // lng: the batch is one long column on all lanes
+  int c;
+  double v, cs, bo, ai, nz;
+};
+
+__device__ __forceinline__ UniSlot load_slot(const BetaUniArgs& a, int4 m, int d, int sub, int lane, int last) {
+  UniSlot s;
+  s.f = m.x;
+  s.beg = m.y;
+  s.end = m.z;
+  s.lng = m.w;
+  const int fc = max(m.x, 0);
+  const int ix = min(m.y + (m.w ? lane : sub), last);
+  s.c = a.Ft_cols[ix];
+  s.v = a.Ft_vals[ix];
+  s.cs = a.colsq[fc];
+  s.bo = a.beta[(size_t)fc * a.K + d];
+  s.ai = a.ainv[(size_t)d * a.nfeat + fc];
+  s.nz = a.nz[(size_t)d * a.nfeat + fc];
+  return s;
+}
 
 // z in LDS (one wave: its LDS operations complete in issue order, so a step
 // sees the previous step's scatter without a barrier), or (LDS = false) in
-// global memory accessed as volatile with a barrier after each scatter
+// global memory accessed as volatile with a barrier after each scatter.
+// Batches hold up to four consecutive features whose columns share no row
+// (built on the host), so updating them in one step gives the sequential
+// sweep's result: lanes 16 s .. 16 s + 15 handle slot s.
 template <bool LDS>
 __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
   using ZPtr = typename std::conditional<LDS, double*, volatile double*>::type;
   __shared__ double zs[LDS ? kUniLdsRows : 1];
-  const int d = blockIdx.x, lane = threadIdx.x;
-  const int K = a.K, N = a.N, nfeat = a.nfeat;
+  const int d = blockIdx.x, lane = threadIdx.x, sub = lane & 15, slot = lane >> 4;
+  const int K = a.K, N = a.N, nb = a.nbatch;
   ZPtr Z = LDS ? zs : a.Zw + (size_t)d * N;
   const double mud = a.mu[d], lamd = a.Lambda[d + d * K];
   for (int i = lane; i < N; i += 64) Z[i] = (double)a.U[(size_t)i * a.KP + d] - mud - a.uhat[(size_t)i * K + d];
   __syncthreads();
-  const int* __restrict__ cols = a.Ft_cols;
-  const double* __restrict__ vals = a.Ft_vals;
-
-  if (nfeat <= 0) return;
-  const int last = max(a.Ft_rowptr[nfeat] - 1, 0);  // the entry arrays hold at least one element
-  UniChunk cur = load_chunk(a, d, 0, lane);
-  UniChunk nxt = load_chunk(a, d, 64, lane);
-  // ring slot q holds the raw loads at min(column start + lane, last) of the
-  // feature f with f % kUniRing == q; lanes past the column's end are
-  // masked when the slot is used
-  int rc[kUniRing];
-  double rv[kUniRing];
+  if (nb <= 0) return;
+  const int last = max(a.Ft_rowptr[a.nfeat] - 1, 0);  // the entry arrays hold at least one element
+  const int4* __restrict__ meta = a.batch_meta;
+  auto meta_of = [&](int b) { return meta[(size_t)min(b, nb - 1) * 4 + slot]; };
+  // data ring: batch b in slot b % D; meta ring: the batch D ahead of it
+  UniSlot dr[kUniDepth];
+  int4 mr[kUniDepth];
 #pragma unroll
-  for (int q = 0; q < kUniRing; q++) {
-    const int ix = min(rdl(cur.rb, q) + lane, last);
-    rc[q] = cols[ix];
-    rv[q] = vals[ix];
+  for (int q = 0; q < kUniDepth; q++) {
+    dr[q] = load_slot(a, meta_of(q), d, sub, lane, last);
+    mr[q] = meta_of(q + kUniDepth);
   }
-  for (int f0 = 0; f0 < nfeat; f0 += 64) {
-    double bnew_mine = cur.bo;
-    const int nloc = nfeat - f0 < 64 ? nfeat - f0 : 64;
-    for (int j0 = 0; j0 < nloc; j0 += kUniRing) {
+  for (int b0 = 0; b0 < nb; b0 += kUniDepth) {
 #pragma unroll
-      for (int q = 0; q < kUniRing; q++) {
-        const int j = j0 + q;
-        const int c0 = rc[q];
-        const double v0r = rv[q];
-        // refill the slot with feature f0 + j + kUniRing (this chunk or the
-        // next), unconditionally so the load writes the slot's register
-        {
-          const int jn = j + kUniRing;
-          const int bc = rdl(cur.rb, jn & 63), bn = rdl(nxt.rb, jn & 63);
-          const int ix = min((jn < 64 ? bc : bn) + lane, last);
-          rc[q] = cols[ix];
-          rv[q] = vals[ix];
+    for (int q = 0; q < kUniDepth; q++) {
+      const int b = b0 + q;
+      const UniSlot& s = dr[q];
+      if (b < nb) {
+        const bool wide = __builtin_amdgcn_readfirstlane(s.lng) != 0;  // uniform: one long column
+        const int off = wide ? lane : sub;
+        const bool live = s.beg + off < s.end;
+        const double v0 = live ? s.v : 0.0;
+        const double zc = live ? (double)Z[s.c] : 0.0;
+        double zx = v0 * zc;
+        if (wide) {
+          // the rest of the column, four 64-entry chunks per trip from clamped
+          // indices so their loads overlap
+          for (int e0 = s.beg + 64; e0 < s.end; e0 += 256) {
+            int c4[4];
+            double v4[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+              const int ix = min(e0 + 64 * t + lane, last);
+              c4[t] = a.Ft_cols[ix];
+              v4[t] = e0 + 64 * t + lane < s.end ? a.Ft_vals[ix] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) zx += v4[t] * Z[c4[t]];
+          }
+          zx = wave_sum64(zx);
+        } else {
+          zx = row16_sum(zx);
         }
-        if (j < nloc) {
-          const int beg = rdl(cur.rb, j), end = rdl(cur.re, j);
-          const bool live = beg + lane < end;
-          const double v0 = live ? v0r : 0.0;
-          const double zc = live ? (double)Z[c0] : 0.0;
-          double zx = v0 * zc;
-          const bool longcol = end - beg > 64;  // uniform
-          if (longcol)
-            for (int e = beg + 64 + lane; e < end; e += 64) zx += vals[e] * Z[cols[e]];
-          zx = wave_sum(zx);
-          const double bold = rdl(cur.bo, j), cs = rdl(cur.cs, j);
-          const double B = lamd * (zx + bold * cs);
-          const double bnew = B * rdl(cur.ai, j) + rdl(cur.nz, j);
-          const double delta = bold - bnew;
-          if (lane == j) bnew_mine = bnew;
-          // z += F_f delta (the column's rows are distinct: no two lanes collide)
-          if (live) Z[c0] = zc + v0 * delta;
-          if (longcol)
-            for (int e = beg + 64 + lane; e < end; e += 64) Z[cols[e]] += vals[e] * delta;
-          if (LDS) {
-            __builtin_amdgcn_wave_barrier();
-          } else {
-            __syncthreads();
+        const double B = lamd * (zx + s.bo * s.cs);
+        const double bnew = B * s.ai + s.nz;
+        const double delta = s.bo - bnew;
+        if (off == 0 && s.f >= 0) a.beta[(size_t)s.f * K + d] = bnew;
+        // z += F_f delta (distinct rows within and across the batch's columns)
+        if (live) Z[s.c] = zc + v0 * delta;
+        if (wide) {
+          for (int e0 = s.beg + 64; e0 < s.end; e0 += 256) {
+            int c4[4];
+            double v4[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+              const int ix = min(e0 + 64 * t + lane, last);
+              c4[t] = a.Ft_cols[ix];
+              v4[t] = a.Ft_vals[ix];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+              if (e0 + 64 * t + lane < s.end) Z[c4[t]] += v4[t] * delta;
           }
         }
+        if (LDS) {
+          __builtin_amdgcn_wave_barrier();
+        } else {
+          __syncthreads();
+        }
       }
+      // refills once the slot's values are dead, unconditionally so the
+      // loads write the ring registers in place
+      dr[q] = load_slot(a, mr[q], d, sub, lane, last);
+      mr[q] = meta_of(b + 2 * kUniDepth);
     }
-    if (lane < nloc) a.beta[(size_t)(f0 + lane) * K + d] = bnew_mine;
-    cur = nxt;
-    nxt = load_chunk(a, d, f0 + 128, lane);
   }
 }
 
