@@ -111,7 +111,8 @@ __device__ __forceinline__ double wave_sum64(double x) {
   return *reinterpret_cast<double*>(&r);
 }
 
-constexpr int kUniDepth = 4;  // batches whose loads are in flight
+constexpr int kUniDepth = 4;     // batches whose loads are in flight
+constexpr int kWideChunks = 16;  // 64-entry chunks of a long column loaded together
 
 // Loads of one batch slot for this lane: one of its feature's first 16 column
 // entries (64 for a long column; index clamped, masked at use), the
@@ -180,19 +181,19 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
         const double zc = live ? (double)Z[s.c] : 0.0;
         double zx = v0 * zc;
         if (wide) {
-          // the rest of the column, four 64-entry chunks per trip from clamped
-          // indices so their loads overlap
-          for (int e0 = s.beg + 64; e0 < s.end; e0 += 256) {
-            int c4[4];
-            double v4[4];
+          // the rest of the column, kWideChunks 64-entry chunks per trip
+          // from clamped indices so their loads overlap
+          for (int e0 = s.beg + 64; e0 < s.end; e0 += 64 * kWideChunks) {
+            int c4[kWideChunks];
+            double v4[kWideChunks];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < kWideChunks; t++) {
               const int ix = min(e0 + 64 * t + lane, last);
               c4[t] = a.Ft_cols[ix];
               v4[t] = e0 + 64 * t + lane < s.end ? a.Ft_vals[ix] : 0.0;
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) zx += v4[t] * Z[c4[t]];
+            for (int t = 0; t < kWideChunks; t++) zx += v4[t] * Z[c4[t]];
           }
           zx = wave_sum64(zx);
         } else {
@@ -205,17 +206,17 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
         // z += F_f delta (distinct rows within and across the batch's columns)
         if (live) Z[s.c] = zc + v0 * delta;
         if (wide) {
-          for (int e0 = s.beg + 64; e0 < s.end; e0 += 256) {
-            int c4[4];
-            double v4[4];
+          for (int e0 = s.beg + 64; e0 < s.end; e0 += 64 * kWideChunks) {
+            int c4[kWideChunks];
+            double v4[kWideChunks];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < kWideChunks; t++) {
               const int ix = min(e0 + 64 * t + lane, last);
               c4[t] = a.Ft_cols[ix];
               v4[t] = a.Ft_vals[ix];
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int t = 0; t < kWideChunks; t++)
               if (e0 + 64 * t + lane < s.end) Z[c4[t]] += v4[t] * delta;
           }
         }
