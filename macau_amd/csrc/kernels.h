@@ -309,7 +309,9 @@ struct BetaUniArgs {
 void launch_uni_diag(int K, int KP, double* Lambda, float* Lambda_f, float* Lmu_f, const double* mu, hipStream_t s);
 // sample_beta: the per-latent feature sweeps
 void launch_beta_uni(const BetaUniArgs& a, hipStream_t s);
-void launch_lambda_beta_uni(const BetaUniArgs& a, hipStream_t s);
+// partial: lambda_beta_uni_partials(K) doubles of scratch
+int lambda_beta_uni_partials(int K);
+void launch_lambda_beta_uni(const BetaUniArgs& a, double* partial, hipStream_t s);
 bool beta_uni_in_lds(int N);
 
 }  // namespace mgpu

@@ -1097,6 +1097,7 @@ void MacauOnePrior::init(Model& m, int mode) {  // macauoneprior.cpp:16-38
   ainv.alloc(std::max<size_t>(kf, 1) * sizeof(double));
   nz.alloc(std::max<size_t>(kf, 1) * sizeof(double));
   Zw.alloc(beta_uni_in_lds(N) ? sizeof(double) : (size_t)K * N * sizeof(double));
+  lb_partial.alloc((size_t)lambda_beta_uni_partials(K) * sizeof(double));
   partial_sumsq.alloc(256 * sizeof(double));
   HIPCHK(hipStreamSynchronize(m.stream));
 }
@@ -1144,7 +1145,7 @@ void MacauOnePrior::update_prior(Model& m, int mode) {  // macauoneprior.cpp:189
   launch_uhat_finish(F(), beta.as<double>(), K, KP, mu.as<double>(), Lambda.as<double>(), Uhat.as<float>(),
                      Lmu_rows.as<float>(), uhat64.as<double>(), s);
   // sample_lambda_beta (:263-289)
-  launch_lambda_beta_uni(b, s);
+  launch_lambda_beta_uni(b, lb_partial.as<double>(), s);
 }
 
 double MacauOnePrior::getLinkLambda(Model& m) {

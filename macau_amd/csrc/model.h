@@ -142,7 +142,7 @@ class MacauOnePrior : public MacauPrior {
   std::vector<int> Fu_rowptr, Fu_cols;  // F^T with repeated entries of a row merged
   std::vector<double> Fu_vals;
   std::vector<int4> batch_meta;  // the sweep's batches of features with disjoint columns
-  DevBuf dFu_rowptr, dFu_cols, dFu_vals, dcolsq, lambda_beta_vec, uhat64, ainv, nz, Zw, dbatch;
+  DevBuf dFu_rowptr, dFu_cols, dFu_vals, dcolsq, lambda_beta_vec, uhat64, ainv, nz, Zw, dbatch, lb_partial;
 };
 
 // noisemodels.h:13-29

@@ -26,6 +26,7 @@
 // gather, a 16- or 64-lane DPP sum, three FP64 operations and the scatter.
 #include <math.h>
 
+#include <stdexcept>
 #include <type_traits>
 
 #include "common.h"
@@ -234,24 +235,40 @@ __global__ void __launch_bounds__(64) k_beta_uni_sweep(BetaUniArgs a) {
   }
 }
 
-// lambda_beta_d ~ Gamma(0.1 + nfeat / 2, scale 1 / (0.1 + sum_f beta_df^2 / 2)),
-// the gamma keyed (d, it, tag(LAMBDA_BETA, mode)); one work-group per latent
-__global__ void __launch_bounds__(256) k_lambda_beta_uni(BetaUniArgs a) {
+// sum_f beta_df^2 per latent d in one coalesced pass: work-group b takes a
+// contiguous range of features; thread t < K (256 / K) holds latent
+// t % K and walks the features t / K, t / K + 256 / K, ... of the range
+// (consecutive threads read consecutive entries of beta); the threads of a
+// latent are combined in thread order into partial[b][d], and the blocks in
+// block order by launch_partial_reduce
+constexpr int kLbBlocks = 64;
+__global__ void __launch_bounds__(256) k_beta_sumsq(BetaUniArgs a, double* partial) {
   __shared__ double red[256];
-  const int d = blockIdx.x;
+  const int K = a.K, per = 256 / K, t = threadIdx.x;
+  const int chunk = (a.nfeat + gridDim.x - 1) / gridDim.x;
+  const int f0 = blockIdx.x * chunk, f1 = min(a.nfeat, f0 + chunk);
   double s = 0.0;
-  for (int f = threadIdx.x; f < a.nfeat; f += 256) {
-    const double b = a.beta[(size_t)f * a.K + d];
-    s += b * b;
+  if (t < K * per) {
+    const int d = t % K;
+    for (int f = f0 + t / K; f < f1; f += per) {
+      const double b = a.beta[(size_t)f * K + d];
+      s += b * b;
+    }
   }
-  red[threadIdx.x] = s;
+  red[t] = s;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
+  if (t < K) {
+    double v = 0.0;
+    for (int m = 0; m < per; m++) v += red[t + m * K];
+    partial[(size_t)blockIdx.x * K + t] = v;
   }
-  if (threadIdx.x == 0) {
-    const double shape = 0.1 + (double)a.nfeat / 2.0, rate = 0.1 + red[0] / 2.0;
+}
+
+// lambda_beta_d ~ Gamma(0.1 + nfeat / 2, scale 1 / (0.1 + sum_f beta_df^2 / 2)),
+// the gamma keyed (d, it, tag(LAMBDA_BETA, mode))
+__global__ void __launch_bounds__(256) k_lambda_beta_uni(BetaUniArgs a, const double* sumsq) {
+  for (int d = threadIdx.x; d < a.K; d += 256) {
+    const double shape = 0.1 + (double)a.nfeat / 2.0, rate = 0.1 + sumsq[d] / 2.0;
     const uint32_t tag = make_tag(TAG_LAMBDA_BETA, (uint32_t)a.mode);
     a.lambda_beta[d] = gamma_f64(a.seed, shape, (uint32_t)d, (uint32_t)*a.iter, tag) / rate;
   }
@@ -277,8 +294,14 @@ void launch_beta_uni(const BetaUniArgs& a, hipStream_t s) {
   }
 }
 
-void launch_lambda_beta_uni(const BetaUniArgs& a, hipStream_t s) {
-  if (a.K > 0) hipLaunchKernelGGL(k_lambda_beta_uni, dim3(a.K), dim3(256), 0, s, a);
+int lambda_beta_uni_partials(int K) { return kLbBlocks * K; }
+
+void launch_lambda_beta_uni(const BetaUniArgs& a, double* partial, hipStream_t s) {
+  if (a.K <= 0) return;
+  if (a.K > 256) throw std::runtime_error("lambda_beta: more than 256 latent dimensions");
+  hipLaunchKernelGGL(k_beta_sumsq, dim3(kLbBlocks), dim3(256), 0, s, a, partial);
+  launch_partial_reduce(partial, kLbBlocks, a.K, s);
+  hipLaunchKernelGGL(k_lambda_beta_uni, dim3(1), dim3(256), 0, s, a, (const double*)partial);
 }
 
 }  // namespace mgpu
