@@ -62,11 +62,19 @@ __global__ void __launch_bounds__(64) k_potrf_diag(double* W, int n, int k0, int
   const int nb = min(kCB, n - k0);
   const int lane = threadIdx.x;
   const int i = lane < nb ? lane : nb - 1;  // lanes >= nb shadow the last row
-  for (int j = 0; j < nb; j++) A[i][j] = j <= i ? W[(size_t)(k0 + i) + (size_t)(k0 + j) * n] : 0.0;
+  {
+    // all 32 loads issued before the first LDS store
+    double v[kCB];
+#pragma unroll
+    for (int j = 0; j < kCB; j++) v[j] = (j < nb && j <= i) ? W[(size_t)(k0 + i) + (size_t)(k0 + j) * n] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kCB; j++) A[i][j] = v[j];
+  }
   lds_wait();
   bool bad = false;
   for (int k = 0; k < nb; k++) {
     double s = A[i][k];
+#pragma unroll 8
     for (int j = 0; j < k; j++) s -= A[i][j] * A[k][j];
     const double d = __shfl(s, k, 64);
     bad = bad || !(d > 0.0);
@@ -80,14 +88,30 @@ __global__ void __launch_bounds__(64) k_potrf_diag(double* W, int n, int k0, int
   if (lane == 0 && bad) atomicMax(fail, kFailDenseChol);
 }
 
+// loads the 32 x 32 lower block at (k0, k0) into L with all of a thread's
+// loads issued before its LDS stores (T threads, kCB * kCB / T loads each)
+template <int T>
+__device__ __forceinline__ void load_diag_block(const double* W, int n, int k0, int nb, double (*L)[kCLd]) {
+  constexpr int PER = kCB * kCB / T;
+  double v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int q = threadIdx.x + T * u;
+    const int r = q / kCB, c = q % kCB;
+    v[u] = (r < nb && c <= r) ? W[(size_t)(k0 + r) + (size_t)(k0 + c) * n] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int q = threadIdx.x + T * u;
+    L[q / kCB][q % kCB] = v[u];
+  }
+}
+
 // rows i >= k0 + 32 of the panel: w_i <- w_i L_kk^{-T} (forward substitution
 // over the 32 columns, one thread per row)
 __global__ void __launch_bounds__(256) k_trsm_panel(double* W, int n, int k0) {
   __shared__ double L[kCB][kCLd];
-  for (int q = threadIdx.x; q < kCB * kCB; q += blockDim.x) {
-    const int r = q / kCB, c = q % kCB;
-    L[r][c] = c <= r ? W[(size_t)(k0 + r) + (size_t)(k0 + c) * n] : 0.0;
-  }
+  load_diag_block<256>(W, n, k0, kCB, L);
   __syncthreads();
   const int i = k0 + kCB + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -117,10 +141,23 @@ __global__ void __launch_bounds__(256) k_syrk_tile(double* W, int n, int k0) {
   const int tj = b - ti * (ti + 1) / 2;
   const int k1 = k0 + kCB;
   const int i0 = k1 + ti * kCB, j0 = k1 + tj * kCB;
-  for (int q = threadIdx.x; q < kCB * kCB; q += blockDim.x) {
-    const int r = q % kCB, t = q / kCB;  // consecutive threads: consecutive rows
-    Pi[r][t] = i0 + r < n ? W[(size_t)(i0 + r) + (size_t)(k0 + t) * n] : 0.0;
-    Pj[r][t] = j0 + r < n ? W[(size_t)(j0 + r) + (size_t)(k0 + t) * n] : 0.0;
+  {
+    // all of a thread's loads issued before its LDS stores
+    constexpr int PER = kCB * kCB / 256;
+    double vi[PER], vj[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int q = threadIdx.x + 256 * u;
+      const int r = q % kCB, t = q / kCB;  // consecutive threads: consecutive rows
+      vi[u] = i0 + r < n ? W[(size_t)(i0 + r) + (size_t)(k0 + t) * n] : 0.0;
+      vj[u] = j0 + r < n ? W[(size_t)(j0 + r) + (size_t)(k0 + t) * n] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int q = threadIdx.x + 256 * u;
+      Pi[q % kCB][q / kCB] = vi[u];
+      Pj[q % kCB][q / kCB] = vj[u];
+    }
   }
   __syncthreads();
   const int r = threadIdx.x % kCB, c0 = (threadIdx.x / kCB) * 4;  // row r, columns c0..c0+3
@@ -147,10 +184,7 @@ template <bool FWD>
 __global__ void __launch_bounds__(128) k_trsv_block(const double* W, int n, int k0, double* X, int nrhs) {
   __shared__ double L[kCB][kCLd];
   const int nb = min(kCB, n - k0);
-  for (int q = threadIdx.x; q < kCB * kCB; q += blockDim.x) {
-    const int r = q / kCB, c = q % kCB;
-    L[r][c] = (r < nb && c <= r) ? W[(size_t)(k0 + r) + (size_t)(k0 + c) * n] : 0.0;
-  }
+  load_diag_block<128>(W, n, k0, nb, L);
   __syncthreads();
   const int k = threadIdx.x;
   if (k >= nrhs) return;
@@ -204,7 +238,9 @@ __global__ void __launch_bounds__(256) k_bwd_update(const double* W, int n, int 
     const int i = (int)(q / nrhs), k = (int)(q % nrhs);
     const double* li = W + (size_t)k0 + (size_t)i * n;  // L[k0 + t][i], t = 0 .. nb-1, contiguous
     double s = 0.0;
-    for (int t = 0; t < nb; t++) s += li[t] * X[(size_t)(k0 + t) * nrhs + k];
+#pragma unroll
+    for (int t = 0; t < kCB; t++)
+      if (t < nb) s += li[t] * X[(size_t)(k0 + t) * nrhs + k];
     X[(size_t)i * nrhs + k] -= s;
   }
 }
