@@ -24,9 +24,9 @@ Other workloads (`--workload`, one JSON line each, not the headline):
       link matrix every iteration); reports the CG iterations per solve and
       the achieved HBM bandwidth of the row prior update;
   c3f the same data with 4,000-dim row features, for which the link matrix
-      takes the F^T F Cholesky branch (side information with at most 10,000
-      features, as the Python API chooses); reports the FP64 rate of the
-      per-iteration factorisation;
+      takes the F^T F branch (side information with at most 10,000 features,
+      as the Python API chooses): one eigendecomposition of F^T F at setup,
+      then two FP64 products per iteration; reports their HBM rate;
   c4  3-mode tensor 50,000 x 2,000 x 100, 10M cells, K = 64;
   c3u the c3 data with the univariate prior (MacauOnePrior) on the rows:
       coordinate-wise row updates and, per latent, a sequential sweep over
@@ -80,7 +80,7 @@ def load_workload(name, seed):
     if name in ("c3", "c3f", "c3u"):
         nfeat = 4000 if name == "c3f" else 100000
         (idx, vals), (tidx, tvals), dims, settings, side = synthetic.c3(seed, nfeat=nfeat)
-        how = "CG tol 1e-6" if nfeat > MAX_FF_SIZE else "F^T F Cholesky"
+        how = "CG tol 1e-6" if nfeat > MAX_FF_SIZE else "F^T F branch"
         if name == "c3u":
             settings = dict(settings, univariate=True)
             how = "univariate prior (coordinate-wise rows, feature-sequential link matrix)"
@@ -165,6 +165,7 @@ def time_sampler(name, w, steps, warmup, seed, log):
         phase = np.zeros(2 * nm + 2)
         check(L.macau_profile(h, nprof, ptr(phase)))
         cg = [int(L.macau_get_last_cg_iters(h, m)) for m in range(nm)]
+        cg.append(int(L.macau_get_ftf_solver(h, 0)))  # F^T F solver of the rows
         log("%s: %d steps in %.3f s -> %.1f it/s; phases(ms) %s" %
             (name, steps, t1 - t0, steps / (t1 - t0), np.round(phase, 3).tolist()))
     finally:
@@ -318,13 +319,24 @@ def main():
                                    "note": "one wave per latent walks the features in order; time includes the "
                                            "Normal-Wishart draw, the parallel pre-pass, Uhat and lambda_beta"}
     elif w["side"][0] is not None and w["side"][0].shape[1] <= MAX_FF_SIZE:
-        # the row prior update is dominated by the Cholesky of F^T F + lambda_beta I
         nf = w["side"][0].shape[1]
-        tf = (nf ** 3 / 3.0) / (ph[nm] * 1e-3) / 1e12
-        out["ftf_cholesky"] = {"features": nf, "prior_rows_ms": float(ph[nm]), "factor_flops": nf ** 3 / 3.0,
-                               "achieved_TFLOPs_lower_bound": tf, "frac_of_fp64_peak": tf / PEAK_FP64_TFLOPS,
-                               "note": "time includes F^T F + lambda I, the two K-column solves, the beta noise "
-                                       "draw, Uhat and lambda_beta updates"}
+        solver = res[head]["cg"][-1]
+        if solver == 2:
+            # (F^T F + lambda_beta I)^{-1} as Q diag(d + lambda_beta)^{-1} Q^T: two n x n FP64
+            # products per iteration stream Q and Q^T once each
+            qb = 2.0 * nf * nf * 8
+            gbs = qb / (ph[nm] * 1e-3) / 1e9
+            out["ftf_spectral"] = {"features": nf, "prior_rows_ms": float(ph[nm]), "eigenvector_bytes": qb,
+                                   "achieved_GBs_lower_bound": gbs, "frac_of_hbm_peak": gbs / PEAK_HBM_GBS,
+                                   "note": "time includes the two products, the beta noise draw, F^T Y, Uhat "
+                                           "and lambda_beta updates and the Normal-Wishart draw"}
+        else:
+            # the row prior update is dominated by the Cholesky of F^T F + lambda_beta I
+            tf = (nf ** 3 / 3.0) / (ph[nm] * 1e-3) / 1e12
+            out["ftf_cholesky"] = {"features": nf, "prior_rows_ms": float(ph[nm]), "factor_flops": nf ** 3 / 3.0,
+                                   "achieved_TFLOPs_lower_bound": tf, "frac_of_fp64_peak": tf / PEAK_FP64_TFLOPS,
+                                   "note": "time includes F^T F + lambda I, the two K-column solves, the beta "
+                                           "noise draw, Uhat and lambda_beta updates"}
     elif w["side"][0] is not None:
         # the row prior update is dominated by the block CG for the link matrix
         iters = res[head]["cg"][0]

@@ -112,6 +112,13 @@ int macau_get_hyper(macau_model* m, int mode, double* mu, double* Lambda); /* K,
 int macau_get_link(macau_model* m, int mode, double* beta);                /* K x nfeat */
 double macau_get_link_lambda(macau_model* m, int mode);
 int macau_get_last_cg_iters(macau_model* m, int mode);
+/* Which solve the F^T F branch uses after macau_setup (no counterpart in the
+ * reference, which always factors F^T F + lambda_beta I by Cholesky,
+ * latentprior.cpp:280-288): 0 = the mode has no F^T F branch, 1 = blocked
+ * Cholesky every iteration, 2 = eigendecomposition of F^T F at setup and two
+ * products per iteration (the default; environment variable
+ * MACAU_AMD_FTF_SOLVER=cholesky selects 1). */
+int macau_get_ftf_solver(macau_model* m, int mode);
 
 /* ---- stateless hot-path entry points ---- */
 

@@ -65,6 +65,7 @@ SIGNATURES = {
     "macau_get_link": (_i, [_p, _i, _p]),
     "macau_get_link_lambda": (_d, [_p, _i]),
     "macau_get_last_cg_iters": (_i, [_p, _i]),
+    "macau_get_ftf_solver": (_i, [_p, _i]),
     "macau_sample_latents": (_i, [_i, _i, _i, _p, _p, _p, _d, _p, _p, _d, _p, _p, _p, _ull, _i, _i, _i, _p]),
     "macau_cond_normal_wishart": (_i, [_i, _i, _p, _p, _d, _p, _d, _ull, _i, _i, _p, _p]),
     "macau_solve_blockcg": (_i, [_i, _i, _i, _ll, _p, _p, _p, _d, _p, _d, _p, _p]),

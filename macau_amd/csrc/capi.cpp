@@ -313,6 +313,13 @@ int macau_get_last_cg_iters(macau_model* h, int mode) {
   return p ? p->last_cg_iters : -1;
   API_END_VAL(-1)
 }
+int macau_get_ftf_solver(macau_model* h, int mode) {
+  API_BEGIN
+  Model& m = M(h);
+  auto* p = dynamic_cast<MacauPrior*>(m.priors.at(mode).get());
+  return p && p->use_ftf ? p->ftf_solver : 0;
+  API_END_VAL(-1)
+}
 
 // ---------------------------------------------------------------- stateless
 int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr, const int* idx, const double* vals,

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace mgpu {
 
 constexpr int kMaxOther = 4;  // up to 5-mode tensors
@@ -222,6 +224,21 @@ void launch_ftf(const CsrDev& F, const CsrDev& Ft, double* FtF, hipStream_t s);
 // a non-positive pivot sets *fail to kFailDenseChol
 void launch_ftf_solve(const double* FtF, int n, const double* reg, double* W, double* X, int nrhs, int* fail,
                       hipStream_t s);
+// the same solve through F^T F = Q diag(d) Q^T: spectral_factor overwrites
+// A (F^T F, lower triangle) with Q, fills d (n) and Qt = Q^T and returns
+// false with a reason when the eigensolver is unavailable or fails;
+// launch_spectral_solve sets X = Q diag(d + *reg)^{-1} Q^T B through
+// Y (n x nrhs) and part (plan.S x n x nrhs), a non-positive d_j + *reg
+// setting *fail to kFailDenseChol
+struct SpectralPlan {
+  int S = 1;       // inner-dimension chunks
+  int tchunk = 0;  // chunk depth
+};
+SpectralPlan spectral_plan(int n, int nrhs);
+bool spectral_factor(double* A, int n, double* d, double* Qt, hipStream_t s, std::string* why);
+void launch_spectral_solve(const double* Q, const double* Qt, const double* d, int n, const double* reg,
+                           const double* B, double* Y, double* part, const SpectralPlan& p, double* X, int nrhs,
+                           int* fail, hipStream_t s);
 constexpr int kSpmmSegNnz = 256;
 // out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr
 // (any nr; beyond 64 the columns are taken in further passes of 64);

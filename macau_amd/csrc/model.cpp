@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -865,6 +866,22 @@ void MacauPrior::init(Model& m, int mode) {
     FtF.alloc(nn);
     Wchol.alloc(nn);
     launch_ftf(F(), Ft(), FtF.as<double>(), m.stream);
+    ftf_solver = kFtfCholesky;
+    const char* env = std::getenv("MACAU_AMD_FTF_SOLVER");
+    if (env && std::strcmp(env, "cholesky") == 0) {
+      ftf_note = "MACAU_AMD_FTF_SOLVER=cholesky";
+    } else {
+      ftf_d.alloc((size_t)nfeat * sizeof(double));
+      if (spectral_factor(FtF.as<double>(), nfeat, ftf_d.as<double>(), Wchol.as<double>(), m.stream, &ftf_note)) {
+        ftf_solver = kFtfSpectral;
+        ftf_plan = spectral_plan(nfeat, K);
+        ftf_y.alloc(kf * sizeof(double));
+        ftf_part.alloc((size_t)ftf_plan.S * kf * sizeof(double));
+      } else {
+        // the eigensolver may have overwritten F^T F: form it again
+        launch_ftf(F(), Ft(), FtF.as<double>(), m.stream);
+      }
+    }
   }
   HIPCHK(hipStreamSynchronize(m.stream));
 }
@@ -954,7 +971,14 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   launch_beta_tmp(b, tmp.as<double>(), s);
   launch_spmm(Ft(), tmp.as<double>(), K, Fty.as<double>(), nullptr, nullptr, s);
   launch_beta_noise_add(b, Fty.as<double>(), s);
-  if (use_ftf) {
+  if (use_ftf && ftf_solver == kFtfSpectral) {
+    // beta = Ft_y (FtF + lambda_beta I)^{-1} (latentprior.cpp:280-288)
+    // through the eigendecomposition of FtF made at init
+    launch_spectral_solve(FtF.as<double>(), Wchol.as<double>(), ftf_d.as<double>(), nfeat, lambda_beta_d.as<double>(),
+                          Fty.as<double>(), ftf_y.as<double>(), ftf_part.as<double>(), ftf_plan, beta.as<double>(), K,
+                          m.d_fail.as<int>(), s);
+    last_cg_iters = 0;
+  } else if (use_ftf) {
     // K = FtF + lambda_beta I; chol_decomp(K); chol_solve_t(K, Ft_y);
     // beta = Ft_y (latentprior.cpp:280-288)
     HIPCHK(hipMemcpyAsync(beta.as<void>(), Fty.as<void>(), (size_t)K * nfeat * sizeof(double),

@@ -90,6 +90,8 @@ class BPMFPrior : public ILatentPrior {
   void update_prior(Model& m, int mode) override;
 };
 
+constexpr int kFtfCholesky = 1, kFtfSpectral = 2;
+
 // MacauPrior<SparseDoubleFeat> (latentprior.h:82-125, latentprior.cpp:180-331)
 class MacauPrior : public ILatentPrior {
  public:
@@ -115,7 +117,15 @@ class MacauPrior : public ILatentPrior {
   // F^T F branch of sample_beta (latentprior.cpp:280-289): set for side
   // information with at most max_ff_size features (macau.pyx:160)
   bool use_ftf = false;
-  DevBuf FtF, Wchol;  // F^T F (lower, FP64) and the per-iteration Cholesky work matrix
+  // F^T F (lower, FP64) and the per-iteration Cholesky work matrix; with the
+  // spectral solve they hold the eigenvectors Q and Q^T instead
+  DevBuf FtF, Wchol;
+  // F^T F solver: kFtfCholesky, or kFtfSpectral after a successful
+  // eigendecomposition at init (MACAU_AMD_FTF_SOLVER=cholesky forces the former)
+  int ftf_solver = 0;
+  std::string ftf_note;  // why the spectral solve is not used, when it is not
+  DevBuf ftf_d, ftf_y, ftf_part;
+  SpectralPlan ftf_plan;
   CsrDev F() const;
   CsrDev Ft() const;
   // uploads F and F^T and builds their SpMM work lists (nrmax right-hand sides)

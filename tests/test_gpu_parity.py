@@ -251,12 +251,17 @@ def test_model_macau_matches_oracle():
     assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
 
 
-@pytest.mark.parametrize("nfeat,binary", [(70, True), (130, False), (33, False)])
-def test_model_macau_ftf_matches_oracle(nfeat, binary):
-    """The F^T F branch of sample_beta (latentprior.cpp:280-289): F^T F on the
-    GPU, blocked Cholesky of F^T F + lambda_beta I, two blocked solves; feature
-    counts that leave a partial last 32-column block."""
-    announce("model macau F^T F branch nfeat=%d binary=%d" % (nfeat, binary))
+@pytest.mark.parametrize("solver", ["spectral", "cholesky"])
+@pytest.mark.parametrize("nfeat,binary", [(70, True), (130, False), (33, False), (700, True)])
+def test_model_macau_ftf_matches_oracle(nfeat, binary, solver, monkeypatch):
+    """The F^T F branch of sample_beta (latentprior.cpp:280-289), F^T F formed
+    on the GPU, against the oracle's Cholesky solve.  "cholesky": blocked
+    Cholesky of F^T F + lambda_beta I and two blocked solves every iteration;
+    "spectral": one eigendecomposition of F^T F at setup and two products per
+    iteration (700 features: six inner-dimension chunks).  Feature counts
+    leave partial 32-column blocks and 128-row tiles."""
+    announce("model macau F^T F branch nfeat=%d binary=%d solver=%s" % (nfeat, binary, solver))
+    monkeypatch.setenv("MACAU_AMD_FTF_SOLVER", solver)
     rng = np.random.default_rng(nfeat)
     nrow, ncol, K = 300, 90, 8
     Fm = rng.random((nrow, nfeat)) < 0.08
@@ -286,6 +291,8 @@ def test_model_macau_ftf_matches_oracle(nfeat, binary):
     assert e0 < TOL_FP32 and e1 < TOL_FP32
     assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < TOL_FP32
     assert _lib.lib().macau_get_last_cg_iters(m.h, 0) == 0
+    assert _lib.lib().macau_get_ftf_solver(m.h, 0) == (2 if solver == "spectral" else 1)
+    assert _lib.lib().macau_get_ftf_solver(m.h, 1) == 0
     m.iterate(8)
     ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, **kw)
     assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
