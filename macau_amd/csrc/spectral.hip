@@ -25,26 +25,34 @@
 
 namespace mgpu {
 
-constexpr int kSmRows = 128;        // output rows per work-group (4 per thread)
-constexpr int kSmK = 32;            // right-hand sides per work-group (4 per thread)
+constexpr int kSmRows = 128;        // output rows per work-group (32 per wave)
+constexpr int kSmK = 32;            // right-hand sides per work-group
 constexpr int kSmT = 32;            // inner-dimension slab staged in LDS
-constexpr int kSmLd = kSmRows + 2;  // LDS row stride of the M slab (16-byte aligned rows)
+// LDS strides chosen so each 64-lane operand read (16 rows x 4 inner
+// indices, or 4 inner rows x 16 columns) takes the minimum two bank passes
+constexpr int kMsLd = kSmT + 4;
+constexpr int kXsLd = kSmK + 16;
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // part[blockIdx.y][r][k] = sum over t in chunk blockIdx.y of M[t + r n] X[t][k]
 // (M column-major n x n, so the sum runs down column r of M: out = M^T X).
 // Work-group (x, y, z): rows r0 .. r0 + 127, inner chunk y, columns
-// k0 .. k0 + 31; thread (rg, kg) keeps a 4 x 4 register tile.  The next
+// k0 .. k0 + 31.  Wave w owns rows r0 + 32 w .. + 31 as 2 x 2 FP64 MFMA
+// tiles of 16 x 16 (v_mfma_f64_16x16x4f64: lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16] and D[4 v + l / 16][l % 16], v = 0..3).  The next
 // slab's global loads are issued before the current slab is consumed.
-__global__ void __launch_bounds__(256) k_spec_mm(const double* __restrict__ M, int n, const double* __restrict__ X,
+__global__ void __launch_bounds__(256, 3) k_spec_mm(const double* __restrict__ M, int n, const double* __restrict__ X,
                                                  int nrhs, int tchunk, double* __restrict__ part) {
-  __shared__ __align__(16) double Ms[kSmT][kSmLd];
-  __shared__ __align__(16) double Xs[kSmT][kSmK];
+  __shared__ __align__(16) double Ms[kSmRows][kMsLd];  // [row][inner]
+  __shared__ __align__(16) double Xs[kSmT][kXsLd];     // [inner][column]
   const int r0 = blockIdx.x * kSmRows;
   const int k0 = blockIdx.z * kSmK;
   const int tb = blockIdx.y * tchunk;
   const int te = min(n, tb + tchunk);
   const int tid = threadIdx.x;
-  const int rg = tid & 31, kg = tid >> 5;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   double mv[16], xv[4];
   auto load = [&](int t0) {
 #pragma unroll
@@ -60,18 +68,18 @@ __global__ void __launch_bounds__(256) k_spec_mm(const double* __restrict__ M, i
       xv[u] = (gt < te && gk < nrhs) ? X[(size_t)gt * nrhs + gk] : 0.0;
     }
   };
-  double acc[4][4];
+  f64x4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+    for (int b = 0; b < 2; b++) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
   if (tb < te) load(tb);
   for (int t0 = tb; t0 < te; t0 += kSmT) {
     __syncthreads();  // the previous slab has been consumed
 #pragma unroll
     for (int u = 0; u < 16; u++) {
       const int q = tid + 256 * u;
-      Ms[q & 31][q >> 5] = mv[u];
+      Ms[q >> 5][q & 31] = mv[u];
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -80,31 +88,29 @@ __global__ void __launch_bounds__(256) k_spec_mm(const double* __restrict__ M, i
     }
     __syncthreads();
     if (t0 + kSmT < te) load(t0 + kSmT);
-#pragma unroll 8
-    for (int t = 0; t < kSmT; t++) {
-      const double2 m01 = *reinterpret_cast<const double2*>(&Ms[t][4 * rg]);
-      const double2 m23 = *reinterpret_cast<const double2*>(&Ms[t][4 * rg + 2]);
-      const double2 x01 = *reinterpret_cast<const double2*>(&Xs[t][4 * kg]);
-      const double2 x23 = *reinterpret_cast<const double2*>(&Xs[t][4 * kg + 2]);
-      const double m[4] = {m01.x, m01.y, m23.x, m23.y};
-      const double x[4] = {x01.x, x01.y, x23.x, x23.y};
 #pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = fma(m[a], x[b], acc[a][b]);
+    for (int st = 0; st < kSmT / 4; st++) {
+      const int tt = 4 * st + lk;
+      const double a0 = Ms[32 * w + li][tt], a1 = Ms[32 * w + 16 + li][tt];
+      const double b0 = Xs[tt][li], b1 = Xs[tt][16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
   }
   double* out = part + (size_t)blockIdx.y * n * nrhs;
 #pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const int r = r0 + 4 * rg + a;
-    if (r >= n) continue;
+  for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int k = k0 + 4 * kg + b;
-      if (k < nrhs) out[(size_t)r * nrhs + k] = acc[a][b];
+    for (int b = 0; b < 2; b++) {
+      const int k = k0 + 16 * b + li;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = r0 + 32 * w + 16 * a + 4 * v + lk;
+        if (r < n && k < nrhs) out[(size_t)r * nrhs + k] = acc[a][b][v];
+      }
     }
-  }
 }
 
 // out[r][k] = sum_s part[s][r][k] in fixed order s = 0 .. S-1, divided by
