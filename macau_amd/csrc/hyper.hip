@@ -748,32 +748,57 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
   const int n = it < a.burnin ? 0 : it - a.burnin;
   const int sub = threadIdx.x % G;
   const int cells_per_block = kEvalThreads / G;
+  // U consecutive cells per lane group and loop step: their index loads,
+  // then their factor-row loads, are issued together, and after the
+  // lane-group sums lane u of the group finishes cell u, so the FP64 running
+  // mean and variance run once per U cells instead of once per cell
+  // (U <= G: every cell needs an owner lane in its group)
+  constexpr int U = PROBIT ? 1 : (G < 4 ? G : 4);
   double se = 0.0, se_avg = 0.0;
-  for (int q0 = blockIdx.x * cells_per_block; q0 < a.ntest; q0 += gridDim.x * cells_per_block) {
-    const int q = q0 + threadIdx.x / G;
-    float dot = 0.f;
-    if (q < a.ntest) {
-      const int* ix = a.idx + (size_t)q * NM;
-      float4 v = reinterpret_cast<const float4*>(a.f[0] + (size_t)ix[0] * KP)[sub];
+  for (int q0 = blockIdx.x * cells_per_block * U; q0 < a.ntest; q0 += gridDim.x * cells_per_block * U) {
+    int qs[U], ix[U][NM];
 #pragma unroll
-      for (int m = 1; m < NM; m++) {
-        const float4 w = reinterpret_cast<const float4*>(a.f[m] + (size_t)ix[m] * KP)[sub];
-        v.x *= w.x;
-        v.y *= w.y;
-        v.z *= w.z;
-        v.w *= w.w;
-      }
-      dot = (v.x + v.y) + (v.z + v.w);
+    for (int u = 0; u < U; u++) {
+      qs[u] = q0 + (threadIdx.x / G) * U + u;
+#pragma unroll
+      for (int m = 0; m < NM; m++) ix[u][m] = qs[u] < a.ntest ? a.idx[(size_t)qs[u] * NM + m] : -1;
     }
+    float dot[U];
 #pragma unroll
-    for (int off = 1; off < G; off <<= 1) dot += __shfl_xor(dot, off, G);
-    if (sub == 0 && q < a.ntest) {
+    for (int u = 0; u < U; u++) {
+      dot[u] = 0.f;
+      if (ix[u][0] >= 0) {
+        float4 v = reinterpret_cast<const float4*>(a.f[0] + (size_t)ix[u][0] * KP)[sub];
+#pragma unroll
+        for (int m = 1; m < NM; m++) {
+          const float4 w = reinterpret_cast<const float4*>(a.f[m] + (size_t)ix[u][m] * KP)[sub];
+          v.x *= w.x;
+          v.y *= w.y;
+          v.z *= w.z;
+          v.w *= w.w;
+        }
+        dot[u] = (v.x + v.y) + (v.z + v.w);
+      }
+    }
+    float d = 0.f;
+    int q = a.ntest;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      float du = dot[u];
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) du += __shfl_xor(du, off, G);
+      if (sub == u) {
+        d = du;
+        q = qs[u];
+      }
+    }
+    if (sub < U && q < a.ntest) {
       double p;
       if constexpr (PROBIT) {
-        p = 0.5 * erfc(-(double)dot * M_SQRT1_2);
+        p = 0.5 * erfc(-(double)d * M_SQRT1_2);
         a.pred1[q] = p;
       } else {
-        p = a.mean + (double)dot;
+        p = a.mean + (double)d;
       }
       double pa;
       if (n == 0) {

@@ -289,9 +289,30 @@ void Model::init() {
   d_eval_out.alloc(2 * sizeof(double));
   d_sse.alloc(sizeof(double));
   d_alpha.alloc(sizeof(double));
-  // test cells
-  upload(d_test_idx, test_idx, stream);
-  upload(d_test_vals, test_vals, stream);
+  // test cells, on the device in the order of the largest mode's index so
+  // that consecutive cells share that mode's factor row (L2 hits in the
+  // evaluation's gathers); d_pred and d_pred_var follow this order and
+  // test_to_host restores the caller's
+  {
+    int km = 0;
+    for (int m = 1; m < nmodes; m++)
+      if (dims[m] > dims[km]) km = m;
+    test_perm.resize(ntest);
+    std::iota(test_perm.begin(), test_perm.end(), 0LL);
+    std::stable_sort(test_perm.begin(), test_perm.end(), [&](long long a, long long b) {
+      return test_idx[(size_t)a * nmodes + km] < test_idx[(size_t)b * nmodes + km];
+    });
+    std::vector<int> didx((size_t)ntest * nmodes);
+    std::vector<double> dvals(ntest);
+    for (long long t = 0; t < ntest; t++) {
+      const long long q = test_perm[t];
+      for (int m = 0; m < nmodes; m++) didx[(size_t)t * nmodes + m] = test_idx[(size_t)q * nmodes + m];
+      dvals[t] = test_vals[q];
+    }
+    upload(d_test_idx, didx, stream);
+    upload(d_test_vals, dvals, stream);
+    HIPCHK(hipStreamSynchronize(stream));  // didx and dvals go out of scope
+  }
   d_pred.alloc(std::max<long long>(ntest, 1) * sizeof(double));
   d_pred_var.alloc(std::max<long long>(ntest, 1) * sizeof(double));
   HIPCHK(hipMemsetAsync(d_pred.as<void>(), 0, d_pred.bytes(), stream));
@@ -634,25 +655,21 @@ void Model::printStatus(int i, double elapsedi) {  // macau.cpp:151-172
   fflush(stdout);
 }
 
-std::vector<double> Model::getPredictions() {
-  std::vector<double> p(ntest);
+std::vector<double> Model::test_to_host(const DevBuf& d) {
+  std::vector<double> dev(ntest), out(ntest);
   if (ntest) {
-    HIPCHK(hipMemcpyAsync(p.data(), d_pred.as<void>(), ntest * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(dev.data(), d.as<void>(), ntest * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
-  return p;
+  for (long long t = 0; t < ntest; t++) out[test_perm[t]] = dev[t];
+  return out;
 }
 
+std::vector<double> Model::getPredictions() { return test_to_host(d_pred); }
+
 std::vector<double> Model::getStds() {  // macau.cpp:175-188
-  std::vector<double> s(ntest);
-  if (nsamples <= 1) {
-    std::fill(s.begin(), s.end(), NAN);
-    return s;
-  }
-  if (ntest) {
-    HIPCHK(hipMemcpyAsync(s.data(), d_pred_var.as<void>(), ntest * sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-  }
+  if (nsamples <= 1) return std::vector<double>(ntest, NAN);
+  std::vector<double> s = test_to_host(d_pred_var);
   const double inorm = 1.0 / (nsamples - 1);
   for (auto& x : s) x = std::sqrt(x * inorm);
   return s;
@@ -1329,14 +1346,7 @@ std::string ProbitNoise::getEvalString() {  // noisemodels.h:100
 }
 
 void ProbitNoise::sync_host(Model& m) {
-  std::vector<double> pred(m.ntest), pred1(m.ntest);
-  if (m.ntest) {
-    HIPCHK(hipMemcpyAsync(pred.data(), m.d_pred.as<void>(), m.ntest * sizeof(double), hipMemcpyDeviceToHost,
-                          m.stream));
-    HIPCHK(hipMemcpyAsync(pred1.data(), d_pred1.as<void>(), m.ntest * sizeof(double), hipMemcpyDeviceToHost,
-                          m.stream));
-    HIPCHK(hipStreamSynchronize(m.stream));
-  }
+  const std::vector<double> pred = m.test_to_host(m.d_pred), pred1 = m.test_to_host(d_pred1);
   auc_test_onesample = auc(pred1, m.test_vals);
   auc_test = auc(pred, m.test_vals);
 }

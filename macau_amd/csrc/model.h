@@ -273,6 +273,10 @@ class Model {
   std::vector<int> test_idx;  // ntest x nmodes
   std::vector<double> test_vals;
   long long ntest = 0;
+  // device order of the test cells: device cell t is test cell test_perm[t]
+  std::vector<long long> test_perm;
+  // a per-test-cell device vector (d_pred, d_pred_var, ...) in the caller's order
+  std::vector<double> test_to_host(const DevBuf& d);
 
   // -- device state --
   hipStream_t stream = nullptr;
