@@ -155,13 +155,14 @@ SpectralPlan spectral_plan(int n, int nrhs) {
   SpectralPlan p;
   const int row_tiles = (n + kSmRows - 1) / kSmRows;
   const int k_tiles = (nrhs + kSmK - 1) / kSmK;
-  // enough inner chunks for about 1024 work-groups, each at least 128 deep
-  int S = (1024 + row_tiles * k_tiles - 1) / (row_tiles * k_tiles);
+  // enough inner chunks for one resident round of work-groups (three per
+  // CU on 256 CUs: 768), each at least 128 deep; the slab loop handles a
+  // chunk depth that is not a multiple of the slab
+  int S = (768 + row_tiles * k_tiles - 1) / (row_tiles * k_tiles);
   const int smax = (n + 127) / 128;
   if (S > smax) S = smax;
   if (S < 1) S = 1;
-  int tchunk = (n + S - 1) / S;
-  tchunk = (tchunk + kSmT - 1) / kSmT * kSmT;
+  const int tchunk = (n + S - 1) / S;
   p.S = (n + tchunk - 1) / tchunk;
   p.tchunk = tchunk;
   return p;
