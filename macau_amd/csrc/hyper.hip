@@ -753,7 +753,7 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
   // lane-group sums lane u of the group finishes cell u, so the FP64 running
   // mean and variance run once per U cells instead of once per cell
   // (U <= G: every cell needs an owner lane in its group)
-  constexpr int U = PROBIT ? 1 : (G < 4 ? G : 4);
+  constexpr int U = G < 4 ? G : 4;
   double se = 0.0, se_avg = 0.0;
   for (int q0 = blockIdx.x * cells_per_block * U; q0 < a.ntest; q0 += gridDim.x * cells_per_block * U) {
     int qs[U], ix[U][NM];
