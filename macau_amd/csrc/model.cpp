@@ -497,8 +497,10 @@ void Model::step_launch() {
 void Model::iterate(int n) {
   if (!initialized) throw std::runtime_error("model not initialized");
   bool graphable = use_graph;
-  for (auto& p : priors)
-    if (dynamic_cast<MacauPrior*>(p.get())) graphable = false;
+  for (auto& p : priors) {
+    const auto* mp = dynamic_cast<const MacauPrior*>(p.get());
+    if (mp && !mp->capturable()) graphable = false;
+  }
   for (int i = 0; i < n; i++) {
     if (graphable) {
       if (!graph_exec) {

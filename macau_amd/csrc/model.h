@@ -103,6 +103,10 @@ class MacauPrior : public ILatentPrior {
   double getLinkLambda(Model& m) override;
   void saveModel(Model& m, int mode, const std::string& prefix) override;
   const float* prior_mean_rows() const override { return Lmu_rows.as<float>(); }
+  // whether update_prior can be captured in a HIP graph: the block CG reads
+  // its convergence flags on the host every 8 iterations; the F^T F branch
+  // launches a fixed sequence of kernels
+  virtual bool capturable() const { return use_ftf; }
 
   int nrow, nfeat;
   double lambda_beta0, tol;
@@ -141,6 +145,7 @@ class MacauPrior : public ILatentPrior {
 // MacauPrior.
 class MacauOnePrior : public MacauPrior {
  public:
+  bool capturable() const override { return true; }  // a fixed sequence of kernels
   MacauOnePrior(int nrow, int ncol, const std::vector<int>& rows, const std::vector<int>& cols,
                 const std::vector<double>& vals, bool binary, double lambda_beta);
   void init(Model& m, int mode) override;
