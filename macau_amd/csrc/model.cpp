@@ -917,8 +917,13 @@ void CgBuffers::alloc(int nr, int nfeat, int nsamples) {
   small.alloc((size_t)(5 * nr * nr + nr + 4) * sizeof(double));
 }
 
+CgBuffers::~CgBuffers() {
+  for (auto& c : chunks)
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+}
+
 int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
-                   double tol, double* X, CgBuffers& b, hipStream_t s) {
+                   double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph) {
   if (nr > b.nrmax || nr > 40) throw std::runtime_error("BlockCG only available for up to 40 RHSs.");
   const int nfeat = F.ncol;
   CgState c;
@@ -943,15 +948,45 @@ int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const d
   const double* dflags = c.small + 5 * nr * nr + nr;
   // 8 iterations per host round trip; kernels of iterations after
   // convergence return immediately (device-side done flag)
-  for (int it = 0; it < 100000; it += 8) {
+  auto launch_chunk = [&](hipStream_t st) {
     for (int q = 0; q < 8; q++) {
-      launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, s, dflags);
-      launch_spmm(Ft, c.inner, nr, c.KPm, reg, c.P, s, dflags);
-      launch_cg_step_a(c, s);
-      launch_cg_update_xr(c, s);
-      launch_cg_step_b(c, tolsq, s);
-      launch_cg_update_p(c, s);
+      launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, st, dflags);
+      launch_spmm(Ft, c.inner, nr, c.KPm, reg, c.P, st, dflags);
+      launch_cg_step_a(c, st);
+      launch_cg_update_xr(c, st);
+      launch_cg_step_b(c, tolsq, st);
+      launch_cg_update_p(c, st);
     }
+  };
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    double tolsq_bits = tolsq;
+    uintptr_t tb = 0;
+    std::memcpy(&tb, &tolsq_bits, sizeof(tb));
+    std::vector<uintptr_t> key = {(uintptr_t)reg, (uintptr_t)c.X, (uintptr_t)c.small, (uintptr_t)nr, tb};
+    for (const CsrDev* A : {&F, &Ft}) {
+      const uintptr_t f[] = {(uintptr_t)A->nrow, (uintptr_t)A->ncol, (uintptr_t)A->row_ptr, (uintptr_t)A->cols,
+                             (uintptr_t)A->vals, (uintptr_t)A->nseg, (uintptr_t)A->seg, (uintptr_t)A->nmulti,
+                             (uintptr_t)A->multi_row, (uintptr_t)A->multi_slot0, (uintptr_t)A->partial};
+      key.insert(key.end(), std::begin(f), std::end(f));
+    }
+    for (auto& ch : b.chunks)
+      if (ch.key == key) exec = ch.exec;
+    if (!exec) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      launch_chunk(s);
+      HIPCHK(hipStreamEndCapture(s, &g));
+      HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      b.chunks.push_back({key, exec});
+    }
+  }
+  for (int it = 0; it < 100000; it += 8) {
+    if (exec)
+      HIPCHK(hipGraphLaunch(exec, s));
+    else
+      launch_chunk(s);
     HIPCHK(hipMemcpyAsync(flags, dflags, sizeof(flags), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (flags[0] != 0.0) break;
@@ -1010,13 +1045,13 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
     const int blocksize = 32, excess = 8;
     if (K <= excess + blocksize) {
       last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, 0, K, tol,
-                                     beta.as<double>(), cg, s);
+                                     beta.as<double>(), cg, s, m.use_graph);
     } else {
       for (int i = 0; i < K; i += blocksize) {
         int nr = blocksize;
         if (i + blocksize + excess >= K) nr = K - i;
         last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, i, nr, tol,
-                                       beta.as<double>(), cg, s);
+                                       beta.as<double>(), cg, s, m.use_graph);
       }
     }
   }

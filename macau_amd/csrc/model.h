@@ -56,12 +56,25 @@ struct CgBuffers {
   DevBuf X, R, P, KP, B, inner, partial, small;
   int nrmax = 0;
   void alloc(int nrmax, int nfeat, int nsamples);
+  // instantiated graphs of 8 CG iterations, one per launch configuration
+  // (the kernels' arguments: operators, buffers, regulariser, tolerance)
+  struct Chunk {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Chunk> chunks;
+  CgBuffers() = default;
+  CgBuffers(const CgBuffers&) = delete;
+  CgBuffers& operator=(const CgBuffers&) = delete;
+  ~CgBuffers();
 };
 // Solves X[r0:r0+nr, :] (F^T F + reg I) = B[r0:r0+nr, :] with B and X of
 // leading dimension ld (= total right-hand sides); reg is a device scalar.
-// Returns the iteration count of the reference's loop.
+// Returns the iteration count of the reference's loop.  With use_graph the
+// 8 CG iterations launched per host check of the convergence flags are
+// captured once and replayed as a HIP graph.
 int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
-                   double tol, double* X, CgBuffers& b, hipStream_t s);
+                   double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph = false);
 
 // Work list of the batched sampler for one mode (rows sorted by nnz,
 // heavy rows split into segments).

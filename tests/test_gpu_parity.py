@@ -334,15 +334,17 @@ def test_model_macau_wide_latent(ftf):
     assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < tol
 
 
-@pytest.mark.parametrize("kind", ["ftf", "univariate"])
-def test_macau_graph_replay_matches_eager(kind):
-    """A Macau chain replayed as one HIP graph per iteration (the F^T F branch
-    with the spectral solve, or the univariate prior, both captured with their
-    side-stream prior updates), with host reads of the latents between the
-    replays, against the same chain launched eagerly: every replay agrees."""
-    announce("macau graph replay kind=%s" % kind)
+@pytest.mark.parametrize("kind,K", [("ftf", 8), ("univariate", 8), ("cg", 8), ("cg", 70)])
+def test_macau_graph_replay_matches_eager(kind, K):
+    """A Macau chain with graph replay on, against the same chain launched
+    eagerly, with host reads of the latents between iterations: the F^T F
+    branch (spectral solve) and the univariate prior are captured with the
+    whole iteration; the block CG replays its 8-iteration chunks as graphs
+    between its host checks of the convergence flags (K = 70: two CG blocks,
+    32 and 38 right-hand sides, each with its own graph)."""
+    announce("macau graph replay kind=%s K=%d" % (kind, K))
     rng = np.random.default_rng(91)
-    nrow, ncol, K, nfeat = 300, 90, 8, 130
+    nrow, ncol, nfeat = 300, 90, 130
     Fm = rng.random((nrow, nfeat)) < 0.08
     fr, fc = np.nonzero(Fm)
     Ut = rng.standard_normal((nrow, 3))
@@ -357,6 +359,8 @@ def test_macau_graph_replay_matches_eager(kind):
         m.data(np.stack([r[tr], c[tr]], 1), y[tr], [nrow, ncol], np.stack([r[te], c[te]], 1), y[te])
         if kind == "ftf":
             m.prior(side=(fr, fc, None, nrow, nfeat), lambda_beta=5.0, tol=1e-6, max_ff_size=10000)
+        elif kind == "cg":
+            m.prior(side=(fr, fc, None, nrow, nfeat), lambda_beta=5.0, tol=1e-6)
         else:
             m.one_prior((fr, fc, None, nrow, nfeat), lambda_beta=5.0)
         m.prior()
@@ -368,6 +372,9 @@ def test_macau_graph_replay_matches_eager(kind):
         e.iterate(1)
         for mode, n in ((0, nrow), (1, ncol)):
             np.testing.assert_allclose(g.latents(mode, n), e.latents(mode, n), rtol=1e-6, atol=1e-6)
+        if kind == "cg":
+            L = _lib.lib()
+            assert L.macau_get_last_cg_iters(g.h, 0) == L.macau_get_last_cg_iters(e.h, 0) > 0
     assert abs(g.rmse() - e.rmse()) <= 1e-6 * abs(e.rmse())
 
 
