@@ -309,6 +309,28 @@ def test_ftf_solve_matches_dense_solve(n, nrhs):
     np.testing.assert_allclose(X, np.linalg.solve(Fd.T @ Fd + reg * np.eye(n), B), rtol=1e-9, atol=1e-11)
 
 
+@pytest.mark.parametrize("nrow,n", [(200, 60), (40, 90)])
+def test_ftf_solve_equals_spectral_solve(nrow, n):
+    """The identity the GPU's F^T F branch relies on: with F^T F = Q diag(d) Q^T
+    (one eigendecomposition for the whole chain), the oracle's per-iteration
+    Cholesky solve of (F^T F + lambda I) X = B equals Q diag(1/(d + lambda))
+    Q^T B for every lambda, including a rank-deficient F^T F (more features
+    than rows, so d has zeros)."""
+    rng = np.random.default_rng(nrow + n)
+    F = scipy.sparse.random(nrow, n, density=0.1, random_state=nrow, format="coo")
+    F.data[:] = 1.0
+    A, At = O.csr_pair(F.row, F.col, np.ones(F.nnz), nrow, n)
+    FtF = O.at_mul_a(A, At)
+    d, Q = np.linalg.eigh(np.tril(FtF) + np.tril(FtF, -1).T)
+    if nrow < n:
+        assert np.sum(np.abs(d) < 1e-9 * max(1.0, d.max())) >= n - nrow  # rank-deficient
+    B = rng.standard_normal((n, 32))
+    for lam in (0.05, 1.0, 5.0, 250.0):
+        X = O.ftf_solve(FtF, lam, B)
+        Y = Q @ ((Q.T @ B) / (d + lam)[:, None])
+        np.testing.assert_allclose(X, Y, rtol=1e-9, atol=1e-10 * np.abs(X).max())
+
+
 def test_ftf_solve_not_positive_definite():
     FtF = np.array([[1.0, 0.0], [3.0, 1.0]])  # lower triangle of an indefinite matrix
     with pytest.raises(RuntimeError):
