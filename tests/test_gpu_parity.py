@@ -298,6 +298,28 @@ def test_model_macau_ftf_matches_oracle(nfeat, binary, solver, monkeypatch):
     assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
 
 
+@pytest.mark.parametrize("solver", ["spectral", "cholesky"])
+def test_model_macau_ftf_not_positive_definite(solver, monkeypatch):
+    """F^T F + lambda_beta I not positive definite (a negative initial
+    lambda_beta): the F^T F branch reports chol_decomp's failure (chol.cpp:21-28)
+    with either solver -- a non-positive pivot, or a non-positive
+    d_j + lambda_beta -- after its kernels have finished normally."""
+    announce("model macau F^T F not positive definite solver=%s" % solver)
+    monkeypatch.setenv("MACAU_AMD_FTF_SOLVER", solver)
+    rng = np.random.default_rng(5)
+    nrow, ncol, K, nfeat = 120, 60, 8, 40
+    fr, fc = np.nonzero(rng.random((nrow, nfeat)) < 0.1)
+    r, c = np.nonzero(rng.random((nrow, ncol)) < 0.2)
+    y = rng.standard_normal(len(r))
+    m = GpuModel(2, K, 10.0)
+    m.data(np.stack([r, c], 1), y, [nrow, ncol], np.stack([r[:10], c[:10]], 1), y[:10])
+    m.prior(side=(fr, fc, None, nrow, nfeat), lambda_beta=-1e6, tol=1e-6, max_ff_size=10000)
+    m.prior()
+    m.setup(2, 2, SEED, PLAIN, graph=False)
+    with pytest.raises(RuntimeError, match="Cholesky decomp failed"):
+        m.iterate(1)
+
+
 @pytest.mark.parametrize("ftf", [True, False])
 def test_model_macau_wide_latent(ftf):
     """K = 70 (more than 64 right-hand sides in the SpMMs of sample_beta and
