@@ -1,5 +1,8 @@
 // batched.h -- helpers shared by the batched (split-BF16 MFMA) samplers.
 #pragma once
+#include <stdexcept>
+#include <string>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -111,6 +114,17 @@ inline void launch_seg_sum(const BatchPlan& p, hipStream_t s) {
   if (p.n_heavy > 0)
     hipLaunchKernelGGL((k_seg_sum<W>), dim3(p.n_heavy * CH), dim3(256), 0, s, p.partial, p.heavy_seg0, p.n_heavy);
 }
+
+// fork of the heavy-row segment kernels onto SegStreams::stream and their join
+inline void seg_check(hipError_t e) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("batched sampler: stream fork/join: ") + hipGetErrorString(e));
+}
+inline void seg_fork(hipStream_t s, const SegStreams& seg) {
+  seg_check(hipEventRecord(seg.fork, s));
+  seg_check(hipStreamWaitEvent(seg.stream, seg.fork, 0));
+}
+inline void seg_record_join(const SegStreams& seg) { seg_check(hipEventRecord(seg.join, seg.stream)); }
+inline void seg_wait_join(hipStream_t s, const SegStreams& seg) { seg_check(hipStreamWaitEvent(s, seg.join, 0)); }
 
 // wave-sum of the SSE shares into sse_partial[slot]
 __device__ inline void write_sse(const SampleArgs& a, double acc, int slot) {

@@ -67,9 +67,16 @@ struct SampleArgs {
 };
 
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
-void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
+// With seg_stream set, the heavy-row segment kernels run on it beside the
+// light rows: seg_fork is recorded on s before them and seg_join after them,
+// and s waits on seg_join before the heavy rows are finished.
+struct SegStreams {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg = {});
 // K in (32, 64] (KP = 64): one row per wave, three 32 x 32 Gram blocks
-void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s);
+void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg = {});
 int batched64_light_grid(const BatchPlan& p);
 int batched64_heavy_grid(const BatchPlan& p);
 int batched_supported(int KP, int nother);

@@ -233,8 +233,14 @@ Model::~Model() {
     (void)hipStreamSynchronize(side);
     (void)hipStreamDestroy(side);
   }
+  if (seg_stream) {
+    (void)hipStreamSynchronize(seg_stream);
+    (void)hipStreamDestroy(seg_stream);
+  }
   for (auto& e : ev_fork) (void)hipEventDestroy(e);
   if (ev_join) (void)hipEventDestroy(ev_join);
+  for (auto& e : ev_seg_fork) (void)hipEventDestroy(e);
+  for (auto& e : ev_seg_join) (void)hipEventDestroy(e);
 }
 
 FactorSet Model::others_of(int mode) const {
@@ -255,10 +261,15 @@ void Model::init() {
   if (rows.empty()) throw std::runtime_error("training data not set");
   if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   if (!side) HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  if (!seg_stream) HIPCHK(hipStreamCreateWithFlags(&seg_stream, hipStreamNonBlocking));
   if (ev_fork.empty()) {
     ev_fork.resize(nmodes);
     for (auto& e : ev_fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    ev_seg_fork.resize(nmodes);
+    ev_seg_join.resize(nmodes);
+    for (auto& e : ev_seg_fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : ev_seg_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
   if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
@@ -727,7 +738,13 @@ void ILatentPrior::init(Model& m, int mode) {
 
 void ILatentPrior::sample_latents(Model& m, int mode) {
   const SampleArgs a = m.sample_args(mode, *this);
-  if (m.sampler == SAMPLER_BATCHED) launch_sample_batched(a, m.plans[mode], m.stream);
+  if (m.sampler == SAMPLER_BATCHED) {
+    SegStreams seg;
+    seg.stream = m.seg_stream;
+    seg.fork = m.ev_seg_fork[mode];
+    seg.join = m.ev_seg_join[mode];
+    launch_sample_batched(a, m.plans[mode], m.stream, seg);
+  }
   else launch_sample_plain(a, m.stream);
 }
 

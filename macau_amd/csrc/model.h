@@ -301,6 +301,9 @@ class Model {
   hipStream_t side = nullptr;  // prior updates overlapped with sampling (step_launch)
   std::vector<hipEvent_t> ev_fork;
   hipEvent_t ev_join = nullptr;
+  // the batched sampler's heavy-row segment kernels, beside its light rows
+  hipStream_t seg_stream = nullptr;
+  std::vector<hipEvent_t> ev_seg_fork, ev_seg_join;
   std::vector<DevBuf> d_rowptr, d_idx, d_r, d_U;
   std::vector<RowsDev> rdev;
   std::vector<BatchPlan> plans;

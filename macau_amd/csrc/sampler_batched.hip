@@ -461,28 +461,33 @@ int batched_heavy_grid(const BatchPlan& p) {
 }
 
 template <int NO>
-static void launch_batched_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s) {
+static void launch_batched_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {
   const int gl = batched_light_grid(p), gh = batched_heavy_grid(p);
+  const bool fork = p.n_seg > 0 && seg.stream;
   if (p.n_seg > 0) {
+    const hipStream_t ss = fork ? seg.stream : s;
+    if (fork) seg_fork(s, seg);
     const int gs = p.n_seg < 16384 ? p.n_seg : 16384;
-    hipLaunchKernelGGL((k_seg32<NO>), dim3(gs), dim3(64), 0, s, a, p);
-    launch_seg_sum<kPartial>(p, s);
+    hipLaunchKernelGGL((k_seg32<NO>), dim3(gs), dim3(64), 0, ss, a, p);
+    launch_seg_sum<kPartial>(p, ss);
+    if (fork) seg_record_join(seg);
   }
   if (gl > 0) hipLaunchKernelGGL((k_light32<NO>), dim3(gl), dim3(64), 0, s, a, p);
+  if (fork) seg_wait_join(s, seg);
   if (gh > 0) hipLaunchKernelGGL(k_heavy32, dim3(gh), dim3(64), 0, s, a, p, gl);
 }
 
-void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s) {
+void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {
   if (!batched_fits(a.others, a.rows.nother, a.KP))
     throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
   if (a.KP != p.KP || !batched_supported(a.KP, a.rows.nother))
     throw std::runtime_error("batched sampler: plan and latent size disagree or are unsupported");
-  if (a.KP == 64) return launch_sample_batched64(a, p, s);
+  if (a.KP == 64) return launch_sample_batched64(a, p, s, seg);
   switch (a.rows.nother) {
-    case 1: return launch_batched_t<1>(a, p, s);
-    case 2: return launch_batched_t<2>(a, p, s);
-    case 3: return launch_batched_t<3>(a, p, s);
-    case 4: return launch_batched_t<4>(a, p, s);
+    case 1: return launch_batched_t<1>(a, p, s, seg);
+    case 2: return launch_batched_t<2>(a, p, s, seg);
+    case 3: return launch_batched_t<3>(a, p, s, seg);
+    case 4: return launch_batched_t<4>(a, p, s, seg);
     default: break;
   }
 }

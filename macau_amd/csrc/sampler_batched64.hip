@@ -424,23 +424,28 @@ int batched64_light_grid(const BatchPlan& p) { return p.n_pair < 8192 ? p.n_pair
 int batched64_heavy_grid(const BatchPlan& p) { return p.n_heavy < 2048 ? p.n_heavy : 2048; }
 
 template <int NO>
-static void launch64_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s) {
+static void launch64_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {
   const int gl = batched64_light_grid(p), gh = batched64_heavy_grid(p);
+  const bool fork = p.n_seg > 0 && seg.stream;
   if (p.n_seg > 0) {
+    const hipStream_t ss = fork ? seg.stream : s;
+    if (fork) seg_fork(s, seg);
     const int gs = p.n_seg < 16384 ? p.n_seg : 16384;
-    hipLaunchKernelGGL((k_seg64<NO>), dim3(gs), dim3(64), 0, s, a, p);
-    launch_seg_sum<kPartial64>(p, s);
+    hipLaunchKernelGGL((k_seg64<NO>), dim3(gs), dim3(64), 0, ss, a, p);
+    launch_seg_sum<kPartial64>(p, ss);
+    if (fork) seg_record_join(seg);
   }
   if (gl > 0) hipLaunchKernelGGL((k_light64<NO>), dim3(gl), dim3(64), 0, s, a, p);
+  if (fork) seg_wait_join(s, seg);
   if (gh > 0) hipLaunchKernelGGL(k_heavy64, dim3(gh), dim3(64), 0, s, a, p, gl);
 }
 
-void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s) {
+void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {
   switch (a.rows.nother) {
-    case 1: return launch64_t<1>(a, p, s);
-    case 2: return launch64_t<2>(a, p, s);
-    case 3: return launch64_t<3>(a, p, s);
-    case 4: return launch64_t<4>(a, p, s);
+    case 1: return launch64_t<1>(a, p, s, seg);
+    case 2: return launch64_t<2>(a, p, s, seg);
+    case 3: return launch64_t<3>(a, p, s, seg);
+    case 4: return launch64_t<4>(a, p, s, seg);
     default: break;
   }
 }
