@@ -7,7 +7,7 @@
 //   u = L^{-T}(L^{-1} b + z),  P = L L^T.
 //
 // Work decomposition of one half-sweep (BatchPlan, built on the host):
-//   * rows with more than 1024 nonzeros are split into 1024-nnz segments;
+//   * rows with more than 2048 nonzeros are split into 2048-nnz segments;
 //     k_seg32 computes each segment's partial Gram and RHS (one wave per
 //     segment), k_seg_sum adds each row's partials in segment order (one
 //     thread per partial entry), and k_heavy32 finishes two rows per wave;

@@ -3,7 +3,7 @@
 //
 // Same mathematics and work decomposition as the K <= 32 sampler
 // (sampler_batched.hip): per row P = Lambda + alpha G, b = alpha c + Lambda mu,
-// u = L^{-T}(L^{-1} b + z); rows above 1024 nonzeros are split into segments
+// u = L^{-T}(L^{-1} b + z); rows above 2048 nonzeros are split into segments
 // whose partial Grams are summed in segment order; light rows are sorted by
 // nnz.  What changes with K = 64:
 //   * a lane gathers components l%32 and l%32 + 32 of its 8 nonzeros per

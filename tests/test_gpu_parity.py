@@ -411,8 +411,9 @@ TOL_BATCHED = 5e-3
 @pytest.mark.parametrize("K", [17, 24, 32, 33, 48, 64])
 def test_batched_sampler_matrix(K):
     announce("batched sampler matrix K=%d" % K)
-    # heavy rows above the 1024-nnz segment length, an empty row, odd row count
-    rng, modes, mean, V, Lam, mu = _matrix_case(K + 7, K, 301, 5000, 9000, heavy=(4000, 1025, 1024, 0, 3000))
+    # heavy rows above the 2048-nnz segment length, rows at the boundary, an
+    # empty row, odd row count
+    rng, modes, mean, V, Lam, mu = _matrix_case(K + 7, K, 301, 5000, 9000, heavy=(4000, 2049, 2048, 0, 3000))
     got = gpu_sample_latents(K, modes[0], mean, [V], 2.5, mu, Lam, SEED, 3, 0, BATCHED)
     ref = O.sample_latents(K, modes[0], mean, [V], 2.5, mu, Lam, SEED, 3, 0)
     err = row_rel_err(got, ref)
@@ -453,9 +454,9 @@ def test_batched_sampler_per_row_mean(K):
 @pytest.mark.parametrize("precision,K", [(5.0, 32), ("adaptive", 32), ("adaptive", 64)])
 def test_batched_model_matches_oracle(precision, K):
     announce("batched model precision=%s K=%d" % (precision, K))
-    nrow, ncol = 3000, 1500
+    nrow, ncol = 3000, 3000
     rng = np.random.default_rng(21)
-    r, c, y = power_law_rows(rng, nrow, ncol, 90000, heavy=(1400, 1300))
+    r, c, y = power_law_rows(rng, nrow, ncol, 90000, heavy=(2600, 2300))  # two heavy rows of two segments
     y = np.round(y * 2) / 2
     perm = rng.permutation(len(r))
     te, tr = perm[:len(r) // 10], perm[len(r) // 10:]
