@@ -72,65 +72,99 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
-        L = ctypes.CDLL(path)
-        L.orc_philox4x32_10.argtypes = [_p, _p, _p]
-        L.orc_normal.restype = _dbl
-        L.orc_normal.argtypes = [_u64, _u32, _u32, _u32, _u32]
-        L.orc_normals.argtypes = [_u64, _u32, _u32, _u32, _u32, _p]
-        L.orc_gamma.restype = _dbl
-        L.orc_gamma.argtypes = [_u64, _dbl, _u32, _u32, _u32]
-        L.orc_sample_latents.restype = _i64
-        L.orc_sample_latents.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p, _dbl, _p, _p, _p,
-                                         _u64, _u32, ctypes.c_int, _p]
-        L.orc_cond_normal_wishart.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _p, _dbl, _u64, _u32,
-                                              ctypes.c_int, _p, _p]
-        L.orc_normal_wishart_from_stats.argtypes = [ctypes.c_int, _i64, _p, _p, _p, _dbl, _p, _dbl, _u64,
-                                                    _u32, ctypes.c_int, _p, _p]
-        L.orc_train_sse.restype = _dbl
-        L.orc_train_sse.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p]
-        L.orc_noise_alpha.restype = _dbl
-        L.orc_noise_alpha.argtypes = [_dbl, _i64, _dbl, ctypes.c_int, _dbl, _u64, _u32]
-        L.orc_eval_rmse.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _dbl, _p, ctypes.c_int, _p,
-                                    _p, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
-        L.orc_csr_A_mul_Bn.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _p, _p]
-        L.orc_AtA_mul_B.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p, _p]
-        L.orc_blockcg.restype = ctypes.c_int
-        L.orc_blockcg.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p, _dbl, _p]
-        L.orc_blockcg_blocked.restype = ctypes.c_int
-        L.orc_blockcg_blocked.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p,
-                                          _dbl, ctypes.c_int, ctypes.c_int, _p]
-        L.orc_posterior_lambda_beta.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _dbl, ctypes.POINTER(_dbl),
-                                                ctypes.POINTER(_dbl)]
-        L.orc_at_mul_a.argtypes = [ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _p]
-        L.orc_ftf_solve.restype = ctypes.c_int
-        L.orc_ftf_solve.argtypes = [ctypes.c_int, _i64, _p, _dbl, _p, _p]
-        L.orc_norm_cdf.restype = _dbl
-        L.orc_norm_cdf.argtypes = [_dbl]
-        L.orc_inv_norm_cdf.restype = _dbl
-        L.orc_inv_norm_cdf.argtypes = [_dbl]
-        L.orc_truncnorm0.restype = _dbl
-        L.orc_truncnorm0.argtypes = [_dbl, _u64, _u32, _u32, _u32]
-        L.orc_probit_z.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _p, _p, _u64, _u32, ctypes.c_int, _p]
-        L.orc_auc.restype = _dbl
-        L.orc_auc.argtypes = [_i64, _p, _p]
-        L.orc_eval_probit.argtypes = [ctypes.c_int, _i64, _p, _p, _p, ctypes.c_int, _p, _p, _p,
-                                      ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
-        L.orc_sample_latents_uni.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p, _dbl, _p, _p, _u64,
-                                             _u32, ctypes.c_int, _p]
-        L.orc_at_mul_bt_col.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _i64, _p, _p]
-        L.orc_add_acol_mul_bt.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _i64, _p, _p]
-        L.orc_beta_sweep_uni.argtypes = [ctypes.c_int, _i64, ctypes.POINTER(OrcCsr), _p, _p, _p, _p, _p, _p, _p,
-                                         _u64, _u32, ctypes.c_int]
-        L.orc_lambda_beta_uni.argtypes = [ctypes.c_int, _i64, _p, _u64, _u32, ctypes.c_int, _p]
-        L.orc_macau_one_update_prior.restype = ctypes.c_int
-        L.orc_macau_one_update_prior.argtypes = [ctypes.c_int, ctypes.c_int, _p, _i64, ctypes.POINTER(OrcCsr),
-                                                 ctypes.POINTER(OrcCsr), _p, _p, _p, _p, _p, _p, _u64, _u32]
-        L.orc_run.restype = ctypes.c_int
-        L.orc_run.argtypes = [ctypes.POINTER(OrcRunCfg)]
-        L.orc_set_threads.argtypes = [ctypes.c_int]
-        L.orc_get_threads.restype = ctypes.c_int
-        _LIB = L
+        _LIB = _load(path)
     return _LIB
+
+
+class use_library:
+    """Context manager: route every oracle call through the library at
+    ``path`` (a variant built from oracle.c with a planted mistake, used by
+    the tests that show the pins catch such mistakes)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _LIB
+        self.saved = lib()
+        _LIB = _load(self.path)
+        return _LIB
+
+    def __exit__(self, *exc):
+        global _LIB
+        _LIB = self.saved
+        return False
+
+
+def _load(path):
+    """Load an oracle library and declare its entry points."""
+    L = ctypes.CDLL(path)
+    L.orc_philox4x32_10.argtypes = [_p, _p, _p]
+    L.orc_normal.restype = _dbl
+    L.orc_normal.argtypes = [_u64, _u32, _u32, _u32, _u32]
+    L.orc_normals.argtypes = [_u64, _u32, _u32, _u32, _u32, _p]
+    L.orc_gamma.restype = _dbl
+    L.orc_gamma.argtypes = [_u64, _dbl, _u32, _u32, _u32]
+    L.orc_sample_latents.restype = _i64
+    L.orc_sample_latents.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p, _dbl, _p, _p, _p,
+                                     _u64, _u32, ctypes.c_int, _p]
+    L.orc_cond_normal_wishart.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _p, _dbl, _u64, _u32,
+                                          ctypes.c_int, _p, _p]
+    L.orc_normal_wishart_from_stats.argtypes = [ctypes.c_int, _i64, _p, _p, _p, _dbl, _p, _dbl, _u64,
+                                                _u32, ctypes.c_int, _p, _p]
+    L.orc_train_sse.restype = _dbl
+    L.orc_train_sse.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p]
+    L.orc_noise_alpha.restype = _dbl
+    L.orc_noise_alpha.argtypes = [_dbl, _i64, _dbl, ctypes.c_int, _dbl, _u64, _u32]
+    L.orc_eval_rmse.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _dbl, _p, ctypes.c_int, _p,
+                                _p, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
+    L.orc_csr_A_mul_Bn.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _p, _p]
+    L.orc_AtA_mul_B.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p, _p]
+    L.orc_blockcg.restype = ctypes.c_int
+    L.orc_blockcg.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p, _dbl, _p]
+    L.orc_blockcg_blocked.restype = ctypes.c_int
+    L.orc_blockcg_blocked.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _dbl, _p,
+                                      _dbl, ctypes.c_int, ctypes.c_int, _p]
+    L.orc_posterior_lambda_beta.argtypes = [ctypes.c_int, _i64, _p, _p, _dbl, _dbl, ctypes.POINTER(_dbl),
+                                            ctypes.POINTER(_dbl)]
+    L.orc_at_mul_a.argtypes = [ctypes.POINTER(OrcCsr), ctypes.POINTER(OrcCsr), _p]
+    L.orc_ftf_solve.restype = ctypes.c_int
+    L.orc_ftf_solve.argtypes = [ctypes.c_int, _i64, _p, _dbl, _p, _p]
+    L.orc_norm_cdf.restype = _dbl
+    L.orc_norm_cdf.argtypes = [_dbl]
+    L.orc_inv_norm_cdf.restype = _dbl
+    L.orc_inv_norm_cdf.argtypes = [_dbl]
+    L.orc_truncnorm0.restype = _dbl
+    L.orc_truncnorm0.argtypes = [_dbl, _u64, _u32, _u32, _u32]
+    L.orc_probit_z.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _p, _p, _u64, _u32, ctypes.c_int, _p]
+    L.orc_auc.restype = _dbl
+    L.orc_auc.argtypes = [_i64, _p, _p]
+    L.orc_eval_probit.argtypes = [ctypes.c_int, _i64, _p, _p, _p, ctypes.c_int, _p, _p, _p,
+                                  ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]
+    L.orc_sample_latents_uni.argtypes = [ctypes.c_int, ctypes.POINTER(OrcRows), _dbl, _p, _dbl, _p, _p, _u64,
+                                         _u32, ctypes.c_int, _p]
+    L.orc_at_mul_bt_col.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _i64, _p, _p]
+    L.orc_add_acol_mul_bt.argtypes = [ctypes.c_int, ctypes.POINTER(OrcCsr), _i64, _p, _p]
+    L.orc_beta_sweep_uni.argtypes = [ctypes.c_int, _i64, ctypes.POINTER(OrcCsr), _p, _p, _p, _p, _p, _p, _p,
+                                     _u64, _u32, ctypes.c_int]
+    L.orc_lambda_beta_uni.argtypes = [ctypes.c_int, _i64, _p, _u64, _u32, ctypes.c_int, _p]
+    L.orc_macau_one_update_prior.restype = ctypes.c_int
+    L.orc_macau_one_update_prior.argtypes = [ctypes.c_int, ctypes.c_int, _p, _i64, ctypes.POINTER(OrcCsr),
+                                             ctypes.POINTER(OrcCsr), _p, _p, _p, _p, _p, _p, _u64, _u32]
+    L.orc_run.restype = ctypes.c_int
+    L.orc_run.argtypes = [ctypes.POINTER(OrcRunCfg)]
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_get_threads.restype = ctypes.c_int
+    L.orc_sample_beta.restype = ctypes.c_int
+    L.orc_sample_beta.argtypes = [ctypes.c_int, ctypes.c_int, _p, _i64, _p, _p, ctypes.POINTER(OrcCsr),
+                                  ctypes.POINTER(OrcCsr), _p, _dbl, _dbl, _u64, _u32, _p,
+                                  ctypes.POINTER(ctypes.c_int)]
+    L.orc_macau_update_prior.restype = ctypes.c_int
+    L.orc_macau_update_prior.argtypes = [ctypes.c_int, ctypes.c_int, _p, _i64, ctypes.POINTER(OrcCsr),
+                                         ctypes.POINTER(OrcCsr), _p, _p, ctypes.c_int, _p, _p,
+                                         ctypes.POINTER(_dbl), _dbl, _p, _p, _u64, _u32,
+                                         ctypes.POINTER(ctypes.c_int)]
+    return L
 
 
 def _ptr(a):
@@ -477,6 +511,48 @@ def ftf_solve(FtF, reg, B):
     if lib().orc_ftf_solve(B.shape[1], B.shape[0], _ptr(FtF), reg, _ptr(B), _ptr(X)):
         raise RuntimeError("Cholesky decomp failed")
     return X
+
+
+def sample_beta(U, mu, Lambda, F, Ft, lambda_beta, seed, it, mode=0, FtF=None, tol=1e-6):
+    """MacauPrior::sample_beta (latentprior.cpp:272-293).  U is (N, K)
+    C-order (K x N column-major); returns (beta as (nfeat, K) C-order, CG
+    iteration count).  FtF (lower triangle, from at_mul_a) selects the F^T F
+    branch."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N, K = U.shape
+    beta = np.zeros((F.ncol, K))
+    it_out = ctypes.c_int(0)
+    ftf = None if FtF is None else np.asfortranarray(FtF, dtype=np.float64)
+    rc = lib().orc_sample_beta(K, mode, _ptr(U), N, _ptr(np.ascontiguousarray(mu, dtype=np.float64)),
+                               _ptr(np.asfortranarray(Lambda, dtype=np.float64)), ctypes.byref(F.c()),
+                               ctypes.byref(Ft.c()), _ptr(ftf), lambda_beta, tol, seed, it, _ptr(beta),
+                               ctypes.byref(it_out))
+    if rc:
+        raise RuntimeError("sample_beta failed (%d)" % rc)
+    return beta, it_out.value
+
+
+def macau_update_prior(U, F, Ft, beta, Uhat, lambda_beta, seed, it, mode=0, FtF=None, tol=1e-6):
+    """MacauPrior::update_prior (latentprior.cpp:254-264) with the prior
+    constants of MacauPrior::init (WI = I, df = K; latentprior.cpp:189-195).
+    U, Uhat are (N, K), beta (nfeat, K), all C-order.  Returns
+    (mu, Lambda, beta, Uhat, lambda_beta, cg_iters) after the update."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N, K = U.shape
+    beta = np.array(beta, dtype=np.float64, order="C", copy=True)
+    Uhat = np.array(Uhat, dtype=np.float64, order="C", copy=True)
+    WI = np.eye(K, order="F")
+    lb = _dbl(lambda_beta)
+    mu = np.zeros(K)
+    Lam = np.zeros((K, K), order="F")
+    it_out = ctypes.c_int(0)
+    ftf = None if FtF is None else np.asfortranarray(FtF, dtype=np.float64)
+    rc = lib().orc_macau_update_prior(K, mode, _ptr(U), N, ctypes.byref(F.c()), ctypes.byref(Ft.c()), _ptr(ftf),
+                                      _ptr(WI), K, _ptr(beta), _ptr(Uhat), ctypes.byref(lb), tol, _ptr(mu),
+                                      _ptr(Lam), seed, it, ctypes.byref(it_out))
+    if rc:
+        raise RuntimeError("MacauPrior update failed (%d)" % rc)
+    return mu, np.array(Lam), beta, Uhat, lb.value, it_out.value
 
 
 def posterior_lambda_beta(beta, Lambda, nu, mu):

@@ -16,6 +16,14 @@
 #include <omp.h>
 #endif
 
+/* ORC_MUTANT = n > 0 builds the oracle with planted mistake n (see the
+ * "planted mistake" switches below).  Only tests/test_oracle_macau.py builds
+ * such variants, into a temporary directory, to show that its pins fail on
+ * them; liboracle.so is always built with ORC_MUTANT = 0. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random  */
 /* numbers: as easy as 1, 2, 3").  Replaces the reference's mt19937    */
@@ -623,7 +631,15 @@ static void mvnormal_prec_cols(int K, const double* Lchol, int64_t n, uint64_t s
   for (int64_t c = 0; c < n; c++) {
     double* o = out + (size_t)c * K;
     orc_normals(seed, (uint32_t)c, (uint32_t)K, it, tag, o);
+#if ORC_MUTANT == 3 /* planted mistake (tests only): covariance Lambda instead of Lambda^{-1} */
+    for (int i = K - 1; i >= 0; i--) {
+      double t = 0.0;
+      for (int k = 0; k <= i; k++) t += Lchol[i + (size_t)k * K] * o[k];
+      o[i] = t;
+    }
+#else
     orc_trsv_upper_t(K, Lchol, o);
+#endif
   }
 }
 
@@ -668,48 +684,85 @@ int orc_ftf_solve(int nrhs, int64_t n, const double* FtF, double reg, const doub
   return 0;
 }
 
-/* MacauPrior::update_prior (latentprior.cpp:254-264) with sample_beta
- * (:272-293); FtF (non-null) selects the F^T F branch, else the block CG */
-static int macau_update_prior_t(int K, int mode, const double* U, int64_t N, const orc_csr* F,
-                                const orc_csr* Ft, const double* FtF, const double* WI, int df, double* beta,
-                                double* Uhat, double* lambda_beta, double tol, double* mu, double* Lambda,
-                                uint64_t seed, uint32_t it, int* cg_iters) {
+/* MacauPrior::sample_beta (latentprior.cpp:272-293): draws the link matrix
+ * beta (K x nfeat) at fixed U, mu, Lambda and lambda_beta.  FtF (non-null)
+ * selects the F^T F branch, else the block CG.  Returns 0, -10 when Lambda is
+ * not positive definite, -11 when F^T F + lambda_beta I is not. */
+int orc_sample_beta(int K, int mode, const double* U, int64_t N, const double* mu, const double* Lambda,
+                    const orc_csr* F, const orc_csr* Ft, const double* FtF, double lambda_beta, double tol,
+                    uint64_t seed, uint32_t it, double* beta, int* cg_iters) {
   const int64_t nfeat = F->ncol;
   const size_t KK = (size_t)K * K;
   int rc = 0;
-  for (size_t q = 0; q < (size_t)K * N; q++) Uhat[q] = U[q] - Uhat[q];
-  double* BBt = (double*)malloc(sizeof(double) * KK);
-  double* T = (double*)malloc(sizeof(double) * KK);
-  double* mu0 = (double*)calloc(K, sizeof(double));
   double* Lc = (double*)malloc(sizeof(double) * KK);
   double* tmp = (double*)malloc(sizeof(double) * (size_t)K * N);
   double* Fty = (double*)malloc(sizeof(double) * (size_t)K * nfeat);
   double* noise = (double*)malloc(sizeof(double) * (size_t)K * nfeat);
-  rows_dot(K, nfeat, beta, beta, BBt);
-  for (size_t q = 0; q < KK; q++) T[q] = WI[q] + (*lambda_beta) * BBt[q];
-  rc = orc_cond_normal_wishart(K, N, Uhat, mu0, 2.0, T, (double)df + (double)nfeat, seed, it, mode, mu, Lambda);
-  if (rc) goto out;
   memcpy(Lc, Lambda, sizeof(double) * KK);
   if (orc_chol_lower(K, Lc)) { rc = -10; goto out; }
   /* tmp = U + MvNormal_prec_omp(Lambda, N) - mu (latentprior.cpp:277) */
   mvnormal_prec_cols(K, Lc, N, seed, it, ORC_TAG(ORC_TAG_BETA_U, mode), tmp);
   for (int64_t n = 0; n < N; n++)
-    for (int k = 0; k < K; k++) tmp[(size_t)n * K + k] += U[(size_t)n * K + k] - mu[k];
+    for (int k = 0; k < K; k++) {
+#if ORC_MUTANT == 2 /* planted mistake (tests only): the mean is not subtracted */
+      tmp[(size_t)n * K + k] += U[(size_t)n * K + k];
+#else
+      tmp[(size_t)n * K + k] += U[(size_t)n * K + k] - mu[k];
+#endif
+    }
   /* Ft_y = tmp F + sqrt(lambda_beta) MvNormal_prec_omp(Lambda, nfeat) (:278) */
   orc_csr_A_mul_Bn(K, Ft, tmp, Fty);
   mvnormal_prec_cols(K, Lc, nfeat, seed, it, ORC_TAG(ORC_TAG_BETA_F, mode), noise);
   {
-    const double sl = sqrt(*lambda_beta);
+#if ORC_MUTANT == 1 /* planted mistake (tests only): sqrt(lambda_beta) dropped */
+    const double sl = 1.0;
+#else
+    const double sl = sqrt(lambda_beta);
+#endif
     for (size_t q = 0; q < (size_t)K * nfeat; q++) Fty[q] += sl * noise[q];
   }
   if (FtF) {
     /* K = FtF + lambda_beta I, chol_decomp, chol_solve_t (:280-288) */
-    if (orc_ftf_solve(K, nfeat, FtF, *lambda_beta, Fty, beta)) { rc = -11; goto out; }
+    if (orc_ftf_solve(K, nfeat, FtF, lambda_beta, Fty, beta)) { rc = -11; goto out; }
     *cg_iters = 0;
   } else {
     /* solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (:291) */
-    *cg_iters = orc_blockcg_blocked(K, F, Ft, *lambda_beta, Fty, tol, 32, 8, beta);
+    *cg_iters = orc_blockcg_blocked(K, F, Ft, lambda_beta, Fty, tol, 32, 8, beta);
   }
+out:
+  free(Lc); free(tmp); free(Fty); free(noise);
+  return rc;
+}
+
+/* MacauPrior::update_prior (latentprior.cpp:254-264): the residual U - Uhat,
+ * the Normal-Wishart draw with WI + lambda_beta beta beta^T and df + nfeat,
+ * sample_beta with the new (mu, Lambda), Uhat = beta F^T and lambda_beta from
+ * its gamma posterior.  Updates mu, Lambda, beta, Uhat and *lambda_beta. */
+int orc_macau_update_prior(int K, int mode, const double* U, int64_t N, const orc_csr* F, const orc_csr* Ft,
+                           const double* FtF, const double* WI, int df, double* beta, double* Uhat,
+                           double* lambda_beta, double tol, double* mu, double* Lambda, uint64_t seed, uint32_t it,
+                           int* cg_iters) {
+  const int64_t nfeat = F->ncol;
+  const size_t KK = (size_t)K * K;
+  int rc = 0;
+#if ORC_MUTANT == 6 /* planted mistake (tests only): the prior sees U, not the residual U - Uhat */
+  for (size_t q = 0; q < (size_t)K * N; q++) Uhat[q] = U[q];
+#else
+  for (size_t q = 0; q < (size_t)K * N; q++) Uhat[q] = U[q] - Uhat[q];
+#endif
+  double* BBt = (double*)malloc(sizeof(double) * KK);
+  double* T = (double*)malloc(sizeof(double) * KK);
+  double* mu0 = (double*)calloc(K, sizeof(double));
+  rows_dot(K, nfeat, beta, beta, BBt);
+#if ORC_MUTANT == 4 /* planted mistake (tests only): WI + beta beta^T without lambda_beta */
+  for (size_t q = 0; q < KK; q++) T[q] = WI[q] + BBt[q];
+#else
+  for (size_t q = 0; q < KK; q++) T[q] = WI[q] + (*lambda_beta) * BBt[q];
+#endif
+  rc = orc_cond_normal_wishart(K, N, Uhat, mu0, 2.0, T, (double)df + (double)nfeat, seed, it, mode, mu, Lambda);
+  if (rc) goto out;
+  rc = orc_sample_beta(K, mode, U, N, mu, Lambda, F, Ft, FtF, *lambda_beta, tol, seed, it, beta, cg_iters);
+  if (rc) goto out;
   /* compute_uhat: Uhat = beta F^T (linop.h:196-198) */
   orc_csr_A_mul_Bn(K, F, beta, Uhat);
   {
@@ -718,7 +771,7 @@ static int macau_update_prior_t(int K, int mode, const double* U, int64_t N, con
     *lambda_beta = orc_gamma(seed, shape, 0, it, ORC_TAG(ORC_TAG_LAMBDA_BETA, mode)) * scale;
   }
 out:
-  free(BBt); free(T); free(mu0); free(Lc); free(tmp); free(Fty); free(noise);
+  free(BBt); free(T); free(mu0);
   return rc;
 }
 
@@ -1103,6 +1156,15 @@ int orc_run(orc_run_cfg* c) {
   int total = c->burnin + c->nsamples;
   if (c->max_iters > 0 && c->max_iters < total) total = c->max_iters;
   for (int i = 0; i < total; i++) {
+#if ORC_MUTANT == 5
+    for (int m = 0; m < M; m++) {
+      double* mu0 = (double*)calloc(K, sizeof(double));
+      if (!c->macau[m])
+        rc = orc_cond_normal_wishart(K, c->dims[m], U[m], mu0, 2.0, WI, (double)K, c->seed, (uint32_t)i, m, mu[m],
+                                     Lambda[m]);
+      free(mu0);
+    }
+#endif
     for (int m = 0; m < M; m++) {
       const double* others[8];
       int t = 0;
@@ -1128,15 +1190,26 @@ int orc_run(orc_run_cfg* c) {
                                c->macau[m] ? murows[m] : NULL, c->seed, (uint32_t)i, m, U[m]);
       }
       if (f) { rc = -1; goto done; }
+#if ORC_MUTANT == 7 /* planted mistake (tests only): noise precision updated between the modes */
+      if (m == 0 && c->noise_adaptive && !c->noise_probit) {
+        const double* Us[8];
+        for (int q = 0; q < M; q++) Us[q] = U[q];
+        alpha = orc_noise_alpha(orc_train_sse(K, M, &c->train[0], c->mean, Us), c->nnz, var_total, c->is_tensor,
+                                alpha_max, c->seed, (uint32_t)i);
+      }
+#endif
     }
     for (int m = 0; m < M; m++) {
+#if ORC_MUTANT == 5 /* planted mistake (tests only): the priors updated before the modes are sampled */
+      if (!c->macau[m]) continue;
+#endif
       if (c->macau[m] && c->uni[m]) {
         rc = orc_macau_one_update_prior(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], colsq[m], mu[m], lam_uni[m],
                                         beta[m], Uhat[m], lb_uni[m], c->seed, (uint32_t)i);
       } else if (c->macau[m]) {
         int cgit = 0;
-        rc = macau_update_prior_t(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], FtF[m], WI, K, beta[m], Uhat[m],
-                                  &lambda_beta[m], c->tol, mu[m], Lambda[m], c->seed, (uint32_t)i, &cgit);
+        rc = orc_macau_update_prior(K, m, U[m], c->dims[m], &c->F[m], &c->Ft[m], FtF[m], WI, K, beta[m], Uhat[m],
+                                    &lambda_beta[m], c->tol, mu[m], Lambda[m], c->seed, (uint32_t)i, &cgit);
         c->cg_iters_last = cgit;
       } else {
         double* mu0 = (double*)calloc(K, sizeof(double));
@@ -1146,7 +1219,7 @@ int orc_run(orc_run_cfg* c) {
       }
       if (rc) { rc = -2; goto done; }
     }
-    if (c->noise_adaptive && !c->noise_probit) {
+    if (c->noise_adaptive && !c->noise_probit && ORC_MUTANT != 7) {
       const double* Us[8];
       for (int m = 0; m < M; m++) Us[m] = U[m];
       double sse = orc_train_sse(K, M, &c->train[0], c->mean, Us);

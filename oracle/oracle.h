@@ -129,6 +129,23 @@ void orc_at_mul_a(const orc_csr* A, const orc_csr* At, double* out);
  * FtF (lower) + reg I and chol_solve_t (chol.cpp:47-53).  Returns -1 when
  * the matrix is not positive definite (chol_decomp throws). */
 int orc_ftf_solve(int nrhs, int64_t n, const double* FtF, double reg, const double* B, double* X);
+/* MacauPrior::sample_beta (latentprior.cpp:272-293): at fixed U (K x N),
+ * mu, Lambda and lambda_beta, beta = Ft_y (F^T F + lambda_beta I)^{-1} with
+ * Ft_y = (U + N(0, Lambda^{-1}) - mu) F + sqrt(lambda_beta) N(0, Lambda^{-1});
+ * FtF (lower triangle) selects the F^T F branch, NULL the block CG
+ * (blocks of 32 + 8).  Returns 0, -10 (Lambda not positive definite) or -11
+ * (F^T F + lambda_beta I not positive definite). */
+int orc_sample_beta(int K, int mode, const double* U, int64_t N, const double* mu, const double* Lambda,
+                    const orc_csr* F, const orc_csr* Ft, const double* FtF, double lambda_beta, double tol,
+                    uint64_t seed, uint32_t it, double* beta, int* cg_iters);
+/* MacauPrior::update_prior (latentprior.cpp:254-264): Uhat <- U - Uhat;
+ * (mu, Lambda) = CondNormalWishart(Uhat, 0, 2, WI + lambda_beta beta beta^T,
+ * df + nfeat); sample_beta; Uhat = beta F^T; lambda_beta from
+ * posterior_lambda_beta(beta, Lambda, 1e-3, 1) (latentprior.cpp:317-331). */
+int orc_macau_update_prior(int K, int mode, const double* U, int64_t N, const orc_csr* F, const orc_csr* Ft,
+                           const double* FtF, const double* WI, int df, double* beta, double* Uhat,
+                           double* lambda_beta, double tol, double* mu, double* Lambda, uint64_t seed, uint32_t it,
+                           int* cg_iters);
 
 /* ---- probit noise (latentprior.cpp:387-414, truncnorm.cpp, noisemodels.cpp:129-159) ---- */
 /* Phi(x) = 0.5 erfc(-x / sqrt 2) (truncnorm.cpp:5-7) */
