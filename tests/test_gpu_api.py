@@ -262,3 +262,124 @@ def test_blockcg_python():  # macau.pyx:67-91
     B = np.asfortranarray(rng.standard_normal((3, 25)))
     out = macau_amd.blockcg(X, B, 0.5, tol=1e-10)
     assert out.shape == (2,) and out[0] >= 1
+
+
+# ---------------------------------------------------------------- values and order
+def _oracle_compare(res, ti, tv, ref, nsamples, colnames, tol=1e-4):
+    """res.prediction in getTestData order (index columns and y exact), y_pred
+    and y_pred_std against the oracle run on the same data and seed."""
+    import oracle as O  # noqa: F401  (the checker)
+    pred = res.prediction
+    assert list(pred.columns) == list(colnames) + ["y", "y_pred", "y_pred_std"]
+    np.testing.assert_array_equal(pred[list(colnames)].to_numpy(), ti)
+    np.testing.assert_array_equal(pred["y"].to_numpy(), tv)
+    yp = pred["y_pred"].to_numpy()
+    err = np.linalg.norm(yp - ref.pred) / np.linalg.norm(ref.pred)
+    sd_ref = np.sqrt(ref.pred_var / (nsamples - 1))
+    err_sd = np.linalg.norm(pred["y_pred_std"].to_numpy() - sd_ref) / np.linalg.norm(sd_ref)
+    print("api y_pred rel err %.3e, y_pred_std rel err %.3e, rmse %.6f vs %.6f" % (err, err_sd, res.rmse_test,
+                                                                                  ref.rmse))
+    assert err < tol and err_sd < 10 * tol
+    assert abs(res.rmse_test - ref.rmse) <= tol * ref.rmse
+
+
+@pytest.mark.parametrize("sampler", ["plain", "batched"])
+def test_api_matrix_predictions_match_oracle(sampler):
+    """bpmf() on a sparse matrix: prediction rows in the CSC order of Ytest
+    (sparsetensor.cpp:63-76), duplicates summed, values equal to the oracle's
+    posterior means (bpmfutils.h:86-114) and stds (macau.cpp:175-188)."""
+    import oracle as O
+    announce("api matrix predictions vs oracle sampler=%s" % sampler)
+    rng = np.random.default_rng(31)
+    nrow, ncol, K = 400, 250, 32 if sampler == "batched" else 8
+    r, c = np.nonzero(rng.random((nrow, ncol)) < 0.08)
+    y = np.round(rng.standard_normal(len(r)) * 2) / 2 + 3.0
+    perm = rng.permutation(len(r))
+    te, tr = perm[:900], perm[900:]
+    # the test matrix lists cells in shuffled order, one of them twice
+    te = np.concatenate([te, te[:1]])
+    Ytr = scipy.sparse.coo_matrix((y[tr], (r[tr], c[tr])), shape=(nrow, ncol))
+    Yte = scipy.sparse.coo_matrix((y[te], (r[te], c[te])), shape=(nrow, ncol))
+    burnin, nsamples = 2, 3
+    res = macau_amd.bpmf(Ytr, Ytest=Yte, num_latent=K, precision=4.0, burnin=burnin, nsamples=nsamples,
+                         verbose=False, seed=33, sampler=sampler)
+    modes, mean, nnz = O.matrix_rows(r[tr], c[tr], y[tr], nrow, ncol)
+    ti, tv = O.matrix_test(r[te], c[te], y[te], nrow, ncol)
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, precision=4.0, burnin=burnin, nsamples=nsamples, seed=33)
+    # plain FP32: 1e-4; split-BF16 Gram: 1e-3
+    _oracle_compare(res, ti, tv, ref, nsamples, ["row", "col"], 1e-4 if sampler == "plain" else 1e-3)
+
+
+def test_api_tensor_predictions_match_oracle():
+    """macau() on a 3-mode DataFrame whose largest mode is mode 1: prediction
+    rows in the mode-0 order of SparseMode (sparsetensor.cpp:178-191), values
+    equal to the oracle's (eval_rmse_tensor, bpmfutils.cpp:10-73)."""
+    import oracle as O
+    announce("api tensor predictions vs oracle")
+    rng = np.random.default_rng(41)
+    dims = (30, 160, 7)
+    n = 6000
+    idx = np.stack([rng.integers(0, d, n) for d in dims], 1)
+    idx = np.unique(idx, axis=0)
+    idx = idx[rng.permutation(len(idx))]
+    vals = rng.standard_normal(len(idx))
+    te, tr = np.arange(700), np.arange(700, len(idx))
+    df = pd.DataFrame(idx, columns=["A", "B", "C"])
+    df["value"] = vals
+    Ytr, Yte = df.iloc[tr], df.iloc[te]
+    # the training frame must span every index so both sides see the same dims
+    assert all(Ytr[c].max() + 1 == d for c, d in zip("ABC", dims))
+    K, burnin, nsamples = 6, 2, 3
+    res = macau_amd.bpmf(Ytr, Ytest=Yte, num_latent=K, precision=3.0, burnin=burnin, nsamples=nsamples,
+                         verbose=False, seed=43)
+    modes, mean, nnz = O.tensor_rows(idx[tr], vals[tr], list(dims))
+    ti, tv = O.tensor_test(idx[te], vals[te], list(dims))
+    ref = O.run(modes, mean, nnz, list(dims), K, ti, tv, is_tensor=True, precision=3.0, burnin=burnin,
+                nsamples=nsamples, seed=43)
+    _oracle_compare(res, ti, tv, ref, nsamples, ["A", "B", "C"])
+
+
+def test_save_model_contents(tmp_path):
+    """Model files (macau.cpp:191-205, latentprior.cpp:307-315) hold the
+    state the C ABI reports, written as writeToCSVfile does (bpmfutils.h:169-173:
+    6 significant digits, "," between columns, "\\n" between rows)."""
+    from _gpu_helpers import GpuModel
+    from macau_amd import _lib
+    from macau_amd._lib import check, ptr
+    announce("api save_prefix file contents")
+    rng = np.random.default_rng(51)
+    nrow, ncol, K, nfeat = 40, 30, 5, 12
+    r, c = np.nonzero(rng.random((nrow, ncol)) < 0.3)
+    y = rng.standard_normal(len(r)) + 2.0
+    fr, fc = np.nonzero(rng.random((nrow, nfeat)) < 0.3)
+    m = GpuModel(2, K, 3.0)
+    m.data(np.stack([r, c], 1), y, [nrow, ncol])
+    m.prior(side=(fr, fc, None, nrow, nfeat), lambda_beta=5.0, tol=1e-6)
+    m.prior()
+    L = _lib.lib()
+    prefix = str(tmp_path / "s")
+    check(L.macau_set_samples(m.h, 1, 2))
+    check(L.macau_set_seed(m.h, 52))
+    check(L.macau_set_verbose(m.h, 0))
+    check(L.macau_set_save_model(m.h, 1))
+    check(L.macau_set_save_prefix(m.h, prefix.encode()))
+    check(L.macau_run(m.h))
+
+    def text(name):
+        with open(prefix + name) as f:
+            return f.read()
+
+    def expect(mat):
+        return "\n".join(",".join("%.6g" % v for v in row) for row in np.atleast_2d(mat))
+
+    assert text("-meanvalue.csv") == "%.6g" % L.macau_get_mean_value(m.h)
+    assert abs(L.macau_get_mean_value(m.h) - y.mean()) < 1e-12
+    for mode, n in ((0, nrow), (1, ncol)):
+        lat = m.latents(mode, n)  # (n, K): the file is K rows x n columns
+        assert text("-sample2-U%d-latents.csv" % (mode + 1)) == expect(lat.T)
+        mu, _ = m.hyper(mode)
+        assert text("-sample2-U%d-latentmean.csv" % (mode + 1)) == expect(mu.reshape(K, 1))
+    beta = m.link(0, nfeat)  # (nfeat, K): the file is K x nfeat
+    assert text("-sample2-U1-link.csv") == expect(beta.T)
+    assert not os.path.exists(prefix + "-sample2-U2-link.csv")
+    assert os.path.exists(prefix + "-sample1-U1-link.csv")

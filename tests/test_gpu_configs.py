@@ -1,0 +1,200 @@
+"""Parity with the FP64 oracle at the benchmark configurations' full sizes.
+
+* C1 (5000 x 2000 BPMF, 400k train cells, K = 10, fixed precision 5, burnin
+  20, nsamples 50): the whole chain; every latent row within 1e-4 of the
+  oracle after iterations 1 and 2, posterior-mean test RMSE of the 70-iteration
+  chain within 0.5%.
+* C2 (138000 x 27000, 18M train cells, K = 32, adaptive noise): state taken
+  from a few iterations of the batched chain, then one half-sweep of each mode
+  through the C ABI with both samplers, compared on the 50 heaviest rows (the
+  heaviest of 16786 nonzeros, 9 segments) and 1950 (rows) / 950 (columns)
+  random rows.
+* C3 (15000 x 100000 binary row features, ~1.5M nonzeros): one block-CG
+  solve with 32 right-hand sides at tolerance 1e-6, within 1e-6 of the oracle
+  and within 2 iterations of its count.
+* C4 (50000 x 2000 x 100 tensor, 9M train cells, K = 64): one half-sweep of
+  every mode with both samplers on a sample of rows, including mode 2's rows of
+  ~90000 nonzeros.
+* Rows of more than 32768 nonzeros (16 or more heavy-row segments) at K = 32
+  and K = 64.
+
+The oracle samples only the compared rows: the other rows are emptied in its
+input, which leaves each compared row's inputs and draws (keyed by the row
+index) unchanged.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from _gpu_helpers import (SEED, GpuModel, announce, f32, gpu_blockcg, gpu_sample_latents, row_rel_err, spd)
+from macau_amd import _lib, synthetic
+
+pytestmark = pytest.mark.gpu
+
+PLAIN, BATCHED = _lib.SAMPLER_PLAIN_FP32, _lib.SAMPLER_BATCHED
+TOL_FP32 = 1e-4
+# split-BF16 Gram with FP32 accumulation: the bound is 10x the largest row
+# error measured at these configurations (printed by each test)
+TOL_BATCHED = 1e-3
+
+
+def keep_rows(rows, keep):
+    """The rows of `rows` with only the rows in `keep` non-empty."""
+    n = rows.nrows
+    mask = np.zeros(n, dtype=bool)
+    mask[keep] = True
+    deg = np.diff(rows.row_ptr)
+    sel = np.repeat(mask, deg)
+    rp = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.where(mask, deg, 0), out=rp[1:])
+    return O.Rows(rp, rows.idx[sel], rows.vals[sel])
+
+
+def pick_rows(rows, nheavy, nrand, rng):
+    deg = np.diff(rows.row_ptr)
+    heavy = np.argsort(-deg, kind="stable")[:nheavy]
+    rest = np.setdiff1d(np.arange(rows.nrows), heavy)
+    return np.sort(np.concatenate([heavy, rng.choice(rest, min(nrand, len(rest)), replace=False)]))
+
+
+def compare_half_sweep(name, K, rows, mean, others, alpha, mu, Lam, it, mode, keep, samplers=(PLAIN, BATCHED),
+                       mu_rows=None):
+    sub = keep_rows(rows, keep)
+    ref = O.sample_latents(K, sub, mean, others, alpha, mu, Lam, SEED, it, mode, mu_rows=mu_rows)[keep]
+    deg = np.diff(rows.row_ptr)[keep]
+    errs = {}
+    for s in samplers:
+        announce("%s mode %d sampler %d" % (name, mode, s))
+        got = gpu_sample_latents(K, rows, mean, others, alpha, mu, Lam, SEED, it, mode, s, mu_rows=mu_rows)[keep]
+        per_row = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        errs[s] = float(per_row.max())
+        hv = deg >= 2048
+        print("%s mode %d sampler %d: max row rel err %.3e (rows of >= 2048 nnz: %.3e, heaviest %d nnz)"
+              % (name, mode, s, errs[s], per_row[hv].max() if hv.any() else 0.0, deg.max()))
+    if PLAIN in errs:
+        assert errs[PLAIN] < TOL_FP32
+    if BATCHED in errs:
+        assert errs[BATCHED] < TOL_BATCHED
+    return errs
+
+
+# ---------------------------------------------------------------- C1
+@pytest.mark.timeout(420)
+def test_c1_full_chain():
+    announce("C1 full chain")
+    (tr_r, tr_c, tr_v), (te_r, te_c, te_v), (nrow, ncol), st = synthetic.c1()
+    K = st["num_latent"]
+    modes, mean, nnz = O.matrix_rows(tr_r, tr_c, tr_v, nrow, ncol)
+    ti, tv = O.matrix_test(te_r, te_c, te_v, nrow, ncol)
+    kw = dict(precision=st["precision"], burnin=st["burnin"], nsamples=st["nsamples"], seed=SEED)
+    samplers = [PLAIN] + ([BATCHED] if _lib.lib().macau_sampler_supported(BATCHED, K, 2) else [])
+    ref_full = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, **kw)
+    for s in samplers:
+        m = GpuModel(2, K, st["precision"])
+        m.data(np.stack([tr_r, tr_c], 1), tr_v, [nrow, ncol], np.stack([te_r, te_c], 1), te_v)
+        m.prior()
+        m.prior()
+        m.setup(st["burnin"], st["nsamples"], SEED, s)
+        tol = TOL_FP32 if s == PLAIN else TOL_BATCHED
+        for it in (1, 2):
+            m.iterate(1)
+            ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, max_iters=it, want_U=True, **kw)
+            e0, e1 = row_rel_err(m.latents(0, nrow), ref.U[0]), row_rel_err(m.latents(1, ncol), ref.U[1])
+            print("C1 sampler %d iteration %d: max row rel err %.3e / %.3e" % (s, it, e0, e1))
+            assert e0 < tol and e1 < tol
+        m.iterate(st["burnin"] + st["nsamples"] - 2)
+        d = abs(m.rmse() - ref_full.rmse) / ref_full.rmse
+        print("C1 sampler %d: posterior-mean RMSE %.6f vs oracle %.6f (%.2e)" % (s, m.rmse(), ref_full.rmse, d))
+        assert d < 0.005
+
+
+# ---------------------------------------------------------------- C2
+@pytest.fixture(scope="module")
+def c2_state():
+    announce("C2 generate")
+    (tr_r, tr_c, tr_v), (te_r, te_c, te_v), (nrow, ncol), st = synthetic.c2()
+    K = st["num_latent"]
+    modes, mean, _ = O.matrix_rows(tr_r, tr_c, tr_v, nrow, ncol)
+    announce("C2 chain state")
+    m = GpuModel(2, K, "adaptive")
+    m.data(np.stack([tr_r, tr_c], 1), tr_v, [nrow, ncol], np.stack([te_r, te_c], 1), te_v)
+    m.prior()
+    m.prior()
+    m.setup(st["burnin"], st["nsamples"], SEED, BATCHED)
+    m.iterate(5)
+    state = dict(K=K, modes=modes, mean=mean, U=f32(m.latents(0, nrow)), V=f32(m.latents(1, ncol)),
+                 hyper=[m.hyper(0), m.hyper(1)], alpha=m.alpha())
+    del m
+    return state
+
+
+@pytest.mark.timeout(420)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_c2_half_sweep(c2_state, mode):
+    s = c2_state
+    rng = np.random.default_rng(mode)
+    rows = s["modes"][mode]
+    keep = pick_rows(rows, 50, 1950 if mode == 0 else 950, rng)
+    other = s["V"] if mode == 0 else s["U"]
+    mu, Lam = s["hyper"][mode]
+    print("C2 state: alpha %.4f" % s["alpha"])
+    compare_half_sweep("C2", s["K"], rows, s["mean"], [other], s["alpha"], mu, Lam, 6, mode, keep)
+
+
+# ---------------------------------------------------------------- C3
+@pytest.mark.timeout(300)
+def test_c3_blockcg_full_size():
+    announce("C3 block CG 100000 features")
+    _, _, (nrow, _), st, side = synthetic.c3()
+    F = side[0].tocoo()
+    nfeat = F.shape[1]
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((32, nfeat))
+    A, At = O.csr_pair(F.row, F.col, None, nrow, nfeat)
+    Xr, itr = O.blockcg(A, At, st["lambda_beta"], np.ascontiguousarray(B.T), st["tol"])
+    X, it = gpu_blockcg(F.row, F.col, None, nrow, nfeat, st["lambda_beta"], B, st["tol"])
+    err = np.linalg.norm(X - Xr.T) / np.linalg.norm(Xr)
+    print("C3 block CG: %d iterations (oracle %d), rel err %.3e" % (it, itr, err))
+    assert abs(it - itr) <= 2
+    assert err < 1e-6
+
+
+# ---------------------------------------------------------------- C4
+@pytest.fixture(scope="module")
+def c4_data():
+    announce("C4 generate")
+    (tr_idx, tr_v), _, dims, st, _ = synthetic.c4()
+    modes, mean, _ = O.tensor_rows(tr_idx, tr_v, dims)
+    return dict(modes=modes, mean=mean, dims=dims, K=st["num_latent"], alpha=st["precision"])
+
+
+@pytest.mark.timeout(420)
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_c4_half_sweep(c4_data, mode):
+    d = c4_data
+    K, dims = d["K"], d["dims"]
+    rng = np.random.default_rng(40 + mode)
+    fs = [f32(rng.standard_normal((n, K)) * 64 ** -0.25) for n in dims]
+    Lam, mu = spd(rng, K, 2.0), rng.standard_normal(K) * 0.1
+    rows = d["modes"][mode]
+    keep = pick_rows(rows, 20 if mode == 2 else 50, 0 if mode == 2 else 950, rng)
+    others = [fs[q] for q in range(3) if q != mode]
+    compare_half_sweep("C4", K, rows, d["mean"], others, d["alpha"], mu, Lam, 2, mode, keep)
+
+
+# ---------------------------------------------------------------- very heavy rows
+@pytest.mark.parametrize("K", [32, 64])
+def test_rows_of_sixteen_segments(K):
+    """Rows of 33000-36000 nonzeros (17-18 segments of 2048) next to light rows."""
+    announce("rows of 16+ segments K=%d" % K)
+    rng = np.random.default_rng(K)
+    nrow, ncol = 40, 36000
+    deg = rng.integers(20, 400, nrow)
+    deg[:4] = [36000, 34000, 33000, 32769]
+    r = np.repeat(np.arange(nrow), deg)
+    c = np.concatenate([rng.choice(ncol, d, replace=False) for d in deg])
+    y = np.round(rng.standard_normal(len(r)) * 2) / 2 + 3.5
+    modes, mean, _ = O.matrix_rows(r, c, y, nrow, ncol)
+    V = f32(rng.standard_normal((ncol, K)) * 0.3)
+    Lam, mu = spd(rng, K, 2.0), rng.standard_normal(K) * 0.2
+    compare_half_sweep("heavy K=%d" % K, K, modes[0], mean, [V], 10.0, mu, Lam, 5, 0, np.arange(nrow))
