@@ -17,7 +17,14 @@ uses: algorithmic FLOPs per half-sweep = nnz (K^2 + 3K) + rows (K^3/3 + 4K^2).
 
 cpu_baseline: the FP64 C/OpenMP oracle timed on this host for a bounded
 sample of the same workload (25 full Gibbs iterations, about 10 s on 16
-cores).
+cores), with the host CPU model.
+
+RMSE gate (headline workload): the oracle's full FP64 chain on the same data
+and seed runs on the host cores while the GPU chains run (after the timed
+region); the line reports each GPU chain's posterior-mean test RMSE relative
+to it, and every sampled quantity's error against the oracle after two
+iterations from the same initial state (rows and columns latents, mu and
+Lambda of both modes, alpha).
 
 Other workloads (`--workload`, one JSON line each, not the headline):
   c3  Macau with 100,000-dim sparse binary row features (block CG for the
@@ -193,23 +200,93 @@ def full_chain_rmse(name, w, seed):
         L.macau_free(h)
 
 
-def cpu_baseline(w, iters, seed, log):
-    import oracle as O  # the FP64 oracle, as the CPU baseline
-    threads = os.cpu_count() or 1
-    O.set_threads(threads)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Host cores this process may use (OMP_NUM_THREADS when set, else the
+    affinity mask)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+_ORACLE_INPUTS = {}
+
+
+def oracle_inputs(w):
+    """The FP64 oracle's view of a matrix workload (train rows of both modes,
+    test cells in the reference's order), built once."""
+    import oracle as O  # the FP64 oracle: the CPU baseline and the parity checker
+    key = id(w)
+    if key not in _ORACLE_INPUTS:
+        r, c = w["idx"][:, 0], w["idx"][:, 1]
+        dims = w["dims"]
+        modes, mean, nnz = O.matrix_rows(r, c, w["vals"], dims[0], dims[1])
+        ti, tv = O.matrix_test(w["tidx"][:, 0], w["tidx"][:, 1], w["tvals"], dims[0], dims[1])
+        _ORACLE_INPUTS[key] = (modes, mean, nnz, ti, tv)
+    return _ORACLE_INPUTS[key]
+
+
+def oracle_run(w, seed, **kw):
+    import oracle as O
     st = w["settings"]
-    r, c = w["idx"][:, 0], w["idx"][:, 1]
-    dims = w["dims"]
-    modes, mean, nnz = O.matrix_rows(r, c, w["vals"], dims[0], dims[1])
-    ti, tv = O.matrix_test(w["tidx"][:, 0], w["tidx"][:, 1], w["tvals"], dims[0], dims[1])
+    modes, mean, nnz, ti, tv = oracle_inputs(w)
+    O.set_threads(host_threads())
+    return O.run(modes, mean, nnz, list(w["dims"]), st["num_latent"], ti, tv, adaptive=True, burnin=st["burnin"],
+                 nsamples=st["nsamples"], seed=seed, **kw)
+
+
+def cpu_baseline(w, iters, seed, log):
+    import oracle as O
+    oracle_inputs(w)
     t0 = time.perf_counter()
-    O.run(modes, mean, nnz, list(dims), st["num_latent"], ti, tv, adaptive=True, burnin=st["burnin"],
-          nsamples=st["nsamples"], seed=seed, max_iters=iters)
+    oracle_run(w, seed, max_iters=iters)
     dt = time.perf_counter() - t0
+    threads = O.get_threads()
     log("cpu oracle: %d iterations in %.2f s on %d threads" % (iters, dt, threads))
-    return {"value": iters / dt, "unit": "iters/s", "cores": O.get_threads(), "kind": "port",
+    return {"value": iters / dt, "unit": "iters/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
             "sample": "%d full Gibbs iterations of the same 138000x27000 / 18M-train K=32 workload, FP64, "
                       "OpenMP dynamic,2 rows (C restatement of the reference)" % iters}
+
+
+def fp64_errors(w, name, seed, iters, log):
+    """Every sampled quantity of the GPU chain against the FP64 oracle after
+    `iters` iterations from the same initial state and draws: the largest
+    row-relative error of each mode's latents, the relative errors of each
+    mode's mu and Lambda, and of the noise precision alpha."""
+    L = _lib.lib()
+    K = w["settings"]["num_latent"]
+    dims = w["dims"]
+    h = build_model(w, kind_of(name), seed)
+    try:
+        check(L.macau_iterate(h, iters))
+        check(L.macau_synchronize(h))
+        out = {}
+        ref = oracle_run(w, seed, max_iters=iters, want_U=True)
+        for m, tag in enumerate(("rows", "cols")):
+            U = np.zeros((dims[m], K))
+            check(L.macau_get_latents(h, m, ptr(U)))
+            Ur = ref.U[m]
+            out["U_%s_max_row_rel" % tag] = float(np.max(np.linalg.norm(U - Ur, axis=1) /
+                                                         np.maximum(np.linalg.norm(Ur, axis=1), 1e-300)))
+            mu = np.zeros(K)
+            Lam = np.zeros((K, K), order="F")
+            check(L.macau_get_hyper(h, m, ptr(mu), ptr(Lam)))
+            out["mu_%s_rel" % tag] = float(np.linalg.norm(mu - ref.mu[m]) / np.linalg.norm(ref.mu[m]))
+            out["Lambda_%s_rel" % tag] = float(np.linalg.norm(Lam - ref.Lambda[m]) / np.linalg.norm(ref.Lambda[m]))
+        out["alpha_rel"] = float(abs(L.macau_get_noise_precision(h) - ref.alpha) / ref.alpha)
+        log("%s vs FP64 after %d iterations: %s" % (name, iters, json.dumps({k: "%.2e" % v for k, v in out.items()})))
+        return out
+    finally:
+        L.macau_free(h)
 
 
 def load_traffic(name):
@@ -246,6 +323,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 oracle chain and error table")
     ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--sampler", choices=["batched", "plain"], default=None)
     args = ap.parse_args()
@@ -270,11 +348,35 @@ def main():
     for name in names:
         sec, phase, cg, setup_s = time_sampler(name, w, args.steps, args.warmup, args.seed, log)
         res[name] = {"sec": sec, "phase": phase, "cg": cg, "setup_s": setup_s}
-        if not args.no_chain:
+    # the headline workload's RMSE gate is against the FP64 oracle's chain on
+    # the same data and seed; it runs on the host cores (a thread: the
+    # oracle's C code runs without the GIL) while the GPU chains run, after
+    # the timed region
+    fp64 = {}
+    gate = args.workload == "c2" and not args.no_chain and not args.no_fp64
+    if gate:
+        import threading
+
+        def fp64_chain():
+            t0 = time.perf_counter()
+            r = oracle_run(w, args.seed)
+            fp64["rmse"], fp64["sec"] = r.rmse, time.perf_counter() - t0
+
+        oracle_inputs(w)
+        th = threading.Thread(target=fp64_chain)
+        th.start()
+    if not args.no_chain:
+        for name in names:
             rmse, ips = full_chain_rmse(name, w, args.seed)
             res[name]["rmse"] = rmse
             log("%s full chain (%d iterations): posterior-mean %s %.6f" %
                 (name, st["burnin"] + st["nsamples"], "AUC" if st["precision"] == "probit" else "RMSE", rmse))
+    if gate:
+        th.join()
+        log("FP64 oracle full chain (%d iterations): posterior-mean RMSE %.6f in %.1f s" %
+            (st["burnin"] + st["nsamples"], fp64["rmse"], fp64["sec"]))
+        for name in names:
+            res[name]["fp64_err"] = fp64_errors(w, name, args.seed, 2, log)
 
     flops = sampler_flops(w)
     ph = res[head]["phase"]
@@ -357,6 +459,13 @@ def main():
         out[key] = {k: v.get("rmse") for k, v in res.items()}
         if "plain" in res and head != "plain":
             out[key]["rel_diff_vs_plain"] = res[head]["rmse"] / res["plain"]["rmse"] - 1.0
+        if gate:
+            out[key]["fp64"] = fp64["rmse"]
+            out[key]["rel_diff_vs_fp64"] = {k: v["rmse"] / fp64["rmse"] - 1.0 for k, v in res.items()}
+            out[key]["gate"] = "posterior-mean test RMSE of the %d-iteration chain within +0.5%% of the FP64 " \
+                               "oracle's on the same data and seed" % (st["burnin"] + st["nsamples"])
+            out[key]["gate_met"] = bool(res[head]["rmse"] <= 1.005 * fp64["rmse"])
+            out["fp64_error_after_2_iterations"] = {k: v["fp64_err"] for k, v in res.items()}
     if not args.no_cpu and args.workload == "c2":
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_iters, args.seed, log)
     print(json.dumps(out), flush=True)

@@ -139,6 +139,22 @@ int macau_cond_normal_wishart(int num_latent, int N, const double* U, const doub
                               const double* T, double nu, unsigned long long seed, int iter, int mode,
                               double* mu_out, double* Lambda_out);
 
+/* Allocation-free CondNormalWishart.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same draw as
+ * macau_cond_normal_wishart, but every array is a DEVICE pointer owned by the
+ * caller (d_U: K x N column-major FP64; d_mu0: K; d_T: K x K; outputs d_mu_out:
+ * K, d_Lambda_out: K x K, *d_status: 0, or nonzero when a matrix was not
+ * positive definite), the work is queued asynchronously on `stream`
+ * (a hipStream_t; NULL = the null stream) and no device memory is allocated
+ * or freed.  Working memory is one static device array of 2,146,560 bytes per
+ * GPU (held until the process ends), so num_latent must be in [1, 64]; larger
+ * values fail with an error.  Calls that overlap on one GPU share that array
+ * and race. */
+int macau_cond_normal_wishart_noalloc(int num_latent, int N, const double* d_U, const double* d_mu0, double kappa,
+                                      const double* d_T, double nu, unsigned long long seed, int iter, int mode,
+                                      double* d_mu_out, double* d_Lambda_out, int* d_status, void* stream);
+
 /* solve_blockcg (linop.h:228-343; Python blockcg, macau.pyx:67-91): solves
  * X (F^T F + reg I) = B for nrhs <= 40 right-hand sides; B and X are
  * nrhs x nfeat column-major; F is COO nrow x ncol (vals NULL = binary).

@@ -68,6 +68,7 @@ SIGNATURES = {
     "macau_get_ftf_solver": (_i, [_p, _i]),
     "macau_sample_latents": (_i, [_i, _i, _i, _p, _p, _p, _d, _p, _p, _d, _p, _p, _p, _ull, _i, _i, _i, _p]),
     "macau_cond_normal_wishart": (_i, [_i, _i, _p, _p, _d, _p, _d, _ull, _i, _i, _p, _p]),
+    "macau_cond_normal_wishart_noalloc": (_i, [_i, _i, _p, _p, _d, _p, _d, _ull, _i, _i, _p, _p, _p, _p]),
     "macau_solve_blockcg": (_i, [_i, _i, _i, _ll, _p, _p, _p, _d, _p, _d, _p, _p]),
     "macau_rng_probe": (_i, [_ull, _u, _u, _u, _i, _p, _p, _p]),
     "macau_gamma_probe": (_i, [_ull, _p, _i, _u, _u, _p]),

@@ -486,6 +486,15 @@ int macau_cond_normal_wishart(int K, int N, const double* U, const double* mu0, 
   API_END_INT
 }
 
+int macau_cond_normal_wishart_noalloc(int num_latent, int N, const double* d_U, const double* d_mu0, double kappa,
+                                      const double* d_T, double nu, unsigned long long seed, int iter, int mode,
+                                      double* d_mu_out, double* d_Lambda_out, int* d_status, void* stream) {
+  API_BEGIN
+  cond_normal_wishart_noalloc(num_latent, N, d_U, d_mu0, kappa, d_T, nu, seed, iter, mode, d_mu_out, d_Lambda_out,
+                              d_status, static_cast<hipStream_t>(stream));
+  API_END_INT
+}
+
 int macau_solve_blockcg(int nrhs, int nrow, int ncol, long long nnz, const int* rows, const int* cols,
                         const double* vals, double reg, const double* B, double tol, double* X, int* iters) {
   API_BEGIN

@@ -337,4 +337,11 @@ void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s));
 }
 
+// CondNormalWishart on caller-owned device memory with no device allocation
+// (noalloc.hip): U, mu0, T, mu_out, Lambda_out and status are device
+// pointers; K <= 64; asynchronous on s
+void cond_normal_wishart_noalloc(int K, int N, const double* U, const double* mu0, double kappa, const double* T,
+                                 double nu, uint64_t seed, int iter, int mode, double* mu_out, double* Lambda_out,
+                                 int* status, hipStream_t s);
+
 }  // namespace mgpu

@@ -10,7 +10,8 @@
 // Layout: one work-group handles G = 64/KP rows (KP <= 64) or one row
 // (KP = 128); a group of KP lanes owns one row and lane i owns row i of P
 // in registers.  The Gram is accumulated in FP32 on the vector ALUs from
-// nonzeros staged through LDS.  The Cholesky is right-looking with the
+// nonzeros staged through LDS, in segments of 2048 nonzeros whose sums are
+// added in order.  The Cholesky is right-looking with the
 // current column broadcast through LDS; the forward solve L y = b rides
 // along as an extra column; the back solve reads L row-major from LDS.
 // Rows are processed in natural order with no load balancing: this is the
@@ -62,11 +63,31 @@ __global__ void __launch_bounds__(KP < 64 ? 64 : KP) k_sample_plain(SampleArgs a
   const int it = *a.iter;
 
   // ---- Gram and right-hand side ----
+  // acc sums at most kSeg nonzeros; each full segment is then added into this
+  // lane's row of Lrow (a lane-private LDS row until the back solve), so a row
+  // of ~10^6 nonzeros is summed as ~500 FP32 segment sums instead of one
+  // running FP32 sum
+  constexpr int kSegSteps = 2048 / CH;
   float acc[KP];
 #pragma unroll
-  for (int j = 0; j < KP; j++) acc[j] = 0.f;
-  float bi = 0.f;
+  for (int j = 0; j < KP; j++) {
+    acc[j] = 0.f;
+    Lrow[g][i][j] = 0.f;
+  }
+  float bi = 0.f, bseg = 0.f;
+  int steps = 0;
   for (int base = 0; __syncthreads_or(base < nnz); base += CH) {
+    if (steps == kSegSteps) {
+#pragma unroll
+      for (int j = 0; j < KP; j++) {
+        Lrow[g][i][j] += acc[j];
+        acc[j] = 0.f;
+      }
+      bi += bseg;
+      bseg = 0.f;
+      steps = 0;
+    }
+    steps++;
 #pragma unroll
     for (int c = 0; c < CH; c++) {
       float x = 0.f, rr = 0.f;
@@ -85,10 +106,13 @@ __global__ void __launch_bounds__(KP < 64 ? 64 : KP) k_sample_plain(SampleArgs a
       const float rc = rbuf[g][c];
 #pragma unroll
       for (int j = 0; j < KP; j++) acc[j] += vi * vbuf[g][c][j];
-      bi += vi * rc;
+      bseg += vi * rc;
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int j = 0; j < KP; j++) acc[j] += Lrow[g][i][j];
+  bi += bseg;
 
   // ---- P = Lambda + alpha G,  b = alpha c + Lambda mu ----
   const float alpha = (float)(*a.alpha);

@@ -62,6 +62,7 @@ class OrcRunCfg(ctypes.Structure):
         ("alpha", _dbl), ("alpha_trace", _p), ("rmse_trace", _p), ("lambda_beta_out", _p),
         ("cg_iters_last", ctypes.c_int), ("use_ftf", ctypes.c_int * 8),
         ("noise_probit", ctypes.c_int), ("auc", _dbl), ("auc1", _dbl), ("uni", ctypes.c_int * 8),
+        ("mu_out", _p * 8), ("Lambda_out", _p * 8),
     ]
 
 
@@ -580,7 +581,8 @@ class RunResult:
 def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False, adaptive=False,
         precision=1.0, sn_init=1.0, sn_max=10.0, side=None, lambda_beta=5.0, tol=1e-6, burnin=10,
         nsamples=10, seed=0, max_iters=0, want_U=False, use_ftf=None, probit=False, uni=None):
-    """Full Gibbs run (MacauX::run).  side: per-mode (F, Ft) Csr pairs or None;
+    """Full Gibbs run (MacauX::run).  want_U: also return the last
+    iteration's U, mu and Lambda of every mode.  side: per-mode (F, Ft) Csr pairs or None;
     use_ftf: per-mode flags selecting the F^T F branch of sample_beta;
     probit: ProbitNoise (matrix data), reporting auc / auc1;
     uni: per-mode flags selecting MacauOnePrior for modes with side information."""
@@ -616,12 +618,16 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
     lb = np.zeros(8)
     cfg.pred, cfg.pred_var, cfg.alpha_trace, cfg.rmse_trace = _ptr(pred), _ptr(pvar), _ptr(at), _ptr(rt)
     cfg.lambda_beta_out = _ptr(lb)
-    Us = []
+    Us, mus, Lams = [], [], []
     if want_U:
         for m in range(M):
             u = np.zeros((dims[m], K))
             Us.append(u)
             cfg.U_out[m] = _ptr(u)
+            mus.append(np.zeros(K))
+            Lams.append(np.zeros((K, K), order="F"))
+            cfg.mu_out[m] = _ptr(mus[m])
+            cfg.Lambda_out[m] = _ptr(Lams[m])
     rc = lib().orc_run(ctypes.byref(cfg))
     if rc:
         raise RuntimeError("oracle run failed (%d)" % rc)
@@ -631,6 +637,7 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
     res.pred, res.pred_var = pred, pvar
     res.alpha_trace, res.rmse_trace = at[:total], rt[:total]
     res.U = Us
+    res.mu, res.Lambda = mus, [np.array(L) for L in Lams]
     res.lambda_beta = lb[:M]
     res.cg_iters_last = cfg.cg_iters_last
     return res

@@ -1241,6 +1241,8 @@ int orc_run(orc_run_cfg* c) {
   c->alpha = alpha;
   for (int m = 0; m < M; m++) {
     if (c->U_out[m]) memcpy(c->U_out[m], U[m], sizeof(double) * (size_t)K * c->dims[m]);
+    if (c->mu_out[m]) memcpy(c->mu_out[m], mu[m], sizeof(double) * K);
+    if (c->Lambda_out[m]) memcpy(c->Lambda_out[m], Lambda[m], sizeof(double) * KK);
     if (c->lambda_beta_out && c->macau[m] && c->uni[m]) {
       double s = 0.0;
       for (int k = 0; k < K; k++) s += lb_uni[m][k];

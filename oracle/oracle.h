@@ -239,6 +239,10 @@ typedef struct {
    * MacauOnePrior (1) instead of MacauPrior (0); lambda_beta_out then
    * receives the mean of its per-latent lambda_beta */
   int      uni[8];
+  /* outputs: per mode mu (K) and Lambda (K x K col-major) after the last
+   * iteration, or NULL */
+  double*  mu_out[8];
+  double*  Lambda_out[8];
 } orc_run_cfg;
 int orc_run(orc_run_cfg* cfg);
 

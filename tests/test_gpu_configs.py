@@ -33,9 +33,10 @@ pytestmark = pytest.mark.gpu
 
 PLAIN, BATCHED = _lib.SAMPLER_PLAIN_FP32, _lib.SAMPLER_BATCHED
 TOL_FP32 = 1e-4
-# split-BF16 Gram with FP32 accumulation: the bound is 10x the largest row
-# error measured at these configurations (printed by each test)
-TOL_BATCHED = 1e-3
+# split-BF16 Gram with FP32 accumulation: held to the FP32 path's bound (the
+# largest row error measured at these configurations is 3.7e-5; each test
+# prints its own)
+TOL_BATCHED = 1e-4
 
 
 def keep_rows(rows, keep):

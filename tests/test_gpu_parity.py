@@ -402,10 +402,11 @@ def test_macau_graph_replay_matches_eager(kind, K):
 
 # ------------------------------------------------------------ batched sampler
 BATCHED = _lib.SAMPLER_BATCHED
-# Split-BF16 Gram (hi*hi + hi*lo + lo*hi, FP32 accumulate) keeps ~16 bits per
-# factor value; the row error against the FP64 oracle is bounded here and
-# printed.  The posterior-mean RMSE gate (0.5%) is checked at model level.
-TOL_BATCHED = 5e-3
+# Split-BF16 Gram (hi*hi + hi*lo + lo*hi, FP32 accumulate): the row error
+# against the FP64 oracle is printed and held to the FP32 path's 1e-4 (the
+# largest measured here is 2e-5); the posterior-mean RMSE gate (0.5%) is
+# checked at model level.
+TOL_BATCHED = 1e-4
 
 
 @pytest.mark.parametrize("K", [17, 24, 32, 33, 48, 64])
@@ -475,7 +476,7 @@ def test_batched_model_matches_oracle(precision, K):
     ea = abs(m.alpha() - ref.alpha) / ref.alpha
     print("batched model iter 1: row err %.3e %.3e, alpha err %.3e" % (e0, e1, ea))
     assert e0 < TOL_BATCHED and e1 < TOL_BATCHED
-    assert ea < 1e-3
+    assert ea < 2e-5
     m.iterate(29)
     ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, **kw)
     print("batched model RMSE %.6f vs oracle %.6f" % (m.rmse(), ref.rmse))

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 PLAIN = _lib.SAMPLER_PLAIN_FP32
 BATCHED = _lib.SAMPLER_BATCHED
 TOL_FP32 = 1e-4
-TOL_BATCHED = 5e-3
+TOL_BATCHED = 1e-4
 
 
 def _side_problem(seed, nrow, ncol, nfeat, density, binary):
