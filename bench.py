@@ -40,7 +40,11 @@ Other workloads (`--workload`, one JSON line each, not the headline):
       the 100,000 features for the link matrix;
   c2p the c2 cells binarised at the train mean, probit noise (the
       augmentation kernel draws one truncated normal per observation before
-      each half-sweep); the full-chain line reports test AUC instead of RMSE.
+      each half-sweep); the full-chain line reports test AUC instead of RMSE;
+  c5  64 independent BPMF instances of uneven size and num_latent in
+      {8, 16, 32, 64, 128} advanced as one launch set (macau_batch_*: one HIP
+      graph per Gibbs iteration of all instances); a step is one iteration of
+      every instance; also reports the same models advanced one after another.
 """
 import argparse
 import ctypes
@@ -313,13 +317,109 @@ def cg_bytes_per_iteration(w):
     return 2 * F.nnz * 12 + 13 * K * F.shape[1] * 8
 
 
+def c5_workloads(seed):
+    out = []
+    for q, (train, test, dims, st) in enumerate(synthetic.c5(seed)):
+        out.append(dict(idx=np.stack([train[0], train[1]], 1), vals=train[2], tidx=np.stack([test[0], test[1]], 1),
+                         tvals=test[2], dims=dims, settings=st, side=[None, None], label="instance %d" % q))
+    return out
+
+
+def bench_c5(args, log):
+    """64 uneven instances as one launch set against the same models advanced
+    one by one."""
+    L = _lib.lib()
+    t0 = time.perf_counter()
+    ws = c5_workloads(args.seed)
+    log("generated c5 in %.1f s: %d instances, %d train cells" % (time.perf_counter() - t0, len(ws),
+                                                                   sum(len(w["vals"]) for w in ws)))
+    kinds = []
+    for w in ws:
+        K = w["settings"]["num_latent"]
+        kinds.append("batched" if L.macau_sampler_supported(_lib.SAMPLER_BATCHED, K, 2) else "plain")
+    t0 = time.perf_counter()
+    hs = [build_model(w, kind_of(k), args.seed + q) for q, (w, k) in enumerate(zip(ws, kinds))]
+    log("built %d models in %.1f s" % (len(hs), time.perf_counter() - t0))
+    arr = (ctypes.c_void_p * len(hs))(*hs)
+    b = L.macau_batch_create(arr, len(hs))
+    if not b:
+        raise RuntimeError(_lib.last_error())
+    try:
+        check(L.macau_batch_iterate(b, args.warmup))
+        check(L.macau_batch_synchronize(b))
+        t0 = time.perf_counter()
+        check(L.macau_batch_iterate(b, args.steps))
+        check(L.macau_batch_synchronize(b))
+        sec = (time.perf_counter() - t0) / args.steps
+        log("batch: %d steps in %.3f s -> %.1f batch iterations/s" % (args.steps, sec * args.steps, 1.0 / sec))
+        # the same models one after another (each its own graph), same number of iterations
+        for h in hs:
+            check(L.macau_iterate(h, 1))
+            check(L.macau_synchronize(h))
+        t0 = time.perf_counter()
+        for h in hs:
+            check(L.macau_iterate(h, args.steps))
+            check(L.macau_synchronize(h))
+        seq = (time.perf_counter() - t0) / args.steps
+        log("one by one: %.3f ms per iteration of all instances" % (seq * 1e3))
+        rmse = []
+        if not args.no_chain:
+            done = args.warmup + args.steps + 1 + args.steps
+            total = max(w["settings"]["burnin"] + w["settings"]["nsamples"] for w in ws)
+            check(L.macau_batch_iterate(b, max(total - done, 0)))
+            check(L.macau_batch_synchronize(b))
+            rmse = [float(L.macau_get_rmse_avg(h)) for h in hs]
+            log("posterior-mean test RMSE after %d iterations: min %.4f median %.4f max %.4f (noise sd 0.5)" %
+                (max(total, done), min(rmse), float(np.median(rmse)), max(rmse)))
+    finally:
+        L.macau_batch_free(b)
+        for h in hs:
+            L.macau_free(h)
+    flops = {"plain": 0.0, "batched": 0.0}
+    by_k = {}
+    for w, k in zip(ws, kinds):
+        flops[k] += sampler_flops(w)
+        K = w["settings"]["num_latent"]
+        e = by_k.setdefault(str(K), {"instances": 0, "train_cells": 0, "sampler": k})
+        e["instances"] += 1
+        e["train_cells"] += len(w["vals"])
+    rows = sum(sum(w["dims"]) for w in ws)
+    out = {
+        "metric": "Gibbs iterations/sec of the 64-instance uneven batch (c5), every instance one iteration per step",
+        "value": 1.0 / sec,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3-f32acc (K 17-64) / f32 (K 8, 16, 128)",
+        "data": "synthetic",
+        "config": {"workload": "64 independent BPMF instances, rows x cols log-uniform in [1000,50000] x "
+                               "[500,10000], density log-uniform in [0.1%,5%], K in {8,16,32,64,128}, fixed "
+                               "precision 4", "parallelism": "single-gpu", "global_batch": sum(len(w["vals"]) for w in ws)},
+        "instance_iterations_per_sec": len(ws) / sec,
+        "row_updates_per_sec": rows / sec,
+        "one_by_one_ms_per_step": seq * 1e3,
+        "speedup_vs_one_by_one": seq / sec,
+        "instances_by_latent": by_k,
+        "sampler_tflops": {k: v / sec / 1e12 for k, v in flops.items()},
+        "note": "sampler_tflops = algorithmic sampler FLOPs of the instances on each sampler / whole batch step time",
+    }
+    if rmse:
+        out["rmse"] = {"min": min(rmse), "median": float(np.median(rmse)), "max": max(rmse), "noise_sd": 0.5}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c3u", "c4", "c2p"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c3u", "c4", "c2p", "c5"], default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
@@ -335,6 +435,8 @@ def main():
     def log(msg):
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
+    if args.workload == "c5":
+        return bench_c5(args, log)
     t0 = time.perf_counter()
     w = load_workload(args.workload, args.seed)
     st = w["settings"]

@@ -120,6 +120,22 @@ int macau_get_last_cg_iters(macau_model* m, int mode);
  * MACAU_AMD_FTF_SOLVER=cholesky selects 1). */
 int macau_get_ftf_solver(macau_model* m, int mode);
 
+/* ---- batches: many independent models as one launch set (no reference
+ *      counterpart: the reference runs one model per process,
+ *      macau.cpp:94-148) ----
+ * A batch advances N initialised models (any sizes, latent dimensions,
+ * priors and noise models; no Macau prior solved by block CG) by one Gibbs
+ * iteration per replay of one HIP graph whose branches are the models'
+ * iterations.  Each model keeps its own state, draws and iteration counter:
+ * the getters above read it as usual after macau_batch_synchronize.  The
+ * batch does not own the models; free the batch before them. */
+typedef struct macau_batch macau_batch;
+macau_batch* macau_batch_create(macau_model* const* models, int n);
+void macau_batch_free(macau_batch* b);
+int macau_batch_set_graph(macau_batch* b, int on); /* 0: launch the iterations without a graph */
+int macau_batch_iterate(macau_batch* b, int n);     /* n iterations of every model, asynchronous */
+int macau_batch_synchronize(macau_batch* b);        /* wait; raise sampler failures */
+
 /* ---- stateless hot-path entry points ---- */
 
 /* One half-sweep: ILatentPrior::sample_latents(U, mat, mean, samples, alpha, K)

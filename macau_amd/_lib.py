@@ -66,6 +66,11 @@ SIGNATURES = {
     "macau_get_link_lambda": (_d, [_p, _i]),
     "macau_get_last_cg_iters": (_i, [_p, _i]),
     "macau_get_ftf_solver": (_i, [_p, _i]),
+    "macau_batch_create": (_p, [_p, _i]),
+    "macau_batch_free": (None, [_p]),
+    "macau_batch_set_graph": (_i, [_p, _i]),
+    "macau_batch_iterate": (_i, [_p, _i]),
+    "macau_batch_synchronize": (_i, [_p]),
     "macau_sample_latents": (_i, [_i, _i, _i, _p, _p, _p, _d, _p, _p, _d, _p, _p, _p, _ull, _i, _i, _i, _p]),
     "macau_cond_normal_wishart": (_i, [_i, _i, _p, _p, _d, _p, _d, _ull, _i, _i, _p, _p]),
     "macau_cond_normal_wishart_noalloc": (_i, [_i, _i, _p, _p, _d, _p, _d, _ull, _i, _i, _p, _p, _p, _p]),

@@ -5,12 +5,17 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "model.h"
 
 using namespace mgpu;
 
 struct macau_model {
   std::unique_ptr<Model> m;
+};
+
+struct macau_batch {
+  std::unique_ptr<ModelBatch> b;
 };
 
 static thread_local std::string g_err;
@@ -483,6 +488,42 @@ int macau_cond_normal_wishart(int K, int N, const double* U, const double* mu0, 
   HIPCHK(hipMemcpyAsync(&fail, dfail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (fail) throw std::runtime_error("CondNormalWishart: matrix not positive definite");
+  API_END_INT
+}
+
+macau_batch* macau_batch_create(macau_model* const* models, int n) {
+  API_BEGIN
+  if (n < 1 || !models) throw std::runtime_error("a batch needs at least one model");
+  std::vector<Model*> ms;
+  for (int q = 0; q < n; q++) ms.push_back(&M(models[q]));
+  auto* h = new macau_batch;
+  h->b.reset(new ModelBatch(ms));
+  return h;
+  API_END_VAL(nullptr)
+}
+
+void macau_batch_free(macau_batch* b) { delete b; }
+
+static ModelBatch& B(macau_batch* h) {
+  if (!h || !h->b) throw std::runtime_error("null batch");
+  return *h->b;
+}
+
+int macau_batch_set_graph(macau_batch* b, int on) {
+  API_BEGIN
+  B(b).use_graph = on != 0;
+  API_END_INT
+}
+
+int macau_batch_iterate(macau_batch* b, int n) {
+  API_BEGIN
+  B(b).iterate(n);
+  API_END_INT
+}
+
+int macau_batch_synchronize(macau_batch* b) {
+  API_BEGIN
+  B(b).synchronize();
   API_END_INT
 }
 

@@ -29,6 +29,15 @@ Each generator returns numpy arrays only (no GPU work):
   (= 1/sqrt(64)) plus N(0, 0.3^2) noise; 10% held out; K = 64, fixed
   precision 1/0.09, burnin 30, nsamples 50.
 
+* ``c5`` -- 64 independent BPMF instances of uneven size: row and column
+  counts log-uniform in [1,000, 50,000] x [500, 10,000], density log-uniform
+  in [0.1%, 5%] (cells uniformly at random), num_latent drawn from
+  {8, 16, 32, 64, 128}; values u.v + N(0, 0.5^2) from a truth of the
+  instance's rank whose factor entries have variance K^-1/2 (so u.v has unit
+  variance at every rank); 10% held out; fixed precision 4 (= 1 / 0.5^2),
+  burnin 20, nsamples 50 (chain lengths are not fixed by the configuration).
+  ``scale`` < 1 shrinks every instance's row and column counts (for tests).
+
 ``c3`` and ``c4`` return the generic form used for N-mode data:
   (train_idx [nnz x nmodes] int32, train_vals), (test_idx, test_vals), dims,
   settings, side (one scipy COO matrix or None per mode).
@@ -193,3 +202,38 @@ def c4(seed=0, dims=(50000, 2000, 100), ncell=10_000_000, K=64, R=64):
     te, tr = perm[:nte], perm[nte:]
     settings = dict(num_latent=K, precision=1.0 / 0.09, burnin=30, nsamples=50)
     return (idx[tr], y[tr]), (idx[te], y[te]), tuple(dims), settings, [None, None, None]
+
+
+def _uniform_cells(rng, nrow, ncol, ncell):
+    """ncell distinct cells drawn uniformly from an nrow x ncol matrix."""
+    total = nrow * ncol
+    ncell = min(ncell, total)
+    keys = np.zeros(0, dtype=np.int64)
+    while len(keys) < ncell:
+        m = int(1.2 * (ncell - len(keys))) + 16
+        keys = np.unique(np.concatenate([keys, rng.integers(0, total, m, dtype=np.int64)]))
+    if len(keys) > ncell:
+        keys = np.sort(rng.choice(keys, ncell, replace=False))
+    return (keys // ncol).astype(np.int32), (keys % ncol).astype(np.int32)
+
+
+def c5(seed=0, n_instances=64, scale=1.0):
+    """List of instances, each (train, test, shape, settings) like ``c1``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n_instances):
+        nrow = int(round(np.exp(rng.uniform(np.log(1000), np.log(50000))) * scale))
+        ncol = int(round(np.exp(rng.uniform(np.log(500), np.log(10000))) * scale))
+        nrow, ncol = max(nrow, 8), max(ncol, 8)
+        dens = np.exp(rng.uniform(np.log(0.001), np.log(0.05)))
+        K = int(rng.choice([8, 16, 32, 64, 128]))
+        ncell = max(int(round(nrow * ncol * dens)), 20)
+        r, c = _uniform_cells(rng, nrow, ncol, ncell)
+        sd = K ** -0.25
+        U = (rng.standard_normal((nrow, K)) * sd).astype(np.float32)
+        V = (rng.standard_normal((ncol, K)) * sd).astype(np.float32)
+        y = np.einsum("ij,ij->i", U[r], V[c]).astype(np.float64) + 0.5 * rng.standard_normal(len(r))
+        train, test = _split(rng, r, c, y, 0.1)
+        settings = dict(num_latent=K, precision=4.0, burnin=20, nsamples=50, density=float(dens))
+        out.append((train, test, (nrow, ncol), settings))
+    return out
