@@ -179,6 +179,10 @@ def time_sampler(name, w, steps, warmup, seed, log):
         check(L.macau_profile(h, nprof, ptr(phase)))
         cg = [int(L.macau_get_last_cg_iters(h, m)) for m in range(nm)]
         cg.append(int(L.macau_get_ftf_solver(h, 0)))  # F^T F solver of the rows
+        flagged, refined = ctypes.c_longlong(), ctypes.c_longlong()
+        check(L.macau_get_refine_stats(h, ctypes.byref(flagged), ctypes.byref(refined)))
+        its = warmup + steps + nprof
+        cg.append({"rows_flagged_per_iteration": flagged.value / its, "rows_refined_per_iteration": refined.value / its})
         log("%s: %d steps in %.3f s -> %.1f it/s; phases(ms) %s" %
             (name, steps, t1 - t0, steps / (t1 - t0), np.round(phase, 3).tolist()))
     finally:
@@ -317,6 +321,39 @@ def cg_bytes_per_iteration(w):
     return 2 * F.nnz * 12 + 13 * K * F.shape[1] * 8
 
 
+def autotune(w, args, log):
+    """Default launch shapes timed, then the autotuner (macau_autotune), whose
+    choices go to a table that the models built afterwards read at init
+    (MACAU_AMD_TUNE_CACHE)."""
+    L = _lib.lib()
+    os.environ["MACAU_AMD_TUNE_DISABLE"] = "1"
+    sec_def, _, _, _ = time_sampler("batched", w, args.steps, args.warmup, args.seed, log)
+    del os.environ["MACAU_AMD_TUNE_DISABLE"]
+    rec = os.path.join(ROOT, "gpurun_out", "tuning_%s.txt" % args.workload)
+    os.makedirs(os.path.dirname(rec), exist_ok=True)
+    if os.path.exists(rec):
+        os.remove(rec)
+    h = build_model(w, kind_of("batched"), args.seed)
+    nm = len(w["dims"])
+    dms, tms = np.zeros(nm), np.zeros(nm)
+    try:
+        t0 = time.perf_counter()
+        check(L.macau_autotune(h, 5, rec.encode(), ptr(dms), ptr(tms)))
+        tune_s = time.perf_counter() - t0
+        knobs = []
+        for m in range(nm):
+            k = np.zeros(4, dtype=np.int32)
+            L.macau_get_tuning(h, m, ptr(k))
+            knobs.append({"seg_len": int(k[0]), "light_cap": int(k[1]), "heavy_cap": int(k[2]), "seg_cap": int(k[3])})
+    finally:
+        L.macau_free(h)
+    os.environ["MACAU_AMD_TUNE_CACHE"] = rec
+    log("autotune (%.1f s): half-sweep ms default %s -> tuned %s; %s" %
+        (tune_s, np.round(dms, 4).tolist(), np.round(tms, 4).tolist(), knobs))
+    return {"default_iters_per_sec": 1.0 / sec_def, "tune_seconds": tune_s, "record": os.path.relpath(rec, ROOT),
+            "modes": [{"default_ms": float(dms[m]), "tuned_ms": float(tms[m]), "shape": knobs[m]} for m in range(nm)]}
+
+
 def c5_workloads(seed):
     out = []
     for q, (train, test, dims, st) in enumerate(synthetic.c5(seed)):
@@ -350,8 +387,21 @@ def bench_c5(args, log):
         t0 = time.perf_counter()
         check(L.macau_batch_iterate(b, args.steps))
         check(L.macau_batch_synchronize(b))
-        sec = (time.perf_counter() - t0) / args.steps
-        log("batch: %d steps in %.3f s -> %.1f batch iterations/s" % (args.steps, sec * args.steps, 1.0 / sec))
+        sec_graph = (time.perf_counter() - t0) / args.steps
+        log("batch graph: %d steps in %.3f s -> %.1f batch iterations/s" %
+            (args.steps, sec_graph * args.steps, 1.0 / sec_graph))
+        # each model's own graph on its own streams, all queued before the wait
+        check(L.macau_batch_set_graph(b, 0))
+        check(L.macau_batch_iterate(b, 1))
+        check(L.macau_batch_synchronize(b))
+        t0 = time.perf_counter()
+        check(L.macau_batch_iterate(b, args.steps))
+        check(L.macau_batch_synchronize(b))
+        sec_streams = (time.perf_counter() - t0) / args.steps
+        log("batch streams: %d steps in %.3f s -> %.1f batch iterations/s" %
+            (args.steps, sec_streams * args.steps, 1.0 / sec_streams))
+        mode = "graph" if sec_graph <= sec_streams else "streams"
+        sec = min(sec_graph, sec_streams)
         # the same models one after another (each its own graph), same number of iterations
         for h in hs:
             check(L.macau_iterate(h, 1))
@@ -364,7 +414,7 @@ def bench_c5(args, log):
         log("one by one: %.3f ms per iteration of all instances" % (seq * 1e3))
         rmse = []
         if not args.no_chain:
-            done = args.warmup + args.steps + 1 + args.steps
+            done = args.warmup + 3 * args.steps + 2
             total = max(w["settings"]["burnin"] + w["settings"]["nsamples"] for w in ws)
             check(L.macau_batch_iterate(b, max(total - done, 0)))
             check(L.macau_batch_synchronize(b))
@@ -402,6 +452,9 @@ def bench_c5(args, log):
                                "precision 4", "parallelism": "single-gpu", "global_batch": sum(len(w["vals"]) for w in ws)},
         "instance_iterations_per_sec": len(ws) / sec,
         "row_updates_per_sec": rows / sec,
+        "launch": mode,
+        "graph_ms_per_step": sec_graph * 1e3,
+        "streams_ms_per_step": sec_streams * 1e3,
         "one_by_one_ms_per_step": seq * 1e3,
         "speedup_vs_one_by_one": seq / sec,
         "instances_by_latent": by_k,
@@ -426,6 +479,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 oracle chain and error table")
     ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--sampler", choices=["batched", "plain"], default=None)
+    ap.add_argument("--tune", action="store_true",
+                    help="time the default launch shapes, run the autotuner, time the tuned shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (one GPU per box)" % args.gpus,
@@ -446,6 +501,9 @@ def main():
     batched_ok = bool(_lib.lib().macau_sampler_supported(_lib.SAMPLER_BATCHED, K, nm))
     head = args.sampler or ("batched" if batched_ok else "plain")
     res = {}
+    tuning = None
+    if args.tune and head == "batched":
+        tuning = autotune(w, args, log)
     names = [head] + ([] if (args.no_plain or head == "plain") else ["plain"])
     for name in names:
         sec, phase, cg, setup_s = time_sampler(name, w, args.steps, args.warmup, args.seed, log)
@@ -526,7 +584,7 @@ def main():
                                            "Normal-Wishart draw, the parallel pre-pass, Uhat and lambda_beta"}
     elif w["side"][0] is not None and w["side"][0].shape[1] <= MAX_FF_SIZE:
         nf = w["side"][0].shape[1]
-        solver = res[head]["cg"][-1]
+        solver = res[head]["cg"][-2]
         if solver == 2:
             # (F^T F + lambda_beta I)^{-1} as Q diag(d + lambda_beta)^{-1} Q^T: two n x n FP64
             # products per iteration stream Q and Q^T once each
@@ -553,6 +611,10 @@ def main():
                            "algorithmic_bytes_per_solve": cgb, "achieved_GBs_lower_bound": gbs,
                            "frac_of_hbm_peak": gbs / PEAK_HBM_GBS,
                            "note": "time includes the beta noise draw, Uhat and lambda_beta updates"}
+    if tuning is not None:
+        out["tuning"] = tuning
+    if head == "batched":
+        out["refinement"] = dict(res[head]["cg"][-1], threshold="(max L_ii / min L_ii)^2 > 10, rows of >= K nonzeros")
     if "plain" in res and head != "plain":
         out["plain_iters_per_sec"] = 1.0 / res["plain"]["sec"]
         out["speedup_vs_plain"] = res["plain"]["sec"] / sec

@@ -30,6 +30,8 @@ typedef struct macau_model macau_model;
 /* sampler kinds (macau_set_sampler) */
 #define MACAU_SAMPLER_PLAIN_FP32 0 /* one work-group per row, FP32 Gram       */
 #define MACAU_SAMPLER_BATCHED    1 /* nnz-bucketed, split-BF16 MFMA Gram      */
+/* macau_sample_latents only: the batched sampler without its refinement step */
+#define MACAU_SAMPLER_BATCHED_NOREFINE 2
 
 const char* macau_last_error(void);
 const char* macau_version(void);
@@ -120,6 +122,41 @@ int macau_get_last_cg_iters(macau_model* m, int mode);
  * MACAU_AMD_FTF_SOLVER=cholesky selects 1). */
 int macau_get_ftf_solver(macau_model* m, int mode);
 
+/* ---- refinement of ill-conditioned rows (batched sampler; no reference
+ *      counterpart: it guards the reference's FP64 solve,
+ *      latentprior.cpp:373-384, against the FP32 / split-BF16 arithmetic) ----
+ * After each row's Cholesky the batched sampler estimates the row's
+ * conditioning from its pivots, (max_i L_ii / min_i L_ii)^2; a row above
+ * `threshold` (default 10) with at least num_latent nonzeros gets one step
+ * of iterative refinement against the exact precision matrix (residual from
+ * per-observation residuals, FP64 sums).  macau_set_refine (before
+ * macau_init) switches it off (on = 0) or sets the threshold (> 0);
+ * macau_get_refine_stats reports the rows flagged and refined so far (a
+ * half-sweep refines at most 4096 rows). */
+int macau_set_refine(macau_model* m, int on, double threshold);
+int macau_get_refine_stats(macau_model* m, long long* flagged, long long* refined);
+
+/* ---- autotuner of the batched sampler (no reference counterpart: the
+ *      reference schedules every row loop as OpenMP "dynamic, 2",
+ *      latentprior.cpp:73) ----
+ * Knobs per mode: heavy-row segment length, and the work-group caps of the
+ * light-row, heavy-row and segment kernels.  macau_autotune (after
+ * macau_init, before or between iterations) times every candidate shape of
+ * every mode on the device with `reps` half-sweeps each, keeps the fastest
+ * (the default unless a candidate is more than 1% faster), leaves the chain's
+ * state as it was, writes each mode's half-sweep ms with the default and the
+ * chosen shape to default_ms[nmodes] / tuned_ms[nmodes] (either may be
+ * NULL) and, when record_path is not NULL, appends the choices to that
+ * table.  macau_init looks the shape up by (GPU architecture, padded latent
+ * size, other modes, rows, nonzeros, log2 row-degree histogram) in the table
+ * named by the environment variable MACAU_AMD_TUNE_CACHE, then in
+ * tuning.txt beside the library.  Plain-sampler models have nothing to tune.
+ * macau_get_tuning writes (segment length, light cap, heavy cap, segment
+ * cap) of a mode to knobs[4] and returns 1 if they come from the tuner or
+ * the table, 0 for the defaults, -1 on error. */
+int macau_autotune(macau_model* m, int reps, const char* record_path, double* default_ms, double* tuned_ms);
+int macau_get_tuning(macau_model* m, int mode, int* knobs);
+
 /* ---- batches: many independent models as one launch set (no reference
  *      counterpart: the reference runs one model per process,
  *      macau.cpp:94-148) ----
@@ -132,7 +169,9 @@ int macau_get_ftf_solver(macau_model* m, int mode);
 typedef struct macau_batch macau_batch;
 macau_batch* macau_batch_create(macau_model* const* models, int n);
 void macau_batch_free(macau_batch* b);
-int macau_batch_set_graph(macau_batch* b, int on); /* 0: launch the iterations without a graph */
+/* 1 (default): one graph per iteration of all models; 0: each model replays
+ * its own iteration graph on its own streams, all queued before any wait */
+int macau_batch_set_graph(macau_batch* b, int on);
 int macau_batch_iterate(macau_batch* b, int n);     /* n iterations of every model, asynchronous */
 int macau_batch_synchronize(macau_batch* b);        /* wait; raise sampler failures */
 

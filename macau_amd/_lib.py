@@ -13,6 +13,7 @@ _lib = None
 
 SAMPLER_PLAIN_FP32 = 0
 SAMPLER_BATCHED = 1
+SAMPLER_BATCHED_NOREFINE = 2  # macau_sample_latents only
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -66,6 +67,10 @@ SIGNATURES = {
     "macau_get_link_lambda": (_d, [_p, _i]),
     "macau_get_last_cg_iters": (_i, [_p, _i]),
     "macau_get_ftf_solver": (_i, [_p, _i]),
+    "macau_set_refine": (_i, [_p, _i, _d]),
+    "macau_get_refine_stats": (_i, [_p, _p, _p]),
+    "macau_autotune": (_i, [_p, _i, ctypes.c_char_p, _p, _p]),
+    "macau_get_tuning": (_i, [_p, _i, _p]),
     "macau_batch_create": (_p, [_p, _i]),
     "macau_batch_free": (None, [_p]),
     "macau_batch_set_graph": (_i, [_p, _i]),

@@ -9,7 +9,9 @@
 // so the instances' kernels are independent branches the GPU can run side by
 // side.  Each model keeps its own device state, draws and iteration counter,
 // so a model advanced in a batch gives the same chain as the same model
-// advanced alone.
+// advanced alone.  Without the batch graph (use_graph off) each model
+// replays its own iteration graph on its own streams, one after another on
+// the host and side by side on the device.
 #include "batch.h"
 
 namespace mgpu {
@@ -55,18 +57,39 @@ void ModelBatch::launch_all() {
 }
 
 void ModelBatch::iterate(int n) {
-  for (int i = 0; i < n; i++) {
-    if (use_graph) {
-      if (!exec) {
-        HIPCHK(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
-        launch_all();
-        HIPCHK(hipStreamEndCapture(root, &graph));
-        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  if (n > 0 && launched_any && use_graph != last_graph) {
+    // switching between the two launch forms: the new form's streams wait
+    // for the work the other form queued
+    for (size_t q = 0; q < models.size(); q++) {
+      if (use_graph) {
+        HIPCHK(hipEventRecord(ev_join[q], models[q]->stream));
+        HIPCHK(hipStreamWaitEvent(root, ev_join[q], 0));
+      } else {
+        HIPCHK(hipEventRecord(ev_fork[q], root));
+        HIPCHK(hipStreamWaitEvent(models[q]->stream, ev_fork[q], 0));
       }
-      HIPCHK(hipGraphLaunch(exec, root));
-    } else {
-      launch_all();
     }
+  }
+  if (n > 0) {
+    launched_any = true;
+    last_graph = use_graph;
+  }
+  if (!use_graph) {
+    // every model replays its own iteration (its own graph) on its own
+    // streams; nothing orders the models, so the hardware queues run them
+    // side by side
+    for (int i = 0; i < n; i++)
+      for (Model* m : models) m->iterate(1);
+    return;
+  }
+  for (int i = 0; i < n; i++) {
+    if (!exec) {
+      HIPCHK(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
+      launch_all();
+      HIPCHK(hipStreamEndCapture(root, &graph));
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(exec, root));
     for (Model* m : models) m->host_iter++;
   }
 }

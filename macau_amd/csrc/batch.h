@@ -23,6 +23,7 @@ class ModelBatch {
   std::vector<hipEvent_t> ev_fork, ev_join;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool launched_any = false, last_graph = true;
 };
 
 }  // namespace mgpu

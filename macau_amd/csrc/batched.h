@@ -126,6 +126,45 @@ inline void seg_fork(hipStream_t s, const SegStreams& seg) {
 inline void seg_record_join(const SegStreams& seg) { seg_check(hipEventRecord(seg.join, seg.stream)); }
 inline void seg_wait_join(hipStream_t s, const SegStreams& seg) { seg_check(hipStreamWaitEvent(s, seg.join, 0)); }
 
+// Condition estimate of one row after its Cholesky and, when it flags the
+// row, the row's entry in the refinement queue.  LANES lanes hold the row
+// (32: a half of the wave, base lane 0 or 32; 64: the whole wave); lane i
+// owns row i of L at Lrow (written by this lane) and inv_diag = 1 / L_ii.
+// The estimate is (max_i L_ii / min_i L_ii)^2 over the K real components.
+template <int KP, int LANES>
+__device__ __forceinline__ void refine_enqueue(const SampleArgs& a, int row, int nnz, int i, int base_lane,
+                                               float inv_diag, const float* Lrow) {
+  const RefineQueue& q = a.refine;
+  if (!q.count) return;
+  float mx = i < a.K ? inv_diag : 0.f;
+  float mn = i < a.K ? inv_diag : 3.0e38f;
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    mn = fminf(mn, __shfl_xor(mn, off, 64));
+  }
+  const float ratio = mx / mn;  // max L_ii / min L_ii
+  const bool flag = row >= 0 && nnz >= q.min_nnz && ratio * ratio > q.thresh;
+  int slot = -1;
+  if (flag && i == 0) {
+    slot = atomicAdd(q.count, 1);
+    if (slot < q.cap) q.row[slot] = row;
+  }
+  slot = __shfl(slot, base_lane, 64);
+  if (flag && slot >= 0 && slot < q.cap) {
+    float* dst = q.L + (size_t)slot * KP * KP + (size_t)i * KP;
+#pragma unroll
+    for (int k = 0; k < KP; k += 4) {
+      float4 v = *reinterpret_cast<const float4*>(Lrow + k);
+      v.x = k <= i ? v.x : 0.f;
+      v.y = k + 1 <= i ? v.y : 0.f;
+      v.z = k + 2 <= i ? v.z : 0.f;
+      v.w = k + 3 <= i ? v.w : 0.f;
+      *reinterpret_cast<float4*>(dst + k) = v;
+    }
+  }
+}
+
 // wave-sum of the SSE shares into sse_partial[slot]
 __device__ inline void write_sse(const SampleArgs& a, double acc, int slot) {
   if (!a.sse_partial) return;

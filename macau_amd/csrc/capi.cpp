@@ -412,11 +412,13 @@ int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr,
   a.iter = d_iter.as<int>();
   a.mode = mode;
   a.fail = d_fail.as<int>();
-  std::vector<DevBuf> pb;
-  if (sampler == SAMPLER_BATCHED) {
+  std::vector<DevBuf> pb, qb;
+  if (sampler == SAMPLER_BATCHED || sampler == MACAU_SAMPLER_BATCHED_NOREFINE) {
     if (!batched_supported(KP, nother)) throw std::runtime_error("batched sampler: unsupported latent size");
     if (!batched_fits(fs, nother, KP)) throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
     BatchPlan p = build_batch_plan(rp, KP, pb, s);
+    if (sampler == SAMPLER_BATCHED) a.refine = make_refine_queue(qb, KP, std::min(std::max(nrows, 1), kRefineCap),
+                                                                 kRefineThresh, K, s);
     launch_sample_batched(a, p, s);
   } else if (sampler == SAMPLER_PLAIN) {
     launch_sample_plain(a, s);
@@ -489,6 +491,54 @@ int macau_cond_normal_wishart(int K, int N, const double* U, const double* mu0, 
   HIPCHK(hipStreamSynchronize(s));
   if (fail) throw std::runtime_error("CondNormalWishart: matrix not positive definite");
   API_END_INT
+}
+
+int macau_set_refine(macau_model* m, int on, double threshold) {
+  API_BEGIN
+  Model& x = M(m);
+  x.refine = on != 0;
+  if (threshold > 0) x.refine_thresh = (float)threshold;
+  API_END_INT
+}
+
+int macau_get_refine_stats(macau_model* m, long long* flagged, long long* refined) {
+  API_BEGIN
+  Model& x = M(m);
+  long long t[2] = {0, 0};
+  int c = 0;
+  if (x.rq.count) {
+    HIPCHK(hipStreamSynchronize(x.stream));
+    HIPCHK(hipMemcpy(t, x.rq.totals, sizeof(t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&c, x.rq.count, sizeof(int), hipMemcpyDeviceToHost));
+  }
+  // the last half-sweep's queue is folded into the totals at the next one
+  if (flagged) *flagged = t[0] + c;
+  if (refined) *refined = t[1] + std::min(c, x.rq.cap);
+  API_END_INT
+}
+
+int macau_autotune(macau_model* m, int reps, const char* record_path, double* default_ms, double* tuned_ms) {
+  API_BEGIN
+  auto r = M(m).autotune(reps, record_path ? std::string(record_path) : std::string());
+  for (size_t q = 0; q < r.size(); q++) {
+    if (default_ms) default_ms[q] = r[q].first;
+    if (tuned_ms) tuned_ms[q] = r[q].second;
+  }
+  API_END_INT
+}
+
+int macau_get_tuning(macau_model* m, int mode, int* knobs) {
+  API_BEGIN
+  Model& x = M(m);
+  if (!x.initialized) throw std::runtime_error("model not initialized");
+  if (mode < 0 || mode >= x.nmodes) throw std::runtime_error("mode out of range");
+  const TuneChoice& c = x.tune[mode];
+  knobs[0] = c.seg_len;
+  knobs[1] = c.light_cap ? c.light_cap : kDefaultLightCap;
+  knobs[2] = c.heavy_cap ? c.heavy_cap : kDefaultHeavyCap;
+  knobs[3] = c.seg_cap ? c.seg_cap : kDefaultSegCap;
+  return c.tuned ? 1 : 0;
+  API_END_VAL(-1)
 }
 
 macau_batch* macau_batch_create(macau_model* const* models, int n) {

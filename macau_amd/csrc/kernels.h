@@ -25,6 +25,11 @@ struct FactorSet {
 };
 
 // Work list of the batched sampler (rows sorted by nnz, heavy rows split).
+constexpr int kDefaultSegLen = 2048;
+constexpr int kDefaultLightCap = 8192, kDefaultHeavyCap = 2048, kDefaultSegCap = 16384;
+// largest light / heavy caps the autotuner tries (SSE partial slots are sized for them)
+constexpr int kMaxLightCap = 16384, kMaxHeavyCap = 4096;
+
 struct BatchPlan {
   int KP = 0;                  // padded latent size the plan's partials are sized for
   int n_light = 0;             // light rows (sorted by nnz, descending)
@@ -42,7 +47,31 @@ struct BatchPlan {
   int n_pair = 0;
   const int4* pair_rng = nullptr;  // nnz ranges [x, y) and [z, w) (z = w = 0: no second row)
   const int2* pair_row = nullptr;  // row ids (y = -1: no second row)
+  // launch shape (0 = the defaults below); chosen by the autotuner
+  int seg_len = kDefaultSegLen;    // nonzeros per heavy-row segment (set when the plan is built)
+  int light_cap = 0;               // most work-groups of the light-row kernel (default 8192)
+  int heavy_cap = 0;               // most work-groups of the heavy-row finishing kernel (default 2048)
+  int seg_cap = 0;                 // most work-groups of the segment kernel (default 16384)
 };
+
+// Rows of a batched half-sweep queued for one refinement step (refine.hip):
+// a row whose pivot ratio (max_i L_ii / min_i L_ii)^2 exceeds thresh and
+// which has at least min_nnz nonzeros is written here with its FP32 factor.
+struct RefineQueue {
+  int* count = nullptr;         // rows queued in the current half-sweep (null: no refinement)
+  int cap = 0;                  // entries
+  int* row = nullptr;           // cap
+  float* L = nullptr;           // cap x KP x KP, row-major, zero above the diagonal
+  float thresh = 0.f;
+  int min_nnz = 0;
+  long long* totals = nullptr;  // [0] rows flagged, [1] rows refined, over the chain
+};
+constexpr int kRefineGrid = 256;
+constexpr int kRefineCap = 4096;
+// default threshold of the pivot-ratio estimate (max_i L_ii / min_i L_ii)^2:
+// measured split-BF16 row errors reach 1e-4 around 15-50 for rows with at
+// least K nonzeros
+constexpr float kRefineThresh = 10.f;
 
 struct SampleArgs {
   RowsDev rows;
@@ -64,7 +93,13 @@ struct SampleArgs {
   // diagonal and each row takes one coordinate-wise (Gauss-Seidel) sweep
   // from its current value instead of an exact draw
   int univariate = 0;
+  // batched sampler: the refinement queue (count == null: no refinement)
+  RefineQueue refine;
 };
+// refinement of the queued rows (refine.hip), launched after a batched
+// half-sweep's epilogues; the reset before them
+void launch_refine_reset(const SampleArgs& a, hipStream_t s);
+void launch_refine(const SampleArgs& a, hipStream_t s);
 
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
 // With seg_stream set, the heavy-row segment kernels run on it beside the
@@ -87,6 +122,7 @@ bool batched_fits(const FactorSet& f, int nother, int KP);
 // [light grid | heavy grid] when SampleArgs::sse_partial is set
 int batched_light_grid(const BatchPlan& p);
 int batched_heavy_grid(const BatchPlan& p);
+int batched_seg_grid(const BatchPlan& p);
 
 // --- hyperprior (FP64) ---
 // sums of x = U[:,n] - sub[:,n] (sub may be null): sum (K) and scatter

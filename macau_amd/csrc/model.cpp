@@ -335,7 +335,15 @@ void Model::init() {
   d_norm_partial.alloc(256 * sizeof(double));
   for (int m = 0; m < nmodes; m++) priors[m]->init(*this, m);
   noise->init(*this);
+  apply_cached_tuning();
   build_plans();
+  rq = RefineQueue();
+  refine_bufs.clear();
+  if (sampler == SAMPLER_BATCHED && refine) {
+    const int maxrows = *std::max_element(dims.begin(), dims.end());
+    rq = make_refine_queue(refine_bufs, KP, std::min(maxrows, kRefineCap), refine_thresh,
+                           refine_min_nnz < 0 ? num_latent : refine_min_nnz, stream);
+  }
   // the batched sampler computes the train SSE of the adaptive noise inside
   // the last half-sweep (u^T G u - 2 u^T c + sum r^2 per row)
   // (not when the last mode is univariate: that mode runs the plain sampler)
@@ -354,12 +362,31 @@ void Model::init() {
       r2[n] = (float)s;
     }
     upload(d_r2_last, r2, stream);
-    sse_partial_n = batched_light_grid(plans[last]) + batched_heavy_grid(plans[last]);
-    d_sse_partial.alloc((size_t)std::max(sse_partial_n, 1) * sizeof(double));
+    // slots for the largest launch shape the autotuner may choose
+    d_sse_partial.alloc((size_t)(kMaxLightCap + kMaxHeavyCap) * sizeof(double));
+    update_sse_slots();
   }
   HIPCHK(hipStreamSynchronize(stream));
   host_iter = 0;
   initialized = true;
+}
+
+RefineQueue make_refine_queue(std::vector<DevBuf>& B, int KP, int cap, float thresh, int min_nnz, hipStream_t s) {
+  B.clear();
+  B.resize(4);
+  upload(B[0], std::vector<int>{0}, s);
+  B[1].alloc((size_t)std::max(cap, 1) * sizeof(int));
+  B[2].alloc((size_t)std::max(cap, 1) * KP * KP * sizeof(float));
+  upload(B[3], std::vector<long long>{0, 0}, s);
+  RefineQueue q;
+  q.count = B[0].as<int>();
+  q.cap = cap;
+  q.row = B[1].as<int>();
+  q.L = B[2].as<float>();
+  q.thresh = thresh;
+  q.min_nnz = min_nnz;
+  q.totals = B[3].as<long long>();
+  return q;
 }
 
 void Model::set_iteration(int it) {
@@ -367,11 +394,13 @@ void Model::set_iteration(int it) {
   host_iter = it;
 }
 
-BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<DevBuf>& B, hipStream_t s) {
+BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<DevBuf>& B, hipStream_t s,
+                           int seg_len) {
   const int N = (int)row_ptr.size() - 1;
-  // rows sorted by nnz (descending); rows above the segment size are split
-  // (2048: fewer, longer segments measured faster than 1024 and 512)
-  const int seg = 2048;
+  // rows sorted by nnz (descending); rows above the segment length are split
+  // (default 2048: fewer, longer segments measured faster than 1024 and 512)
+  if (seg_len < 64 || seg_len % 64) throw std::runtime_error("segment length must be a positive multiple of 64");
+  const int seg = seg_len;
   std::vector<int> order(N);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -433,6 +462,7 @@ BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<
   p.n_pair = npair;
   p.pair_rng = reinterpret_cast<const int4*>(B[7].as<int>());
   p.pair_row = reinterpret_cast<const int2*>(B[8].as<int>());
+  p.seg_len = seg;
   HIPCHK(hipStreamSynchronize(s));
   return p;
 }
@@ -447,7 +477,8 @@ void Model::build_plans() {
       throw std::runtime_error("batched sampler: unsupported latent size / mode count");
     if (!batched_fits(others_of(m), nmodes - 1, KP))
       throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
-    plans[m] = build_batch_plan(rows[m].row_ptr, KP, plan_bufs[m], stream);
+    plans[m] = build_batch_plan(rows[m].row_ptr, KP, plan_bufs[m], stream, tune[m].seg_len);
+    apply_plan_shape(m);
   }
 }
 
@@ -470,6 +501,7 @@ SampleArgs Model::sample_args(int mode, const ILatentPrior& p) {
     a.sse_partial = d_sse_partial.as<double>();
     a.r2 = d_r2_last.as<float>();
   }
+  if (sampler == SAMPLER_BATCHED && refine) a.refine = rq;
   return a;
 }
 

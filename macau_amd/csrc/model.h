@@ -42,6 +42,13 @@ class DevBuf {
 
 enum SamplerKind { SAMPLER_PLAIN = 0, SAMPLER_BATCHED = 1 };
 
+// launch shape of the batched sampler for one mode (tune.cpp); 0 caps = defaults
+struct TuneChoice {
+  int seg_len = kDefaultSegLen, light_cap = 0, heavy_cap = 0, seg_cap = 0;
+  bool tuned = false;               // chosen by the autotuner (now or from the table)
+  double default_ms = 0, ms = 0;    // measured half-sweep times (default shape, chosen shape)
+};
+
 // Sparse rows of one mode on the host (SparseMode, sparsetensor.h:76-91).
 struct HostRows {
   std::vector<int> row_ptr;  // N + 1
@@ -78,7 +85,8 @@ int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const d
 
 // Work list of the batched sampler for one mode (rows sorted by nnz,
 // heavy rows split into segments).
-BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<DevBuf>& bufs, hipStream_t s);
+BatchPlan build_batch_plan(const std::vector<int>& row_ptr, int KP, std::vector<DevBuf>& bufs, hipStream_t s,
+                           int seg_len = kDefaultSegLen);
 
 // latentprior.h:17-49
 class ILatentPrior {
@@ -320,6 +328,22 @@ class Model {
 
   FactorSet others_of(int mode) const;
   SampleArgs sample_args(int mode, const ILatentPrior& p);
+  // batched sampler: one refinement step for rows flagged by the pivot-ratio
+  // condition estimate (refine.hip); settings read at init
+  bool refine = true;
+  float refine_thresh = kRefineThresh;
+  int refine_min_nnz = -1;  // < 0: num_latent
+  std::vector<DevBuf> refine_bufs;
+  RefineQueue rq;
+  // batched sampler launch shapes (tune.cpp): per mode, looked up at init
+  std::vector<TuneChoice> tune;
+  std::string tune_key(int mode) const;
+  void apply_cached_tuning();
+  void apply_plan_shape(int mode);
+  void update_sse_slots();
+  // times the candidate shapes of every mode, keeps the fastest, returns
+  // (default ms, chosen ms) per mode and appends the choices to record_path
+  std::vector<std::pair<double, double>> autotune(int reps, const std::string& record_path);
   void step_launch();  // enqueue one Gibbs iteration
   double factor_norm(int mode);
   double host_eval_out(int which);
@@ -327,6 +351,10 @@ class Model {
  private:
   void build_plans();
 };
+
+// Device queue of rows to refine, sized for cap rows of KP x KP factors
+// (bufs receives the allocations).
+RefineQueue make_refine_queue(std::vector<DevBuf>& bufs, int KP, int cap, float thresh, int min_nnz, hipStream_t s);
 
 // Setup-time copy of a host vector into a new allocation; synchronous, so
 // the host vector may be a temporary.

@@ -1,0 +1,158 @@
+// refine.hip -- one step of iterative refinement for the batched sampler's
+// ill-conditioned rows.
+//
+// The batched sampler factors P~ = Lambda + alpha G~ in FP32, with G~ the
+// split-BF16 Gram, and returns u0 = L~^{-T}(L~^{-1} b + z), i.e. the solution
+// of P~ u0 = b + L~ z.  For a row whose Cholesky pivots spread widely (the
+// cheap condition estimate (max_i L_ii / min_i L_ii)^2 above a threshold) the
+// rounding of G~ is amplified by the conditioning, so the epilogue queues
+// the row with its factor L~, and k_refine applies one refinement step to
+// the same system with the exact P = Lambda + alpha sum_e c_e c_e^T:
+//   r = b + L~ z - P u0
+//     = alpha sum_e c_e (y_e - mean - c_e . u0) + Lambda mu_n - Lambda u0 + L~ z,
+//   u1 = u0 + L~^{-T} L~^{-1} r.
+// The data term is accumulated from per-observation residuals (one pass over
+// the row's nonzeros; c_e . u0 in FP64, the sums in FP32 per thread and the
+// threads' sums combined in FP64),
+// so P is never formed and the residual carries no cancellation between b
+// and P u0.  The mean part of the sample (P^{-1} b) then carries the accuracy
+// of the residual instead of the split-BF16 Gram's; the noise part
+// (L^{-T} z) keeps the factor's.
+//
+// The train SSE fused into the last half-sweep keeps the unrefined sample of
+// a refined row (the change is second order in the correction).
+#include "batched.h"
+
+namespace mgpu {
+
+constexpr int kRefineThreads = 256;
+
+// before the epilogues of a half-sweep: fold the last half-sweep's counts
+// into the running totals and clear the queue
+__global__ void k_refine_reset(RefineQueue q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int c = *q.count;
+    q.totals[0] += c;
+    q.totals[1] += c < q.cap ? c : q.cap;
+    *q.count = 0;
+  }
+}
+
+template <int KP, int NO>
+__global__ void __launch_bounds__(kRefineThreads) k_refine(SampleArgs a) {
+  const RefineQueue& q = a.refine;
+  __shared__ float Ls[KP][KP + 1];
+  __shared__ float u0[KP], zs[KP], rs[KP];
+  __shared__ float part[kRefineThreads / 64][KP];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int it = *a.iter;
+  const double alpha = *a.alpha;
+  int nq = *q.count;
+  if (nq > q.cap) nq = q.cap;
+  for (int e = blockIdx.x; e < nq; e += gridDim.x) {
+    const int row = q.row[e];
+    const float* Lq = q.L + (size_t)e * KP * KP;
+    for (int x = t; x < KP * KP; x += kRefineThreads) Ls[x / KP][x % KP] = Lq[x];
+    if (t < KP) {
+      u0[t] = a.U[(size_t)row * KP + t];
+      zs[t] = t < a.K ? normal_f32(a.seed, (uint32_t)row, (uint32_t)t, (uint32_t)it, make_tag(TAG_LATENT, a.mode))
+                      : 0.f;
+    }
+    __syncthreads();
+    // data term: sum_e c_e (r_e - c_e . u0), one nonzero per thread per step
+    float acc[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) acc[k] = 0.f;
+    const int beg = a.rows.row_ptr[row], end = a.rows.row_ptr[row + 1];
+    for (int x = beg + t; x < end; x += kRefineThreads) {
+      float c[KP];
+      const float* f0 = a.others.f[0] + (size_t)a.rows.idx[(size_t)x * NO] * KP;
+#pragma unroll
+      for (int k = 0; k < KP; k += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(f0 + k);
+        c[k] = v.x;
+        c[k + 1] = v.y;
+        c[k + 2] = v.z;
+        c[k + 3] = v.w;
+      }
+#pragma unroll
+      for (int o = 1; o < NO; o++) {
+        const float* fo = a.others.f[o] + (size_t)a.rows.idx[(size_t)x * NO + o] * KP;
+#pragma unroll
+        for (int k = 0; k < KP; k += 4) {
+          const float4 v = *reinterpret_cast<const float4*>(fo + k);
+          c[k] *= v.x;
+          c[k + 1] *= v.y;
+          c[k + 2] *= v.z;
+          c[k + 3] *= v.w;
+        }
+      }
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < KP; k++) s = fma((double)c[k], (double)u0[k], s);
+      const float w = (float)((double)a.rows.r[x] - s);
+#pragma unroll
+      for (int k = 0; k < KP; k++) acc[k] = fmaf(c[k], w, acc[k]);
+    }
+    // combine the threads' sums: a wave sum per component, then the waves
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      float v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) part[wave][k] = v;
+    }
+    __syncthreads();
+    if (t < KP) {
+      double R = 0.0;
+#pragma unroll
+      for (int w = 0; w < kRefineThreads / 64; w++) R += (double)part[w][t];
+      // r = alpha R + Lambda mu_n - Lambda u0 + L~ z
+      const double lmu = a.Lmu_rows ? (double)a.Lmu_rows[(size_t)row * KP + t] : (double)a.Lmu_f[t];
+      double lu = 0.0, lz = 0.0;
+      for (int j = 0; j < KP; j++) lu += (double)a.Lambda_f[t * KP + j] * (double)u0[j];
+      for (int j = 0; j <= t; j++) lz += (double)Ls[t][j] * (double)zs[j];
+      rs[t] = t < a.K ? (float)(alpha * R + lmu - lu + lz) : 0.f;
+    }
+    __syncthreads();
+    // d = L~^{-T} L~^{-1} r on the first wave (lane = row), u1 = u0 + d
+    if (wave == 0) {
+      float y = lane < KP ? rs[lane] : 0.f;
+      for (int j = 0; j < KP; j++) {
+        const float yj = __shfl(lane == j ? y / Ls[j][j] : 0.f, j, 64);
+        if (lane == j) y = yj;
+        else if (lane > j && lane < KP) y -= Ls[lane][j] * yj;
+      }
+      for (int j = KP - 1; j >= 0; j--) {
+        const float dj = __shfl(lane == j ? y / Ls[j][j] : 0.f, j, 64);
+        if (lane == j) y = dj;
+        else if (lane < j) y -= Ls[j][lane] * dj;
+      }
+      if (lane < a.K) a.U[(size_t)row * KP + lane] = u0[lane] + y;
+    }
+    __syncthreads();
+  }
+}
+
+template <int KP>
+static void launch_refine_kp(const SampleArgs& a, hipStream_t s) {
+  const dim3 g(kRefineGrid), b(kRefineThreads);
+  switch (a.rows.nother) {
+    case 1: hipLaunchKernelGGL((k_refine<KP, 1>), g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_refine<KP, 2>), g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_refine<KP, 3>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_refine<KP, 4>), g, b, 0, s, a); break;
+    default: break;
+  }
+}
+
+void launch_refine_reset(const SampleArgs& a, hipStream_t s) {
+  if (a.refine.count) hipLaunchKernelGGL(k_refine_reset, dim3(1), dim3(64), 0, s, a.refine);
+}
+
+void launch_refine(const SampleArgs& a, hipStream_t s) {
+  if (!a.refine.count) return;
+  if (a.KP == 32) launch_refine_kp<32>(a, s);
+  else if (a.KP == 64) launch_refine_kp<64>(a, s);
+}
+
+}  // namespace mgpu

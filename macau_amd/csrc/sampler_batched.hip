@@ -204,7 +204,7 @@ struct PairLds {
 //      matrix B in lanes 32-63).
 // L rows go to LDS as they are produced, for the back solve.
 __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 accA, f32x16 accB, float bpA,
-                             float bpB, int it, float alpha, PairLds& s) {
+                             float bpB, int it, float alpha, PairLds& s, int nnzA, int nnzB) {
   int lane = threadIdx.x & 63;
   opaque(lane);
   const int half = lane >> 5;
@@ -329,6 +329,7 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
     b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
   }
   if (active) a.U[(size_t)row * 32 + i] = b;
+  refine_enqueue<32, 32>(a, active ? row : -1, half ? nnzB : nnzA, i, half * 32, inv_diag, &s.Lf[half][i][0]);
   if (!a.sse_partial) return 0.f;
   // train SSE share of this lane: (w^2 - u (Lambda u)) / alpha - 2 u c (+ sum r^2)
   s.us[half][i] = b;
@@ -389,7 +390,8 @@ __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
     float bpA = 0.f, bpB = 0.f;
     GramStream<NO, true>(a, F, blk, rng.x, rng.y, rng.z, rng.w).run(b0, b1, accA, bpA, accB, bpB);
     if (qn < p.n_pair) GramStream<NO, true>::first_blocks(a, rng_n, lane, b0, b1);
-    sse += (double)finish_pair(a, rows.x, rows.y, accA, accB, bpA, bpB, it, alpha, s);
+    sse += (double)finish_pair(a, rows.x, rows.y, accA, accB, bpA, bpB, it, alpha, s, rng.y - rng.x,
+                               rng.w - rng.z);
     rng = rng_n;
     rows = rows_n;
   }
@@ -445,19 +447,25 @@ __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, in
     const int rowB = (2 * q + 1 < p.n_heavy) ? p.heavy[2 * q + 1] : -1;
     slot_partial(p, 2 * q, lane, accA, bpA);
     if (rowB >= 0) slot_partial(p, 2 * q + 1, lane, accB, bpB);
-    sse += (double)finish_pair(a, rowA, rowB, accA, accB, bpA, bpB, it, alpha, s);
+    sse += (double)finish_pair(a, rowA, rowB, accA, accB, bpA, bpB, it, alpha, s, p.seg_len, p.seg_len);
   }
   write_sse(a, sse, sse_slot0 + blockIdx.x);
 }
 
 int batched_light_grid(const BatchPlan& p) {
   if (p.KP == 64) return batched64_light_grid(p);
-  return p.n_pair < 8192 ? p.n_pair : 8192;
+  const int cap = p.light_cap > 0 ? p.light_cap : kDefaultLightCap;
+  return p.n_pair < cap ? p.n_pair : cap;
 }
 int batched_heavy_grid(const BatchPlan& p) {
   if (p.KP == 64) return batched64_heavy_grid(p);
   const int npairs = (p.n_heavy + 1) / 2;
-  return npairs < 2048 ? npairs : 2048;
+  const int cap = p.heavy_cap > 0 ? p.heavy_cap : kDefaultHeavyCap;
+  return npairs < cap ? npairs : cap;
+}
+int batched_seg_grid(const BatchPlan& p) {
+  const int cap = p.seg_cap > 0 ? p.seg_cap : kDefaultSegCap;
+  return p.n_seg < cap ? p.n_seg : cap;
 }
 
 template <int NO>
@@ -467,14 +475,16 @@ static void launch_batched_t(const SampleArgs& a, const BatchPlan& p, hipStream_
   if (p.n_seg > 0) {
     const hipStream_t ss = fork ? seg.stream : s;
     if (fork) seg_fork(s, seg);
-    const int gs = p.n_seg < 16384 ? p.n_seg : 16384;
+    const int gs = batched_seg_grid(p);
     hipLaunchKernelGGL((k_seg32<NO>), dim3(gs), dim3(64), 0, ss, a, p);
     launch_seg_sum<kPartial>(p, ss);
     if (fork) seg_record_join(seg);
   }
+  launch_refine_reset(a, s);
   if (gl > 0) hipLaunchKernelGGL((k_light32<NO>), dim3(gl), dim3(64), 0, s, a, p);
   if (fork) seg_wait_join(s, seg);
   if (gh > 0) hipLaunchKernelGGL(k_heavy32, dim3(gh), dim3(64), 0, s, a, p, gl);
+  launch_refine(a, s);
 }
 
 void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {

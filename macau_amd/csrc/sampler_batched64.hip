@@ -188,7 +188,7 @@ struct Fin64Lds {
 
 // Finish one row whose Gram is in g (row < 0: no row); returns this lane's
 // SSE share.  Lane i owns row i of P.
-__device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float alpha, Fin64Lds& s) {
+__device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float alpha, Fin64Lds& s, int nnz) {
   int lane = threadIdx.x & 63;
   opaque(lane);
   const int half = lane >> 5, c32 = lane & 31;
@@ -316,6 +316,7 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
     b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
   }
   if (active) a.U[(size_t)row * 64 + i] = b;
+  refine_enqueue<64, 64>(a, active ? row : -1, nnz, i, 0, inv_diag, &s.Lf[i][0]);
   if (!a.sse_partial) return 0.f;
   s.us[i] = b;
   lds_wait();
@@ -367,7 +368,7 @@ __global__ void __launch_bounds__(64, 2) k_light64(SampleArgs a, BatchPlan p) {
     zero_gram(g);
     GramStream64<NO>(a, F, blk, rng.x, rng.y).run(f0, f1, g);
     if (qn < p.n_pair) GramStream64<NO>::first_blocks(a, rng_n.x, rng_n.y, lane, f0, f1);
-    sse += (double)finish64(a, row, g, it, alpha, s);
+    sse += (double)finish64(a, row, g, it, alpha, s, rng.y - rng.x);
     rng = rng_n;
     row = row_n;
   }
@@ -415,13 +416,19 @@ __global__ void __launch_bounds__(64, 2) k_heavy64(SampleArgs a, BatchPlan p, in
     }
     g.b0 = in[3072 + lane];
     g.b1 = in[3136 + lane];
-    sse += (double)finish64(a, p.heavy[q], g, it, alpha, s);
+    sse += (double)finish64(a, p.heavy[q], g, it, alpha, s, p.seg_len);
   }
   write_sse(a, sse, sse_slot0 + blockIdx.x);
 }
 
-int batched64_light_grid(const BatchPlan& p) { return p.n_pair < 8192 ? p.n_pair : 8192; }
-int batched64_heavy_grid(const BatchPlan& p) { return p.n_heavy < 2048 ? p.n_heavy : 2048; }
+int batched64_light_grid(const BatchPlan& p) {
+  const int cap = p.light_cap > 0 ? p.light_cap : kDefaultLightCap;
+  return p.n_pair < cap ? p.n_pair : cap;
+}
+int batched64_heavy_grid(const BatchPlan& p) {
+  const int cap = p.heavy_cap > 0 ? p.heavy_cap : kDefaultHeavyCap;
+  return p.n_heavy < cap ? p.n_heavy : cap;
+}
 
 template <int NO>
 static void launch64_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {
@@ -430,14 +437,16 @@ static void launch64_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s, c
   if (p.n_seg > 0) {
     const hipStream_t ss = fork ? seg.stream : s;
     if (fork) seg_fork(s, seg);
-    const int gs = p.n_seg < 16384 ? p.n_seg : 16384;
+    const int gs = batched_seg_grid(p);
     hipLaunchKernelGGL((k_seg64<NO>), dim3(gs), dim3(64), 0, ss, a, p);
     launch_seg_sum<kPartial64>(p, ss);
     if (fork) seg_record_join(seg);
   }
+  launch_refine_reset(a, s);
   if (gl > 0) hipLaunchKernelGGL((k_light64<NO>), dim3(gl), dim3(64), 0, s, a, p);
   if (fork) seg_wait_join(s, seg);
   if (gh > 0) hipLaunchKernelGGL(k_heavy64, dim3(gh), dim3(64), 0, s, a, p, gl);
+  launch_refine(a, s);
 }
 
 void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg) {
