@@ -199,7 +199,7 @@ int macau_profile(macau_model* h, int n, double* phase_ms) {
 int macau_sampler_supported(int kind, int num_latent, int nmodes) {
   if (num_latent < 1 || num_latent > 128 || nmodes < 2 || nmodes > kMaxOther + 1) return 0;
   if (kind == SAMPLER_PLAIN) return 1;
-  if (kind == SAMPLER_BATCHED) return batched_supported(pad_latent(num_latent), nmodes - 1);
+  if (kind == SAMPLER_BATCHED) return batched_supported(pad_latent_batched(num_latent), nmodes - 1);
   return 0;
 }
 int macau_get_iteration(macau_model* h) {
@@ -333,8 +333,9 @@ int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr,
                          int iter, int mode, int sampler, double* U_out) {
   API_BEGIN
   if (K < 1 || K > 128) throw std::runtime_error("num_latent must be in [1, 128]");
-  if (nother < 1 || nother > kMaxOther) throw std::runtime_error("nother must be in [1, 4]");
-  const int KP = pad_latent(K);
+  if (nother < 1 || nother > kMaxOther) throw std::runtime_error("nother must be in [1, 9]");
+  const bool batched = sampler == SAMPLER_BATCHED || sampler == MACAU_SAMPLER_BATCHED_NOREFINE;
+  const int KP = batched ? pad_latent_batched(K) : pad_latent(K);
   const long long nnz = row_ptr[nrows];
   if (nnz > 0x7fffffffLL) throw std::runtime_error("too many nonzeros for one mode");
   hipStream_t s;
@@ -413,7 +414,7 @@ int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr,
   a.mode = mode;
   a.fail = d_fail.as<int>();
   std::vector<DevBuf> pb, qb;
-  if (sampler == SAMPLER_BATCHED || sampler == MACAU_SAMPLER_BATCHED_NOREFINE) {
+  if (batched) {
     if (!batched_supported(KP, nother)) throw std::runtime_error("batched sampler: unsupported latent size");
     if (!batched_fits(fs, nother, KP)) throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
     BatchPlan p = build_batch_plan(rp, KP, pb, s);

@@ -179,5 +179,8 @@ inline int pad_latent(int K) {
   if (K <= 64) return 64;
   return 128;
 }
+// The batched sampler's kernels tile 32 or 64 components: smaller latent
+// sizes are stored padded to 32 (the padding components are zero).
+inline int pad_latent_batched(int K) { return K <= 32 ? 32 : pad_latent(K); }
 
 }  // namespace mgpu

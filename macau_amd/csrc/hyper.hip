@@ -755,13 +755,15 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
   // (U <= G: every cell needs an owner lane in its group)
   constexpr int U = G < 4 ? G : 4;
   double se = 0.0, se_avg = 0.0;
+  // NM modes (NM = kMaxOther + 1: nm = the run-time count, at most NM)
+  const int nm = NM <= kMaxOther ? NM : a.nmodes;
   for (int q0 = blockIdx.x * cells_per_block * U; q0 < a.ntest; q0 += gridDim.x * cells_per_block * U) {
     int qs[U], ix[U][NM];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       qs[u] = q0 + (threadIdx.x / G) * U + u;
 #pragma unroll
-      for (int m = 0; m < NM; m++) ix[u][m] = qs[u] < a.ntest ? a.idx[(size_t)qs[u] * NM + m] : -1;
+      for (int m = 0; m < NM; m++) ix[u][m] = (qs[u] < a.ntest && m < nm) ? a.idx[(size_t)qs[u] * nm + m] : -1;
     }
     float dot[U];
 #pragma unroll
@@ -771,6 +773,7 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval(EvalArgs a) {
         float4 v = reinterpret_cast<const float4*>(a.f[0] + (size_t)ix[u][0] * KP)[sub];
 #pragma unroll
         for (int m = 1; m < NM; m++) {
+          if (m >= nm) break;
           const float4 w = reinterpret_cast<const float4*>(a.f[m] + (size_t)ix[u][m] * KP)[sub];
           v.x *= w.x;
           v.y *= w.y;
@@ -869,7 +872,8 @@ static void launch_eval_k(const EvalArgs& a, hipStream_t s) {
     case 2: hipLaunchKernelGGL((k_eval<KP, 2, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
     case 3: hipLaunchKernelGGL((k_eval<KP, 3, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
     case 4: hipLaunchKernelGGL((k_eval<KP, 4, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_eval<KP, 5, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_eval<KP, 5, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_eval<KP, kMaxOther + 1, false>), dim3(a.nblk), dim3(kEvalThreads), 0, s, a); break;
   }
 }
 

@@ -7,7 +7,10 @@
 
 namespace mgpu {
 
-constexpr int kMaxOther = 4;  // up to 5-mode tensors
+constexpr int kMaxOther = 9;  // up to 10-mode tensors (the reference's Data takes 10 index columns)
+// kernels are compiled for 1..4 other modes; 5..kMaxOther use one variant
+// compiled for kMaxOther with the count read at run time
+constexpr int kMaxBatchedOther = 4;  // the batched sampler: up to 5-mode tensors
 
 // Observed cells of one mode, grouped by that mode's index (the reference's
 // SparseMode / Eigen CSC column view, sparsetensor.h:76-91).
@@ -20,8 +23,8 @@ struct RowsDev {
 };
 
 struct FactorSet {
-  const float* f[kMaxOther] = {nullptr, nullptr, nullptr, nullptr};  // KP x N_m
-  int n[kMaxOther] = {0, 0, 0, 0};                                    // N_m
+  const float* f[kMaxOther] = {};  // KP x N_m
+  int n[kMaxOther] = {};           // N_m
 };
 
 // Work list of the batched sampler (rows sorted by nnz, heavy rows split).
@@ -193,7 +196,7 @@ struct EvalArgs {
   int ntest = 0, nmodes = 2, K = 0, KP = 0;
   const int* idx = nullptr;      // ntest x nmodes
   const double* vals = nullptr;  // ntest
-  const float* f[kMaxOther + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  const float* f[kMaxOther + 1] = {};
   double mean = 0;
   double* pred = nullptr;
   double* pred_var = nullptr;

@@ -219,7 +219,7 @@ Model::Model(int nm, int K, std::unique_ptr<INoiseModel> nz)
     : nmodes(nm), num_latent(K), KP(pad_latent(K)), noise(std::move(nz)) {
   is_tensor = nm != 2;
   if (K < 1 || K > 128) throw std::runtime_error("num_latent must be in [1, 128]");
-  if (nm < 2 || nm > kMaxOther + 1) throw std::runtime_error("number of modes must be in [2, 5]");
+  if (nm < 2 || nm > kMaxOther + 1) throw std::runtime_error("number of modes must be in [2, 10]");
 }
 
 Model::~Model() {
@@ -259,6 +259,7 @@ void Model::init() {
   if ((int)priors.size() != nmodes)
     throw std::runtime_error("number of priors must equal the number of data modes");
   if (rows.empty()) throw std::runtime_error("training data not set");
+  KP = sampler == SAMPLER_BATCHED ? pad_latent_batched(num_latent) : pad_latent(num_latent);
   if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   if (!side) HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   if (!seg_stream) HIPCHK(hipStreamCreateWithFlags(&seg_stream, hipStreamNonBlocking));

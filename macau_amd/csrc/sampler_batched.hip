@@ -38,7 +38,9 @@ namespace mgpu {
 
 constexpr int kPartial = 1024 + 64;  // floats per segment partial: fragment + RHS lanes
 
-int batched_supported(int KP, int nother) { return (KP == 32 || KP == 64) && nother >= 1 && nother <= kMaxOther; }
+int batched_supported(int KP, int nother) {
+  return (KP == 32 || KP == 64) && nother >= 1 && nother <= kMaxBatchedOther;
+}
 
 bool batched_fits(const FactorSet& f, int nother, int KP) {
   for (int t = 0; t < nother; t++)

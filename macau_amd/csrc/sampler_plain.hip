@@ -31,12 +31,14 @@
 
 namespace mgpu {
 
+// NO other modes (NO = kMaxOther: no = the run-time count, at most NO)
 template <int NO>
 __device__ __forceinline__ float gather_comp(const FactorSet& F, const int* __restrict__ idx, int e, int i,
-                                             int KP) {
-  float x = F.f[0][(size_t)idx[(size_t)e * NO] * KP + i];
+                                             int KP, int no) {
+  float x = F.f[0][(size_t)idx[(size_t)e * no] * KP + i];
 #pragma unroll
-  for (int t = 1; t < NO; t++) x *= F.f[t][(size_t)idx[(size_t)e * NO + t] * KP + i];
+  for (int t = 1; t < NO; t++)
+    if (t < no) x *= F.f[t][(size_t)idx[(size_t)e * no + t] * KP + i];
   return x;
 }
 
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(KP < 64 ? 64 : KP) k_sample_plain(SampleArgs a
       float x = 0.f, rr = 0.f;
       if (base + c < nnz) {
         const int e = beg + base + c;
-        x = gather_comp<NO>(a.others, a.rows.idx, e, i, KP);
+        x = gather_comp<NO>(a.others, a.rows.idx, e, i, KP, NO < kMaxOther ? NO : a.rows.nother);
         rr = a.rows.r[e];
       }
       vbuf[g][c][i] = x;
@@ -238,7 +240,9 @@ static void launch_plain_k(const SampleArgs& a, hipStream_t s) {
     case 2: return launch_plain_t<KP, 2>(a, s);
     case 3: return launch_plain_t<KP, 3>(a, s);
     case 4: return launch_plain_t<KP, 4>(a, s);
-    default: break;
+    default:
+      if (a.rows.nother <= kMaxOther) return launch_plain_t<KP, kMaxOther>(a, s);
+      break;
   }
 }
 

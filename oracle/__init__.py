@@ -28,6 +28,7 @@ TAG_LATENT, TAG_WISHART_Z, TAG_WISHART_CHI, TAG_NW_MU = 1, 2, 3, 4
 TAG_NOISE, TAG_BETA_U, TAG_BETA_F, TAG_LAMBDA_BETA = 5, 6, 7, 8
 TAG_PROBIT = 9
 TAG_BETA_ONE = 10
+MAX_MODES = 10  # ORC_MAX_MODES
 
 
 def tag(stream, mode=0):
@@ -52,17 +53,17 @@ class OrcCsr(ctypes.Structure):
 class OrcRunCfg(ctypes.Structure):
     _fields_ = [
         ("nmodes", ctypes.c_int), ("K", ctypes.c_int), ("is_tensor", ctypes.c_int),
-        ("dims", _i64 * 8), ("train", OrcRows * 8), ("mean", _dbl), ("nnz", _i64),
+        ("dims", _i64 * MAX_MODES), ("train", OrcRows * MAX_MODES), ("mean", _dbl), ("nnz", _i64),
         ("ntest", _i64), ("test_idx", _p), ("test_vals", _p),
         ("noise_adaptive", ctypes.c_int), ("precision", _dbl), ("sn_init", _dbl), ("sn_max", _dbl),
-        ("macau", ctypes.c_int * 8), ("F", OrcCsr * 8), ("Ft", OrcCsr * 8),
+        ("macau", ctypes.c_int * MAX_MODES), ("F", OrcCsr * MAX_MODES), ("Ft", OrcCsr * MAX_MODES),
         ("lambda_beta0", _dbl), ("tol", _dbl), ("burnin", ctypes.c_int), ("nsamples", ctypes.c_int),
         ("seed", _u64), ("max_iters", ctypes.c_int),
-        ("pred", _p), ("pred_var", _p), ("U_out", _p * 8), ("rmse", _dbl), ("rmse1", _dbl),
+        ("pred", _p), ("pred_var", _p), ("U_out", _p * MAX_MODES), ("rmse", _dbl), ("rmse1", _dbl),
         ("alpha", _dbl), ("alpha_trace", _p), ("rmse_trace", _p), ("lambda_beta_out", _p),
-        ("cg_iters_last", ctypes.c_int), ("use_ftf", ctypes.c_int * 8),
-        ("noise_probit", ctypes.c_int), ("auc", _dbl), ("auc1", _dbl), ("uni", ctypes.c_int * 8),
-        ("mu_out", _p * 8), ("Lambda_out", _p * 8),
+        ("cg_iters_last", ctypes.c_int), ("use_ftf", ctypes.c_int * MAX_MODES),
+        ("noise_probit", ctypes.c_int), ("auc", _dbl), ("auc1", _dbl), ("uni", ctypes.c_int * MAX_MODES),
+        ("mu_out", _p * MAX_MODES), ("Lambda_out", _p * MAX_MODES),
     ]
 
 
@@ -615,7 +616,7 @@ def run(train_modes, mean, nnz, dims, K, test_idx, test_vals, *, is_tensor=False
     total = burnin + nsamples if max_iters <= 0 else min(max_iters, burnin + nsamples)
     at = np.zeros(max(total, 1))
     rt = np.zeros(max(total, 1))
-    lb = np.zeros(8)
+    lb = np.zeros(MAX_MODES)
     cfg.pred, cfg.pred_var, cfg.alpha_trace, cfg.rmse_trace = _ptr(pred), _ptr(pvar), _ptr(at), _ptr(rt)
     cfg.lambda_beta_out = _ptr(lb)
     Us, mus, Lams = [], [], []

@@ -1080,18 +1080,19 @@ void orc_eval_probit(int K, int64_t ntest, const int32_t* test_idx, const double
 /* ------------------------------------------------------------------ */
 int orc_run(orc_run_cfg* c) {
   const int K = c->K, M = c->nmodes;
+  if (M < 2 || M > ORC_MAX_MODES) return -5;
   const size_t KK = (size_t)K * K;
-  double* U[8] = {0};
-  double* mu[8] = {0};
-  double* Lambda[8] = {0};
-  double* beta[8] = {0};
-  double* Uhat[8] = {0};
-  double* murows[8] = {0};
-  double* FtF[8] = {0};
-  double* lam_uni[8] = {0};   /* MacauOnePrior: diagonal prior precision */
-  double* lb_uni[8] = {0};    /* MacauOnePrior: per-latent lambda_beta */
-  double* colsq[8] = {0};     /* MacauOnePrior: squared column norms of F */
-  double lambda_beta[8];
+  double* U[ORC_MAX_MODES] = {0};
+  double* mu[ORC_MAX_MODES] = {0};
+  double* Lambda[ORC_MAX_MODES] = {0};
+  double* beta[ORC_MAX_MODES] = {0};
+  double* Uhat[ORC_MAX_MODES] = {0};
+  double* murows[ORC_MAX_MODES] = {0};
+  double* FtF[ORC_MAX_MODES] = {0};
+  double* lam_uni[ORC_MAX_MODES] = {0};   /* MacauOnePrior: diagonal prior precision */
+  double* lb_uni[ORC_MAX_MODES] = {0};    /* MacauOnePrior: per-latent lambda_beta */
+  double* colsq[ORC_MAX_MODES] = {0};     /* MacauOnePrior: squared column norms of F */
+  double lambda_beta[ORC_MAX_MODES];
   double* WI = (double*)calloc(KK, sizeof(double));
   int rc = 0;
   for (int i = 0; i < K; i++) WI[i + i * K] = 1.0;
@@ -1166,7 +1167,7 @@ int orc_run(orc_run_cfg* c) {
     }
 #endif
     for (int m = 0; m < M; m++) {
-      const double* others[8];
+      const double* others[ORC_MAX_MODES];
       int t = 0;
       for (int q = 0; q < M; q++)
         if (q != m) others[t++] = U[q];
@@ -1192,7 +1193,7 @@ int orc_run(orc_run_cfg* c) {
       if (f) { rc = -1; goto done; }
 #if ORC_MUTANT == 7 /* planted mistake (tests only): noise precision updated between the modes */
       if (m == 0 && c->noise_adaptive && !c->noise_probit) {
-        const double* Us[8];
+        const double* Us[ORC_MAX_MODES];
         for (int q = 0; q < M; q++) Us[q] = U[q];
         alpha = orc_noise_alpha(orc_train_sse(K, M, &c->train[0], c->mean, Us), c->nnz, var_total, c->is_tensor,
                                 alpha_max, c->seed, (uint32_t)i);
@@ -1220,13 +1221,13 @@ int orc_run(orc_run_cfg* c) {
       if (rc) { rc = -2; goto done; }
     }
     if (c->noise_adaptive && !c->noise_probit && ORC_MUTANT != 7) {
-      const double* Us[8];
+      const double* Us[ORC_MAX_MODES];
       for (int m = 0; m < M; m++) Us[m] = U[m];
       double sse = orc_train_sse(K, M, &c->train[0], c->mean, Us);
       alpha = orc_noise_alpha(sse, c->nnz, var_total, c->is_tensor, alpha_max, c->seed, (uint32_t)i);
     }
     {
-      const double* Us[8];
+      const double* Us[ORC_MAX_MODES];
       for (int m = 0; m < M; m++) Us[m] = U[m];
       if (c->noise_probit)
         orc_eval_probit(K, c->ntest, c->test_idx, c->test_vals, Us, i < c->burnin ? 0 : i - c->burnin, c->pred,

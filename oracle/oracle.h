@@ -203,10 +203,11 @@ int orc_macau_one_update_prior(int K, int mode, const double* U, int64_t N, cons
                                double* lambda_beta, uint64_t seed, uint32_t it);
 
 /* ---- a4: full Gibbs run (macau.cpp:72-86, 94-148) ---- */
+#define ORC_MAX_MODES 10  /* the reference's Data takes up to 10 index columns (macau.pyx:283-284) */
 typedef struct {
   int      nmodes, K, is_tensor;
-  int64_t  dims[8];
-  orc_rows train[8];       /* per-mode rows, raw values                      */
+  int64_t  dims[ORC_MAX_MODES];
+  orc_rows train[ORC_MAX_MODES];       /* per-mode rows, raw values                      */
   double   mean;
   int64_t  nnz;            /* train cells (after duplicate merging)         */
   int64_t  ntest;
@@ -214,8 +215,8 @@ typedef struct {
   const double*  test_vals;
   int      noise_adaptive;
   double   precision, sn_init, sn_max;
-  int      macau[8];       /* 1 = MacauPrior with sparse side information   */
-  orc_csr  F[8], Ft[8];
+  int      macau[ORC_MAX_MODES];       /* 1 = MacauPrior with sparse side information   */
+  orc_csr  F[ORC_MAX_MODES], Ft[ORC_MAX_MODES];
   double   lambda_beta0, tol;
   int      burnin, nsamples;
   uint64_t seed;
@@ -223,14 +224,14 @@ typedef struct {
   /* outputs */
   double*  pred;           /* ntest */
   double*  pred_var;       /* ntest */
-  double*  U_out[8];       /* K x dims[m] each, or NULL                     */
+  double*  U_out[ORC_MAX_MODES];       /* K x dims[m] each, or NULL                     */
   double   rmse, rmse1, alpha;
   double*  alpha_trace;    /* per-iteration noise precision, or NULL         */
   double*  rmse_trace;     /* per-iteration posterior-mean RMSE, or NULL     */
   double*  lambda_beta_out;/* per mode, or NULL                              */
   int      cg_iters_last;
   /* input: per mode, solve beta through F^T F (1) or the block CG (0) */
-  int      use_ftf[8];
+  int      use_ftf[ORC_MAX_MODES];
   /* input: probit noise (matrix data; overrides precision / adaptive) */
   int      noise_probit;
   /* outputs of probit runs (rmse / rmse1 stay NaN) */
@@ -238,11 +239,11 @@ typedef struct {
   /* input: per mode with side information, the univariate prior
    * MacauOnePrior (1) instead of MacauPrior (0); lambda_beta_out then
    * receives the mean of its per-latent lambda_beta */
-  int      uni[8];
+  int      uni[ORC_MAX_MODES];
   /* outputs: per mode mu (K) and Lambda (K x K col-major) after the last
    * iteration, or NULL */
-  double*  mu_out[8];
-  double*  Lambda_out[8];
+  double*  mu_out[ORC_MAX_MODES];
+  double*  Lambda_out[ORC_MAX_MODES];
 } orc_run_cfg;
 int orc_run(orc_run_cfg* cfg);
 

@@ -383,3 +383,32 @@ def test_save_model_contents(tmp_path):
     assert text("-sample2-U1-link.csv") == expect(beta.T)
     assert not os.path.exists(prefix + "-sample2-U2-link.csv")
     assert os.path.exists(prefix + "-sample1-U1-link.csv")
+
+
+@pytest.mark.parametrize("nmodes", [7, 10])
+def test_api_tensor_many_modes_matches_oracle(nmodes):
+    """DataFrames with up to 10 index columns (macau.pyx:283-284): the
+    predictions of a short chain equal the oracle's (plain sampler; the batched
+    sampler covers up to 5 modes)."""
+    import oracle as O
+    announce("api tensor with %d modes" % nmodes)
+    rng = np.random.default_rng(80 + nmodes)
+    dims = [12, 9, 7, 6, 5, 4, 4, 3, 3, 2][:nmodes]
+    cols = ["m%d" % m for m in range(nmodes)]
+    idx = np.unique(np.stack([rng.integers(0, d, 5000) for d in dims], axis=1), axis=0)
+    idx = idx[rng.permutation(len(idx))]
+    F = [rng.standard_normal((d, 3)) * 0.9 for d in dims]
+    vals = np.prod([F[m][idx[:, m]] for m in range(nmodes)], axis=0).sum(1) + 0.1 * rng.standard_normal(len(idx))
+    te, tr = np.arange(400), np.arange(400, len(idx))
+    df = pd.DataFrame(idx, columns=cols)
+    df["value"] = vals
+    Ytr, Yte = df.iloc[tr], df.iloc[te]
+    assert all(Ytr[c].max() + 1 == d for c, d in zip(cols, dims))
+    K, burnin, nsamples = 4, 2, 3
+    res = macau_amd.bpmf(Ytr, Ytest=Yte, num_latent=K, precision=10.0, burnin=burnin, nsamples=nsamples,
+                         verbose=False, seed=45)
+    modes, mean, nnz = O.tensor_rows(idx[tr], vals[tr], list(dims))
+    ti, tv = O.tensor_test(idx[te], vals[te], list(dims))
+    ref = O.run(modes, mean, nnz, list(dims), K, ti, tv, is_tensor=True, precision=10.0, burnin=burnin,
+                nsamples=nsamples, seed=45)
+    _oracle_compare(res, ti, tv, ref, nsamples, cols)

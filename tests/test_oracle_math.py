@@ -355,3 +355,37 @@ def test_full_run_ftf_branch_matches_cg_branch():
     assert a.cg_iters_last == 0 and b.cg_iters_last > 0
     np.testing.assert_allclose(a.U[0], b.U[0], rtol=1e-6, atol=1e-8)
     assert a.lambda_beta[0] == pytest.approx(b.lambda_beta[0], rel=1e-6)
+
+
+def test_sampler_with_six_other_modes_matches_textbook():
+    """Rows of a 7-mode tensor: the Hadamard product of six other factors."""
+    rng = np.random.default_rng(70)
+    K = 5
+    dims = [12, 6, 5, 4, 3, 3, 2]
+    idx = np.stack([rng.integers(0, d, 400) for d in dims], axis=1)
+    vals = rng.standard_normal(400)
+    modes, mean, _ = O.tensor_rows(idx, vals, dims)
+    fs = [rng.standard_normal((d, K)) * 0.8 for d in dims]
+    Lam, mu = _spd(rng, K, 2.0), rng.standard_normal(K)
+    for mode in (0, 3, 6):
+        others = [fs[q] for q in range(7) if q != mode]
+        U = O.sample_latents(K, modes[mode], mean, others, 2.0, mu, Lam, SEED, 5, mode)
+        ref = _textbook_rows(K, modes[mode], mean, others, 2.0, mu, Lam, 5, mode)
+        np.testing.assert_allclose(U, ref, rtol=1e-10, atol=1e-12)
+
+
+def test_run_ten_mode_tensor():
+    """The reference's Data takes up to 10 index columns (macau.pyx:283-284):
+    a 10-mode CP tensor is fitted (test RMSE well below the data's sd)."""
+    rng = np.random.default_rng(71)
+    dims = [8, 6, 5, 4, 4, 3, 3, 3, 2, 2]
+    F = [rng.standard_normal((d, 1)) for d in dims]
+    idx = np.stack([rng.integers(0, d, 4000) for d in dims], axis=1)
+    idx = np.unique(idx, axis=0)
+    val = np.prod([F[m][idx[:, m], 0] for m in range(10)], axis=0)
+    perm = rng.permutation(len(idx))
+    te, tr = perm[:300], perm[300:]
+    modes, mean, nnz = O.tensor_rows(idx[tr], val[tr], dims)
+    ti, tv = O.tensor_test(idx[te], val[te], dims)
+    res = O.run(modes, mean, nnz, dims, 2, ti, tv, is_tensor=True, precision=50.0, burnin=20, nsamples=20, seed=3)
+    assert res.rmse < 0.5 * np.std(val)
