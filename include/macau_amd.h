@@ -126,8 +126,10 @@ int macau_get_ftf_solver(macau_model* m, int mode);
  *      counterpart: it guards the reference's FP64 solve,
  *      latentprior.cpp:373-384, against the FP32 / split-BF16 arithmetic) ----
  * After each row's Cholesky the batched sampler estimates the row's
- * conditioning from its pivots, (max_i L_ii / min_i L_ii)^2; a row above
- * `threshold` (default 10) with at least num_latent nonzeros gets one step
+ * conditioning with three probe solves L^{-T} x beside the sample's back
+ * solve (x = the row's normals, ones, alternating signs):
+ * est = max_i P_ii max_x |L^{-T} x|^2 / |x|^2, about kappa(P) / K; a row above
+ * `threshold` (default 4) with at least num_latent nonzeros gets one step
  * of iterative refinement against the exact precision matrix (residual from
  * per-observation residuals, FP64 sums).  macau_set_refine (before
  * macau_init) switches it off (on = 0) or sets the threshold (> 0);

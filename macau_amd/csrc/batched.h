@@ -126,25 +126,39 @@ inline void seg_fork(hipStream_t s, const SegStreams& seg) {
 inline void seg_record_join(const SegStreams& seg) { seg_check(hipEventRecord(seg.join, seg.stream)); }
 inline void seg_wait_join(hipStream_t s, const SegStreams& seg) { seg_check(hipStreamWaitEvent(s, seg.join, 0)); }
 
-// Condition estimate of one row after its Cholesky and, when it flags the
-// row, the row's entry in the refinement queue.  LANES lanes hold the row
-// (32: a half of the wave, base lane 0 or 32; 64: the whole wave); lane i
-// owns row i of L at Lrow (written by this lane) and inv_diag = 1 / L_ii.
-// The estimate is (max_i L_ii / min_i L_ii)^2 over the K real components.
-template <int KP, int LANES>
-__device__ __forceinline__ void refine_enqueue(const SampleArgs& a, int row, int nnz, int i, int base_lane,
-                                               float inv_diag, const float* Lrow) {
-  const RefineQueue& q = a.refine;
-  if (!q.count) return;
-  float mx = i < a.K ? inv_diag : 0.f;
-  float mn = i < a.K ? inv_diag : 3.0e38f;
+// Condition estimate of one row from three probe solves y = L^{-T} x done
+// beside the sample's back solve (x = the row's normals z, all ones,
+// alternating signs): est = max_i P_ii * max_x |y|^2 / |x|^2.  It is 1 for
+// P = c I and about kappa(P) / K when a direction is nearly singular, with
+// three probes so that a weak direction orthogonal to one probe is still
+// seen.  LANES lanes hold the row (32: a half of the wave; 64: the whole
+// wave); each passes its row's entries: squared norm of its row of L (P_ii),
+// y_i of the three probes and z_i (zero for i >= K).
+template <int LANES>
+__device__ __forceinline__ float probe_estimate(int i, int K, float pii, float y1, float y2, float y3, float z) {
+  float pm = i < K ? pii : 0.f, s1 = y1 * y1, s2 = y2 * y2, s3 = y3 * y3, zz = z * z;
 #pragma unroll
   for (int off = LANES / 2; off > 0; off >>= 1) {
-    mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    mn = fminf(mn, __shfl_xor(mn, off, 64));
+    pm = fmaxf(pm, __shfl_xor(pm, off, 64));
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+    s3 += __shfl_xor(s3, off, 64);
+    zz += __shfl_xor(zz, off, 64);
   }
-  const float ratio = mx / mn;  // max L_ii / min L_ii
-  const bool flag = row >= 0 && nnz >= q.min_nnz && ratio * ratio > q.thresh;
+  const float kk = (float)K;
+  return pm * fmaxf(fmaxf(s1 / fmaxf(zz, 1e-30f), s2 / kk), s3 / kk);
+}
+
+// The row's entry in the refinement queue when its estimate exceeds the
+// threshold (and it has at least min_nnz nonzeros).  LANES as above; lane i
+// owns row i of L at Lrow (written by this lane); est is the same on all
+// the row's lanes.
+template <int KP, int LANES>
+__device__ __forceinline__ void refine_enqueue(const SampleArgs& a, int row, int nnz, int i, int base_lane,
+                                               float est, const float* Lrow) {
+  const RefineQueue& q = a.refine;
+  if (!q.count) return;
+  const bool flag = row >= 0 && nnz >= q.min_nnz && est > q.thresh;
   int slot = -1;
   if (flag && i == 0) {
     slot = atomicAdd(q.count, 1);

@@ -58,8 +58,9 @@ struct BatchPlan {
 };
 
 // Rows of a batched half-sweep queued for one refinement step (refine.hip):
-// a row whose pivot ratio (max_i L_ii / min_i L_ii)^2 exceeds thresh and
-// which has at least min_nnz nonzeros is written here with its FP32 factor.
+// a row whose probe condition estimate (batched.h, probe_estimate) exceeds
+// thresh and which has at least min_nnz nonzeros is written here with its
+// FP32 factor.
 struct RefineQueue {
   int* count = nullptr;         // rows queued in the current half-sweep (null: no refinement)
   int cap = 0;                  // entries
@@ -71,10 +72,10 @@ struct RefineQueue {
 };
 constexpr int kRefineGrid = 256;
 constexpr int kRefineCap = 4096;
-// default threshold of the pivot-ratio estimate (max_i L_ii / min_i L_ii)^2:
-// measured split-BF16 row errors reach 1e-4 around 15-50 for rows with at
-// least K nonzeros
-constexpr float kRefineThresh = 10.f;
+// default threshold of the probe condition estimate (about kappa / K; 1 for a
+// multiple of the identity): split-BF16 row errors reach 1e-4 near
+// kappa ~ 300, i.e. an estimate of 10 at K = 32 and 5 at K = 64
+constexpr float kRefineThresh = 4.f;
 
 struct SampleArgs {
   RowsDev rows;

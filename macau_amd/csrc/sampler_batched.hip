@@ -240,6 +240,7 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
   }
   bool bad = false;
   float inv_diag = 1.f;  // 1 / L[i][i]
+  float pii = 0.f;       // |row i of L|^2 = P_ii
 #pragma unroll
   for (int k0 = 0; k0 < 32; k0 += 4) {
     opaque(i);
@@ -290,6 +291,7 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
     const float p2 = (w2 - p0 * l20 - p1 * l21) * r2;
     const float p3 = (w3 - p0 * l30 - p1 * l31 - p2 * l32) * r3;
     *reinterpret_cast<float4*>(&s.Lf[half][i][k0]) = make_float4(p0, p1, p2, p3);
+    if (k0 <= i) pii += p0 * p0 + p1 * p1 + p2 * p2 + p3 * p3;
     const int t = i - k0;
     const bool inb = t >= 0 && t < 4, below = t >= 4;
     const float yt = t == 0 ? y0 : (t == 1 ? y1 : (t == 2 ? y2 : y3));
@@ -312,9 +314,14 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
     }
   }
   if (bad && active && i == 0) atomicMax(a.fail, row + 1);
-  if (i < a.K) b += normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
-                               make_tag(TAG_LATENT, a.mode));
+  const float zi = i < a.K ? normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
+                                        make_tag(TAG_LATENT, a.mode))
+                           : 0.f;
+  b += zi;
   const float w = b;  // L^{-1} b + z
+  // condition probes (refinement only): L^{-T} x for x = z, ones, alternating signs
+  const bool probe = a.refine.count != nullptr;
+  float y1 = zi, y2 = i < a.K ? 1.f : 0.f, y3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
   // back solve L^T u = w.  Lane i needs column i of L (L[j][i], j > i),
   // i.e. entries of other lanes' rows: each lane reads its column from the
   // rows in LDS once; the solved values are broadcast with readlane.
@@ -329,9 +336,21 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
     const float x0 = rdlane(xl, j), x1 = rdlane(xl, 32 + j);
     const float xj = half ? x1 : x0;
     b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
+    if (probe) {
+      const float p1 = rdlane(y1 * inv_diag, j), q1 = rdlane(y1 * inv_diag, 32 + j);
+      const float p2 = rdlane(y2 * inv_diag, j), q2 = rdlane(y2 * inv_diag, 32 + j);
+      const float p3 = rdlane(y3 * inv_diag, j), q3 = rdlane(y3 * inv_diag, 32 + j);
+      const float v1 = half ? q1 : p1, v2 = half ? q2 : p2, v3 = half ? q3 : p3;
+      y1 = (i < j) ? y1 - Lc[j] * v1 : (i == j ? v1 : y1);
+      y2 = (i < j) ? y2 - Lc[j] * v2 : (i == j ? v2 : y2);
+      y3 = (i < j) ? y3 - Lc[j] * v3 : (i == j ? v3 : y3);
+    }
   }
   if (active) a.U[(size_t)row * 32 + i] = b;
-  refine_enqueue<32, 32>(a, active ? row : -1, half ? nnzB : nnzA, i, half * 32, inv_diag, &s.Lf[half][i][0]);
+  if (probe) {
+    const float est = probe_estimate<32>(i, a.K, pii, y1, y2, y3, zi);
+    refine_enqueue<32, 32>(a, active ? row : -1, half ? nnzB : nnzA, i, half * 32, est, &s.Lf[half][i][0]);
+  }
   if (!a.sse_partial) return 0.f;
   // train SSE share of this lane: (w^2 - u (Lambda u)) / alpha - 2 u c (+ sum r^2)
   s.us[half][i] = b;

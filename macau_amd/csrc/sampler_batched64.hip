@@ -222,6 +222,7 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
   }
   bool bad = false;
   float inv_diag = 1.f;
+  float pii = 0.f;  // |row i of L|^2 = P_ii
 #pragma unroll
   for (int k0 = 0; k0 < 64; k0 += 4) {
     opaque(i);
@@ -276,6 +277,7 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
     const float p2 = (w2 - p0 * l20 - p1 * l21) * r2;
     const float p3 = (w3 - p0 * l30 - p1 * l31 - p2 * l32) * r3;
     *reinterpret_cast<float4*>(&s.Lf[i][k0]) = make_float4(p0, p1, p2, p3);
+    if (k0 <= i) pii += p0 * p0 + p1 * p1 + p2 * p2 + p3 * p3;
     const int t = i - k0;
     const bool inb = t >= 0 && t < 4, below = t >= 4;
     const float yt = t == 0 ? y0 : (t == 1 ? y1 : (t == 2 ? y2 : y3));
@@ -301,9 +303,14 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
     }
   }
   if (bad && active && i == 0) atomicMax(a.fail, row + 1);
-  if (i < a.K) b += normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
-                               make_tag(TAG_LATENT, a.mode));
+  const float zi = i < a.K ? normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
+                                        make_tag(TAG_LATENT, a.mode))
+                           : 0.f;
+  b += zi;
   const float w = b;  // L^{-1} b + z
+  // condition probes (refinement only): L^{-T} x for x = z, ones, alternating signs
+  const bool probe = a.refine.count != nullptr;
+  float y1 = zi, y2 = i < a.K ? 1.f : 0.f, y3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
   // back solve L^T u = w (column i of L from the rows in LDS; readlane broadcast)
   lds_wait();
   float Lc[64];
@@ -314,9 +321,18 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
     opaque(i);
     const float xj = rdlane(b * inv_diag, j);
     b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
+    if (probe) {
+      const float v1 = rdlane(y1 * inv_diag, j), v2 = rdlane(y2 * inv_diag, j), v3 = rdlane(y3 * inv_diag, j);
+      y1 = (i < j) ? y1 - Lc[j] * v1 : (i == j ? v1 : y1);
+      y2 = (i < j) ? y2 - Lc[j] * v2 : (i == j ? v2 : y2);
+      y3 = (i < j) ? y3 - Lc[j] * v3 : (i == j ? v3 : y3);
+    }
   }
   if (active) a.U[(size_t)row * 64 + i] = b;
-  refine_enqueue<64, 64>(a, active ? row : -1, nnz, i, 0, inv_diag, &s.Lf[i][0]);
+  if (probe) {
+    const float est = probe_estimate<64>(i, a.K, pii, y1, y2, y3, zi);
+    refine_enqueue<64, 64>(a, active ? row : -1, nnz, i, 0, est, &s.Lf[i][0]);
+  }
   if (!a.sse_partial) return 0.f;
   s.us[i] = b;
   lds_wait();
