@@ -614,7 +614,7 @@ def main():
     if tuning is not None:
         out["tuning"] = tuning
     if head == "batched":
-        out["refinement"] = dict(res[head]["cg"][-1], threshold="probe condition estimate > 4, rows of >= K nonzeros")
+        out["refinement"] = dict(res[head]["cg"][-1], threshold="estimated condition number > 500, rows of >= max(K, 128) nonzeros")
     if "plain" in res and head != "plain":
         out["plain_iters_per_sec"] = 1.0 / res["plain"]["sec"]
         out["speedup_vs_plain"] = res["plain"]["sec"] / sec

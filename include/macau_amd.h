@@ -128,8 +128,9 @@ int macau_get_ftf_solver(macau_model* m, int mode);
  * After each row's Cholesky the batched sampler estimates the row's
  * conditioning with three probe solves L^{-T} x beside the sample's back
  * solve (x = the row's normals, ones, alternating signs):
- * est = max_i P_ii max_x |L^{-T} x|^2 / |x|^2, about kappa(P) / K; a row above
- * `threshold` (default 4) with at least num_latent nonzeros gets one step
+ * est = K max_i P_ii max_x |L^{-T} x|^2 / |x|^2, an estimate of kappa(P); a
+ * row above `threshold` (default 500) with at least max(num_latent, 128)
+ * nonzeros (fewer: not probed) gets one step
  * of iterative refinement against the exact precision matrix (residual from
  * per-observation residuals, FP64 sums).  macau_set_refine (before
  * macau_init) switches it off (on = 0) or sets the threshold (> 0);

@@ -128,10 +128,9 @@ inline void seg_wait_join(hipStream_t s, const SegStreams& seg) { seg_check(hipS
 
 // Condition estimate of one row from three probe solves y = L^{-T} x done
 // beside the sample's back solve (x = the row's normals z, all ones,
-// alternating signs): est = max_i P_ii * max_x |y|^2 / |x|^2.  It is 1 for
-// P = c I and about kappa(P) / K when a direction is nearly singular, with
-// three probes so that a weak direction orthogonal to one probe is still
-// seen.  LANES lanes hold the row (32: a half of the wave; 64: the whole
+// alternating signs): est = K max_i P_ii max_x |y|^2 / |x|^2.  It is K for
+// P = c I and about kappa(P) when a direction is nearly singular, with three
+// probes so that a weak direction orthogonal to one probe is still seen.  LANES lanes hold the row (32: a half of the wave; 64: the whole
 // wave); each passes its row's entries: squared norm of its row of L (P_ii),
 // y_i of the three probes and z_i (zero for i >= K).
 template <int LANES>
@@ -146,7 +145,7 @@ __device__ __forceinline__ float probe_estimate(int i, int K, float pii, float y
     zz += __shfl_xor(zz, off, 64);
   }
   const float kk = (float)K;
-  return pm * fmaxf(fmaxf(s1 / fmaxf(zz, 1e-30f), s2 / kk), s3 / kk);
+  return kk * pm * fmaxf(fmaxf(s1 / fmaxf(zz, 1e-30f), s2 / kk), s3 / kk);
 }
 
 // The row's entry in the refinement queue when its estimate exceeds the

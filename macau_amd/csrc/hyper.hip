@@ -386,7 +386,7 @@ __device__ void nw_draw(const NWArgs& a, double* base) {
       for (int j = 0; j < K; j++) s += D[i + j * K] * v_mu[j];
     a.Lmu_f[i] = (float)s;
   }
-  if (t == 0 && !ok && a.fail) atomicMax(a.fail, 1 << 30);
+  if (t == 0 && !ok && a.fail) atomicMax(a.fail, kFailNormalWishart);
 }
 
 // ------------------------------------------------------------------
@@ -602,7 +602,7 @@ __device__ void nw_draw_w(const NWArgs& a, NWWaveLds& s) {
   __syncthreads();
   if (wave == 0 && lane < a.KP)
     a.Lmu_f[lane] = (float)((s.lm[0][lane] + s.lm[1][lane]) + (s.lm[2][lane] + s.lm[3][lane]));
-  if (threadIdx.x == 0 && bad && a.fail) atomicMax(a.fail, 1 << 30);
+  if (threadIdx.x == 0 && bad && a.fail) atomicMax(a.fail, kFailNormalWishart);
 }
 
 __global__ void __launch_bounds__(kNWThreadsW) k_nw_draw_w(NWBatch b) {

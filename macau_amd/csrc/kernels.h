@@ -72,10 +72,13 @@ struct RefineQueue {
 };
 constexpr int kRefineGrid = 256;
 constexpr int kRefineCap = 4096;
-// default threshold of the probe condition estimate (about kappa / K; 1 for a
-// multiple of the identity): split-BF16 row errors reach 1e-4 near
-// kappa ~ 300, i.e. an estimate of 10 at K = 32 and 5 at K = 64
-constexpr float kRefineThresh = 4.f;
+// default threshold of the probe condition estimate (an estimate of kappa(P);
+// K for a multiple of the identity): measured split-BF16 row errors stay
+// below 5e-5 up to kappa ~ 500
+constexpr float kRefineThresh = 500.f;
+// rows with fewer nonzeros are neither probed nor queued (their errors
+// stay small: the data term's rounding is spread over few products)
+constexpr int kRefineMinNnz = 128;
 
 struct SampleArgs {
   RowsDev rows;
@@ -263,6 +266,8 @@ struct CsrDev {
 // --- F^T F branch of MacauPrior::sample_beta (latentprior.cpp:280-289) ---
 // fail-flag value set when F^T F + lambda_beta I is not positive definite
 constexpr int kFailDenseChol = 1 << 29;
+// the Normal-Wishart draw's Cholesky of its scale or precision matrix failed
+constexpr int kFailNormalWishart = 1 << 30;
 // F^T F (ncol x ncol, col-major, lower triangle; the upper is zeroed) of the
 // CSR F and its transpose Ft, in the reference's accumulation order
 void launch_ftf(const CsrDev& F, const CsrDev& Ft, double* FtF, hipStream_t s);

@@ -343,7 +343,7 @@ void Model::init() {
   if (sampler == SAMPLER_BATCHED && refine) {
     const int maxrows = *std::max_element(dims.begin(), dims.end());
     rq = make_refine_queue(refine_bufs, KP, std::min(maxrows, kRefineCap), refine_thresh,
-                           refine_min_nnz < 0 ? num_latent : refine_min_nnz, stream);
+                           refine_min_nnz < 0 ? std::max(num_latent, kRefineMinNnz) : refine_min_nnz, stream);
   }
   // the batched sampler computes the train SSE of the adaptive noise inside
   // the last half-sweep (u^T G u - 2 u^T c + sum r^2 per row)
@@ -608,6 +608,10 @@ void Model::check_fail() {
     // chol_decomp's message for the F^T F branch (chol.cpp:21-28), the per-row
     // sampler's otherwise (latentprior.cpp:375)
     if (f == kFailDenseChol) throw std::runtime_error("c++ error: Cholesky decomp failed");
+    // the reference's CondNormalWishart does not check its LLTs (its draw
+    // would carry NaNs on); here the chain stops with this message
+    if (f == kFailNormalWishart)
+      throw std::runtime_error("CondNormalWishart: the scale or precision matrix is not positive definite");
     throw std::runtime_error("Cholesky Decomposition failed!");
   }
 }
@@ -641,6 +645,7 @@ void Model::download_latents(int mode, std::vector<double>& out) {
 }
 
 static volatile sig_atomic_t keep_running = 1;
+constexpr int kFailPollIters = 10;
 static void int_handler(int) {
   keep_running = 0;
   printf("[Received Ctrl-C. Stopping after finishing the current iteration.]\n");
@@ -674,6 +679,12 @@ void Model::run() {
     }
     if (save_model && i >= burnin) saveModel(i - burnin + 1);
     if (verbose) printStatus(i, tick() - starti);
+    // a failed factorisation stops the chain within kFailPollIters iterations
+    // (every iteration when the host reads the state anyway)
+    if (!need_host && (i + 1) % kFailPollIters == 0) {
+      sync();
+      check_fail();
+    }
   }
   (void)start;
   sync();

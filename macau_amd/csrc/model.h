@@ -332,7 +332,7 @@ class Model {
   // condition estimate (refine.hip); settings read at init
   bool refine = true;
   float refine_thresh = kRefineThresh;
-  int refine_min_nnz = -1;  // < 0: num_latent
+  int refine_min_nnz = -1;  // < 0: max(num_latent, kRefineMinNnz)
   std::vector<DevBuf> refine_bufs;
   RefineQueue rq;
   // batched sampler launch shapes (tune.cpp): per mode, looked up at init

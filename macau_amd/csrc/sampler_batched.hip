@@ -16,8 +16,11 @@
 // Gram: each MFMA step takes 16 nonzeros (lanes 0-31 the first 8, lanes
 // 32-63 the next 8; lane l holds component l%32), splits every value into
 // hi = bf16(x) and lo = bf16(x - hi), and accumulates G += Hi Hi^T + Hi Lo^T
-// + Lo Hi^T with three v_mfma_f32_32x32x16_bf16 into FP32 (the dropped Lo Lo^T
-// term is ~2^-18 relative).  The RHS accumulates in FP32 on the vector ALU.
+// + Lo Hi^T + Lo Lo^T with four v_mfma_f32_32x32x16_bf16 into FP32.  Without
+// Lo Lo^T the diagonal loses a positive bias of ~5e-6 |x|^2 per product,
+// which rows of few nonzeros against a weak prior direction turn into row
+// errors near 1e-4; the fourth MFMA costs no measurable time (the sampler is
+// not MFMA-bound).  The RHS accumulates in FP32 on the vector ALU.
 // The nonzeros stream through in 64-nonzero blocks with the index loads two
 // blocks and the factor gathers one block ahead of the MFMAs (GramStream).
 // The 32x32 accumulator fragment (lane l, register r holds
@@ -115,6 +118,7 @@ __device__ __forceinline__ void mfma_blk(const BlkLds<NO>& L, int buf, int half,
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc, 0, 0, 0);
   }
 }
 
@@ -320,7 +324,7 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
   b += zi;
   const float w = b;  // L^{-1} b + z
   // condition probes (refinement only): L^{-T} x for x = z, ones, alternating signs
-  const bool probe = a.refine.count != nullptr;
+  const bool probe = a.refine.count != nullptr && (nnzA >= a.refine.min_nnz || nnzB >= a.refine.min_nnz);
   float y1 = zi, y2 = i < a.K ? 1.f : 0.f, y3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
   // back solve L^T u = w.  Lane i needs column i of L (L[j][i], j > i),
   // i.e. entries of other lanes' rows: each lane reads its column from the

@@ -9,8 +9,8 @@
 //   * a lane gathers components l%32 and l%32 + 32 of its 8 nonzeros per
 //     16-nonzero chunk; the Gram is three 32 x 32 accumulator blocks
 //     G00 (rows/cols 0-31), G10 (rows 32-63 x cols 0-31), G11 (rows/cols
-//     32-63), each HiHi^T + HiLo^T + LoHi^T (9 v_mfma_f32_32x32x16_bf16 per
-//     chunk); the stream stages 32-nonzero blocks (index loads two blocks and
+//     32-63), each HiHi^T + HiLo^T + LoHi^T + LoLo^T (12 v_mfma_f32_32x32x16_bf16
+//     per chunk); the stream stages 32-nonzero blocks (index loads two blocks and
 //     gathers one block ahead of the MFMAs);
 //   * the epilogue factors one 64 x 64 matrix per wave (lane i = row i) in
 //     4-column blocks with the trailing rank-4 updates on
@@ -117,12 +117,15 @@ __device__ __forceinline__ void mfma_blk64(const BlkLds64<NO>& L, int buf, int h
     g.g00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h0, h0, g.g00, 0, 0, 0);
     g.g00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h0, l0, g.g00, 0, 0, 0);
     g.g00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(l0, h0, g.g00, 0, 0, 0);
+    g.g00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(l0, l0, g.g00, 0, 0, 0);
     g.g10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h1, h0, g.g10, 0, 0, 0);
     g.g10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h1, l0, g.g10, 0, 0, 0);
     g.g10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(l1, h0, g.g10, 0, 0, 0);
+    g.g10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(l1, l0, g.g10, 0, 0, 0);
     g.g11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h1, h1, g.g11, 0, 0, 0);
     g.g11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h1, l1, g.g11, 0, 0, 0);
     g.g11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(l1, h1, g.g11, 0, 0, 0);
+    g.g11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(l1, l1, g.g11, 0, 0, 0);
   }
 }
 
@@ -309,7 +312,7 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
   b += zi;
   const float w = b;  // L^{-1} b + z
   // condition probes (refinement only): L^{-T} x for x = z, ones, alternating signs
-  const bool probe = a.refine.count != nullptr;
+  const bool probe = a.refine.count != nullptr && nnz >= a.refine.min_nnz;
   float y1 = zi, y2 = i < a.K ? 1.f : 0.f, y3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
   // back solve L^T u = w (column i of L from the rows in LDS; readlane broadcast)
   lds_wait();
