@@ -62,8 +62,8 @@ from macau_amd import _lib, synthetic  # noqa: E402
 from macau_amd._lib import check, ptr  # noqa: E402
 
 METRIC = "Gibbs iters/sec, batched-mixed vs plain FP32, 20M nnz K=32, RMSE≤+0.5%"
-PEAK_TFLOPS = {"plain": 157.3, "batched": 2500.0}  # FP32 vector/MFMA; BF16 dense MFMA
-UNIT_OF = {"plain": "f32", "batched": "bf16x3-f32acc"}
+PEAK_TFLOPS = {"plain": 157.3, "batched": 2500.0, "batched-norefine": 2500.0}  # FP32 vector/MFMA; BF16 dense MFMA
+UNIT_OF = {"plain": "f32", "batched": "bf16x4-f32acc", "batched-norefine": "bf16x4-f32acc"}
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 MAX_FF_SIZE = 10000  # side information with at most this many features uses F^T F (macau.pyx:334)
@@ -106,7 +106,7 @@ def load_workload(name, seed):
     raise ValueError(name)
 
 
-def build_model(w, sampler_kind, seed, graph=True):
+def build_model(w, sampler_kind, seed, graph=True, refine=True):
     L = _lib.lib()
     st = w["settings"]
     K = st["num_latent"]
@@ -142,6 +142,7 @@ def build_model(w, sampler_kind, seed, graph=True):
     check(L.macau_set_samples(h, st["burnin"], st["nsamples"]))
     check(L.macau_set_seed(h, seed))
     check(L.macau_set_sampler(h, sampler_kind))
+    check(L.macau_set_refine(h, int(refine), 0.0))
     check(L.macau_set_graph(h, int(graph)))
     check(L.macau_set_verbose(h, 0))
     check(L.macau_init(h))
@@ -158,13 +159,13 @@ def sampler_flops(w):
 
 
 def kind_of(name):
-    return _lib.SAMPLER_BATCHED if name == "batched" else _lib.SAMPLER_PLAIN_FP32
+    return _lib.SAMPLER_PLAIN_FP32 if name == "plain" else _lib.SAMPLER_BATCHED
 
 
 def time_sampler(name, w, steps, warmup, seed, log):
     L = _lib.lib()
     ts = time.perf_counter()
-    h = build_model(w, kind_of(name), seed)
+    h = build_model(w, kind_of(name), seed, refine=name != "batched-norefine")
     setup_s = time.perf_counter() - ts
     nm = len(w["dims"])
     try:
@@ -192,7 +193,7 @@ def time_sampler(name, w, steps, warmup, seed, log):
 
 def full_chain_rmse(name, w, seed):
     L = _lib.lib()
-    h = build_model(w, kind_of(name), seed)
+    h = build_model(w, kind_of(name), seed, refine=name != "batched-norefine")
     try:
         n = w["settings"]["burnin"] + w["settings"]["nsamples"]
         t0 = time.perf_counter()
@@ -273,7 +274,7 @@ def fp64_errors(w, name, seed, iters, log):
     L = _lib.lib()
     K = w["settings"]["num_latent"]
     dims = w["dims"]
-    h = build_model(w, kind_of(name), seed)
+    h = build_model(w, kind_of(name), seed, refine=name != "batched-norefine")
     try:
         check(L.macau_iterate(h, iters))
         check(L.macau_synchronize(h))
@@ -478,7 +479,8 @@ def main():
     ap.add_argument("--no-chain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 oracle chain and error table")
     ap.add_argument("--cpu-iters", type=int, default=25)
-    ap.add_argument("--sampler", choices=["batched", "plain"], default=None)
+    ap.add_argument("--sampler", choices=["batched", "batched-norefine", "plain"], default=None,
+                    help="batched-norefine: the batched sampler without the condition probes and refinement")
     ap.add_argument("--tune", action="store_true",
                     help="time the default launch shapes, run the autotuner, time the tuned shapes")
     args = ap.parse_args()
@@ -613,7 +615,7 @@ def main():
                            "note": "time includes the beta noise draw, Uhat and lambda_beta updates"}
     if tuning is not None:
         out["tuning"] = tuning
-    if head == "batched":
+    if head.startswith("batched"):
         out["refinement"] = dict(res[head]["cg"][-1], threshold="estimated condition number > 500, rows of >= max(K, 128) nonzeros")
     if "plain" in res and head != "plain":
         out["plain_iters_per_sec"] = 1.0 / res["plain"]["sec"]

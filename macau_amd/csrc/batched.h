@@ -69,11 +69,12 @@ __device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo) {
   lo = __builtin_bit_cast(bf16x8, l);
 }
 
-// nonzero e0 + lane of [e0, end) (index -1 and residual 0 past the end)
-template <int NO>
+// nonzero e0 + lane of [e0, end) for the first BLK lanes (index -1 and
+// residual 0 past the end)
+template <int NO, int BLK>
 __device__ __forceinline__ void load_blk(const SampleArgs& a, int e0, int end, int lane, BlkReg<NO>& b) {
   const int e = e0 + lane;
-  if (e < end) {
+  if ((BLK == 64 || lane < BLK) && e < end) {
 #pragma unroll
     for (int t = 0; t < NO; t++) b.ix[t] = a.rows.idx[(size_t)e * NO + t];
     b.r = a.rows.r[e];
@@ -126,32 +127,53 @@ inline void seg_fork(hipStream_t s, const SegStreams& seg) {
 inline void seg_record_join(const SegStreams& seg) { seg_check(hipEventRecord(seg.join, seg.stream)); }
 inline void seg_wait_join(hipStream_t s, const SegStreams& seg) { seg_check(hipStreamWaitEvent(s, seg.join, 0)); }
 
-// Condition estimate of one row from three probe solves y = L^{-T} x done
-// beside the sample's back solve (x = the row's normals z, all ones,
-// alternating signs): est = K max_i P_ii max_x |y|^2 / |x|^2.  It is K for
-// P = c I and about kappa(P) when a direction is nearly singular, with three
-// probes so that a weak direction orthogonal to one probe is still seen.  LANES lanes hold the row (32: a half of the wave; 64: the whole
-// wave); each passes its row's entries: squared norm of its row of L (P_ii),
-// y_i of the three probes and z_i (zero for i >= K).
+// Entry r (per lane, 0..15) of a 16-register accumulator fragment.
+__device__ __forceinline__ float frag_at(const f32x16& acc, int r) {
+  float v = acc[0];
+#pragma unroll
+  for (int k = 1; k < 16; k++) v = r == k ? acc[k] : v;
+  return v;
+}
+
+// Diagonal entry (j, j), j = lane % 32, of a 32 x 32 block in the MFMA
+// accumulator layout (lane l, register r holds entry (8(r/4) + 4(l/32) + r%4,
+// l%32)): lane j + 32 ((j >> 2) & 1) holds it in register 4 (j >> 3) + (j & 3).
+__device__ __forceinline__ float frag_diag(const f32x16& acc, int lane) {
+  const int j = lane & 31;
+  const float mine = frag_at(acc, 4 * (j >> 3) + (j & 3));  // the diagonal on the holding lanes
+  return __shfl(mine, j + 32 * ((j >> 2) & 1), 64);
+}
+
+// Condition estimate of one row from three probe solves y = L^{-1} D^{1/2} x
+// done beside the forward solve of the RHS (D = diag(P); x = the row's
+// normals z, all ones, alternating signs): |y|^2 = x^T Pt^{-1} x for the
+// Jacobi-scaled Pt = D^{-1/2} P D^{-1/2} (unit diagonal, largest eigenvalue
+// between 1 and K), and est = K max_x |y|^2 / |x|^2.  It is K for a diagonal
+// P and about kappa(Pt) when a direction is nearly singular (a probe catches
+// about 1/K of a weak direction), with three probes so that a weak direction
+// orthogonal to one probe is still seen.  The scaling keeps rows whose
+// spectrum is merely spread (one large eigenvalue along a common mean
+// direction of the factors) from reading as ill-conditioned.  LANES lanes
+// hold the row (32: a half of the wave; 64: the whole wave); each passes y_i
+// of the three probes and z_i (zero for i >= K).
 template <int LANES>
-__device__ __forceinline__ float probe_estimate(int i, int K, float pii, float y1, float y2, float y3, float z) {
-  float pm = i < K ? pii : 0.f, s1 = y1 * y1, s2 = y2 * y2, s3 = y3 * y3, zz = z * z;
+__device__ __forceinline__ float probe_estimate(int K, float y1, float y2, float y3, float z) {
+  float s1 = y1 * y1, s2 = y2 * y2, s3 = y3 * y3, zz = z * z;
 #pragma unroll
   for (int off = LANES / 2; off > 0; off >>= 1) {
-    pm = fmaxf(pm, __shfl_xor(pm, off, 64));
     s1 += __shfl_xor(s1, off, 64);
     s2 += __shfl_xor(s2, off, 64);
     s3 += __shfl_xor(s3, off, 64);
     zz += __shfl_xor(zz, off, 64);
   }
   const float kk = (float)K;
-  return kk * pm * fmaxf(fmaxf(s1 / fmaxf(zz, 1e-30f), s2 / kk), s3 / kk);
+  return kk * fmaxf(fmaxf(s1 / fmaxf(zz, 1e-30f), s2 / kk), s3 / kk);
 }
 
 // The row's entry in the refinement queue when its estimate exceeds the
 // threshold (and it has at least min_nnz nonzeros).  LANES as above; lane i
-// owns row i of L at Lrow (written by this lane); est is the same on all
-// the row's lanes.
+// owns row i of L, whose entries 0..i are at Lrow (the triangular layout of
+// the epilogues); est is the same on all the row's lanes.
 template <int KP, int LANES>
 __device__ __forceinline__ void refine_enqueue(const SampleArgs& a, int row, int nnz, int i, int base_lane,
                                                float est, const float* Lrow) {
@@ -168,11 +190,11 @@ __device__ __forceinline__ void refine_enqueue(const SampleArgs& a, int row, int
     float* dst = q.L + (size_t)slot * KP * KP + (size_t)i * KP;
 #pragma unroll
     for (int k = 0; k < KP; k += 4) {
-      float4 v = *reinterpret_cast<const float4*>(Lrow + k);
-      v.x = k <= i ? v.x : 0.f;
-      v.y = k + 1 <= i ? v.y : 0.f;
-      v.z = k + 2 <= i ? v.z : 0.f;
-      v.w = k + 3 <= i ? v.w : 0.f;
+      float4 v;
+      v.x = k <= i ? Lrow[k] : 0.f;
+      v.y = k + 1 <= i ? Lrow[k + 1] : 0.f;
+      v.z = k + 2 <= i ? Lrow[k + 2] : 0.f;
+      v.w = k + 3 <= i ? Lrow[k + 3] : 0.f;
       *reinterpret_cast<float4*>(dst + k) = v;
     }
   }

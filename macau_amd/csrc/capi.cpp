@@ -512,7 +512,8 @@ int macau_get_refine_stats(macau_model* m, long long* flagged, long long* refine
     HIPCHK(hipMemcpy(t, x.rq.totals, sizeof(t), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(&c, x.rq.count, sizeof(int), hipMemcpyDeviceToHost));
   }
-  // the last half-sweep's queue is folded into the totals at the next one
+  // the refinement launch folds each half-sweep's queue into the totals (c is
+  // nonzero only while a half-sweep is in flight)
   if (flagged) *flagged = t[0] + c;
   if (refined) *refined = t[1] + std::min(c, x.rq.cap);
   API_END_INT

@@ -62,7 +62,8 @@ struct BatchPlan {
 // thresh and which has at least min_nnz nonzeros is written here with its
 // FP32 factor.
 struct RefineQueue {
-  int* count = nullptr;         // rows queued in the current half-sweep (null: no refinement)
+  int* count = nullptr;         // [0] rows queued in the current half-sweep (null: no refinement),
+                                // [1] work-groups of the refinement launch that have finished
   int cap = 0;                  // entries
   int* row = nullptr;           // cap
   float* L = nullptr;           // cap x KP x KP, row-major, zero above the diagonal
@@ -72,9 +73,9 @@ struct RefineQueue {
 };
 constexpr int kRefineGrid = 256;
 constexpr int kRefineCap = 4096;
-// default threshold of the probe condition estimate (an estimate of kappa(P);
-// K for a multiple of the identity): measured split-BF16 row errors stay
-// below 5e-5 up to kappa ~ 500
+// default threshold of the probe condition estimate (an estimate of the
+// condition number of the Jacobi-scaled P; K for a diagonal P): measured
+// split-BF16 row errors stay below 5e-5 up to kappa ~ 500
 constexpr float kRefineThresh = 500.f;
 // rows with fewer nonzeros are neither probed nor queued (their errors
 // stay small: the data term's rounding is spread over few products)
@@ -104,14 +105,14 @@ struct SampleArgs {
   RefineQueue refine;
 };
 // refinement of the queued rows (refine.hip), launched after a batched
-// half-sweep's epilogues; the reset before them
-void launch_refine_reset(const SampleArgs& a, hipStream_t s);
+// half-sweep's epilogues; it leaves the queue cleared for the next half-sweep
 void launch_refine(const SampleArgs& a, hipStream_t s);
 
 void launch_sample_plain(const SampleArgs& a, hipStream_t s);
-// With seg_stream set, the heavy-row segment kernels run on it beside the
-// light rows: seg_fork is recorded on s before them and seg_join after them,
-// and s waits on seg_join before the heavy rows are finished.
+// With seg_stream set, the heavy-row kernels (segments, their sums and the
+// heavy rows' finish) run on it beside the light rows: seg_fork is recorded
+// on s before them and seg_join after them, and s waits on seg_join before
+// the refinement step.
 struct SegStreams {
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;

@@ -375,7 +375,7 @@ void Model::init() {
 RefineQueue make_refine_queue(std::vector<DevBuf>& B, int KP, int cap, float thresh, int min_nnz, hipStream_t s) {
   B.clear();
   B.resize(4);
-  upload(B[0], std::vector<int>{0}, s);
+  upload(B[0], std::vector<int>{0, 0}, s);
   B[1].alloc((size_t)std::max(cap, 1) * sizeof(int));
   B[2].alloc((size_t)std::max(cap, 1) * KP * KP * sizeof(float));
   upload(B[3], std::vector<long long>{0, 0}, s);

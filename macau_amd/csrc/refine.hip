@@ -21,22 +21,15 @@
 //
 // The train SSE fused into the last half-sweep keeps the unrefined sample of
 // a refined row (the change is second order in the correction).
+//
+// The queue needs no separate reset launch: the last work-group of k_refine
+// to finish (a ticket counter beside the queue count) folds the half-sweep's
+// count into the running totals and clears the count for the next one.
 #include "batched.h"
 
 namespace mgpu {
 
 constexpr int kRefineThreads = 256;
-
-// before the epilogues of a half-sweep: fold the last half-sweep's counts
-// into the running totals and clear the queue
-__global__ void k_refine_reset(RefineQueue q) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const int c = *q.count;
-    q.totals[0] += c;
-    q.totals[1] += c < q.cap ? c : q.cap;
-    *q.count = 0;
-  }
-}
 
 template <int KP, int NO>
 __global__ void __launch_bounds__(kRefineThreads) k_refine(SampleArgs a) {
@@ -47,8 +40,8 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(SampleArgs a) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int it = *a.iter;
   const double alpha = *a.alpha;
-  int nq = *q.count;
-  if (nq > q.cap) nq = q.cap;
+  const int queued = *(volatile const int*)q.count;
+  const int nq = queued < q.cap ? queued : q.cap;
   for (int e = blockIdx.x; e < nq; e += gridDim.x) {
     const int row = q.row[e];
     const float* Lq = q.L + (size_t)e * KP * KP;
@@ -131,6 +124,18 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(SampleArgs a) {
     }
     __syncthreads();
   }
+  // every work-group has read the count before taking its ticket; the last
+  // one folds the count into the totals and clears the queue
+  if (t == 0) {
+    __threadfence();
+    if (atomicAdd(q.count + 1, 1) == (int)gridDim.x - 1) {
+      q.totals[0] += queued;
+      q.totals[1] += nq;
+      *(volatile int*)q.count = 0;
+      *(volatile int*)(q.count + 1) = 0;
+      __threadfence();
+    }
+  }
 }
 
 template <int KP>
@@ -143,10 +148,6 @@ static void launch_refine_kp(const SampleArgs& a, hipStream_t s) {
     case 4: hipLaunchKernelGGL((k_refine<KP, 4>), g, b, 0, s, a); break;
     default: break;
   }
-}
-
-void launch_refine_reset(const SampleArgs& a, hipStream_t s) {
-  if (a.refine.count) hipLaunchKernelGGL(k_refine_reset, dim3(1), dim3(64), 0, s, a.refine);
 }
 
 void launch_refine(const SampleArgs& a, hipStream_t s) {

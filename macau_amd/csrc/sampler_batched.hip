@@ -21,8 +21,9 @@
 // which rows of few nonzeros against a weak prior direction turn into row
 // errors near 1e-4; the fourth MFMA costs no measurable time (the sampler is
 // not MFMA-bound).  The RHS accumulates in FP32 on the vector ALU.
-// The nonzeros stream through in 64-nonzero blocks with the index loads two
-// blocks and the factor gathers one block ahead of the MFMAs (GramStream).
+// The nonzeros stream through in blocks (32 nonzeros in the light-row
+// kernel, 64 in the segment kernel) with the index loads two blocks and the
+// factor gathers one block ahead of the MFMAs (GramStream).
 // The 32x32 accumulator fragment (lane l, register r holds
 // G[8(r/4) + 4(l/32) + r%4][l%32]) of two rows is exchanged between the wave
 // halves so that lane i of half h owns row i of P of row h; the Cholesky with
@@ -52,42 +53,47 @@ bool batched_fits(const FactorSet& f, int nother, int KP) {
 }
 
 // ---------------------------------------------------------------------------
-// Gram stream.  The nonzeros of one or two rows are consumed in 64-nonzero
-// blocks (a block never spans two rows).  Per block:
-//   1. every lane loads one nonzero's indices and residual (coalesced),
-//      two blocks ahead of the block being multiplied;
+// Gram stream.  The nonzeros of one or two rows are consumed in blocks of
+// BLK = 32 or 64 nonzeros (a block never spans two rows).  Per block:
+//   1. each of the first BLK lanes loads one nonzero's indices and residual
+//      (coalesced), two blocks ahead of the block being multiplied;
 //   2. the block is staged in this wave's LDS, and each lane reads, per
 //      16-nonzero chunk, the 8 indices of its half with two 16-byte broadcast
 //      reads, then gathers component l%32 of those 8 factor columns with
 //      bounds-checked buffer loads (index -1 = padding reads as 0), one block
 //      ahead of the block being multiplied;
-//   3. the block's four chunks go through the split-BF16 MFMA (a chunk that
-//      is all padding is skipped).
-// So each wave keeps the index loads of block k+2 and the 32 gathers of block
-// k+1 in flight while it multiplies block k.
+//   3. the block's BLK / 16 chunks go through the split-BF16 MFMA (a chunk
+//      that is all padding is skipped).
+// So each wave keeps the index loads of block k+2 and the BLK / 2 gathers of
+// block k+1 in flight while it multiplies block k.
 // ---------------------------------------------------------------------------
-constexpr int kBlk = 64;
+// Block sizes: 32 nonzeros in the light-row kernel (its values in flight
+// take 16 registers per block, which keeps it at 4 waves per SIMD), 64 in the
+// segment kernel (few registers; more loads in flight per wave).
+constexpr int kBlkLight = 32, kBlkSeg = 64;
 
-template <int NO>
+template <int NO, int BLK>
 struct BlkLds {
-  int ix[2][NO][kBlk];  // double buffered
-  float r[2][kBlk];
+  int ix[2][NO][BLK];  // double buffered
+  float r[2][BLK];
 };
 
 
-template <int NO>
-__device__ __forceinline__ void stage_blk(BlkLds<NO>& L, int buf, int lane, const BlkReg<NO>& b) {
+template <int NO, int BLK>
+__device__ __forceinline__ void stage_blk(BlkLds<NO, BLK>& L, int buf, int lane, const BlkReg<NO>& b) {
+  if (BLK == 64 || lane < BLK) {
 #pragma unroll
-  for (int t = 0; t < NO; t++) L.ix[buf][t][lane] = b.ix[t];
-  L.r[buf][lane] = b.r;
+    for (int t = 0; t < NO; t++) L.ix[buf][t][lane] = b.ix[t];
+    L.r[buf][lane] = b.r;
+  }
 }
 
 
-template <int NO>
-__device__ __forceinline__ void gather_blk(const Factors<NO>& F, const BlkLds<NO>& L, int buf, int half, int comp,
-                                           float v[32]) {
+template <int NO, int BLK>
+__device__ __forceinline__ void gather_blk(const Factors<NO>& F, const BlkLds<NO, BLK>& L, int buf, int half,
+                                           int comp, float v[BLK / 2]) {
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
+  for (int c = 0; c < BLK / 16; c++) {
     const int o = c * 16 + half * 8;
     int ix[NO][8];
 #pragma unroll
@@ -104,11 +110,11 @@ __device__ __forceinline__ void gather_blk(const Factors<NO>& F, const BlkLds<NO
   }
 }
 
-template <int NO>
-__device__ __forceinline__ void mfma_blk(const BlkLds<NO>& L, int buf, int half, const float v[32], int nvalid,
-                                         f32x16& acc, float& bp) {
+template <int NO, int BLK>
+__device__ __forceinline__ void mfma_blk(const BlkLds<NO, BLK>& L, int buf, int half, const float v[BLK / 2],
+                                         int nvalid, f32x16& acc, float& bp) {
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
+  for (int c = 0; c < BLK / 16; c++) {
     if (c * 16 >= nvalid) break;
     const int o = c * 16 + half * 8;
     bf16x8 hi, lo;
@@ -124,31 +130,31 @@ __device__ __forceinline__ void mfma_blk(const BlkLds<NO>& L, int buf, int half,
 
 // Gram and RHS partials of nonzeros [begA, endA) into accA/bpA and, when
 // PAIR, of [begB, endB) into accB/bpB, as one pipelined block stream.
-template <int NO, bool PAIR>
+template <int NO, bool PAIR, int BLK>
 struct GramStream {
   const SampleArgs& a;
   const Factors<NO>& F;
-  BlkLds<NO>& L;
+  BlkLds<NO, BLK>& L;
   int begA, endA, begB, endB, nbA, nb, lane, half, comp;
   BlkReg<NO> nxt;
 
-  __device__ __forceinline__ GramStream(const SampleArgs& a_, const Factors<NO>& F_, BlkLds<NO>& L_, int bA,
+  __device__ __forceinline__ GramStream(const SampleArgs& a_, const Factors<NO>& F_, BlkLds<NO, BLK>& L_, int bA,
                                         int eA, int bB, int eB)
       : a(a_), F(F_), L(L_), begA(bA), endA(eA), begB(bB), endB(eB) {
     lane = threadIdx.x & 63;
     half = lane >> 5;
     comp = lane & 31;
-    nbA = (endA - begA + kBlk - 1) / kBlk;
-    nb = nbA + (PAIR ? (endB - begB + kBlk - 1) / kBlk : 0);
+    nbA = (endA - begA + BLK - 1) / BLK;
+    nb = nbA + (PAIR ? (endB - begB + BLK - 1) / BLK : 0);
   }
-  __device__ __forceinline__ int start(int k) const { return k < nbA ? begA + k * kBlk : begB + (k - nbA) * kBlk; }
+  __device__ __forceinline__ int start(int k) const { return k < nbA ? begA + k * BLK : begB + (k - nbA) * BLK; }
   __device__ __forceinline__ int stop(int k) const { return k < nbA ? endA : endB; }
 
   // multiply block k (gathered in vc) and gather block k + 1 into vn
-  __device__ __forceinline__ void step(int k, const float vc[32], float vn[32], f32x16& accA, float& bpA,
-                                       f32x16& accB, float& bpB) {
+  __device__ __forceinline__ void step(int k, const float vc[BLK / 2], float vn[BLK / 2], f32x16& accA,
+                                       float& bpA, f32x16& accB, float& bpB) {
     BlkReg<NO> after;
-    if (k + 2 < nb) load_blk<NO>(a, start(k + 2), stop(k + 2), lane, after);
+    if (k + 2 < nb) load_blk<NO, BLK>(a, start(k + 2), stop(k + 2), lane, after);
     if (k + 1 < nb) {
       stage_blk<NO>(L, (k + 1) & 1, lane, nxt);
       lds_wait();
@@ -165,15 +171,15 @@ struct GramStream {
   // caller issues them ahead of time and hands them to run()
   static __device__ __forceinline__ void first_blocks(const SampleArgs& a, int4 r, int lane, BlkReg<NO>& b0,
                                                       BlkReg<NO>& b1) {
-    const int na = (r.y - r.x + kBlk - 1) / kBlk;
-    load_blk<NO>(a, na > 0 ? r.x : r.z, na > 0 ? r.y : r.w, lane, b0);
-    load_blk<NO>(a, na > 1 ? r.x + kBlk : r.z + (1 - na) * kBlk, na > 1 ? r.y : r.w, lane, b1);
+    const int na = (r.y - r.x + BLK - 1) / BLK;
+    load_blk<NO, BLK>(a, na > 0 ? r.x : r.z, na > 0 ? r.y : r.w, lane, b0);
+    load_blk<NO, BLK>(a, na > 1 ? r.x + BLK : r.z + (1 - na) * BLK, na > 1 ? r.y : r.w, lane, b1);
   }
 
   __device__ __forceinline__ void run(const BlkReg<NO>& b0, const BlkReg<NO>& b1, f32x16& accA, float& bpA,
                                       f32x16& accB, float& bpB) {
     if (nb == 0) return;
-    float v0[32], v1[32];
+    float v0[BLK / 2], v1[BLK / 2];
     nxt = b1;
     stage_blk<NO>(L, 0, lane, b0);
     lds_wait();
@@ -185,10 +191,15 @@ struct GramStream {
   }
 };
 
+// L of each half as its lower triangle, row i at i (i + 1) / 2; half 1
+// starts kTri32 floats in, a multiple of 32 banks away from half 0, so the
+// two halves' reads of one row in the back solve hit disjoint banks
+constexpr int kTri32 = 544;
+
 struct PairLds {
   alignas(16) float WT[2][4][32];   // current block columns of -P, per half (column-major)
-  alignas(16) float Wb[2][32];      // RHS, per half
-  alignas(16) float Lf[2][32][36];  // L per half, row i at Lf[half][i]
+  alignas(16) float Wb[2][4][32];   // RHS and the three probe vectors, per half
+  alignas(16) float Lf[2 * kTri32]; // L per half (lower triangle)
   alignas(16) float us[2][32];      // the new samples (SSE)
 };
 
@@ -208,7 +219,10 @@ struct PairLds {
 //      the panel rows of the lanes below the block, whose A/B operands are
 //      formed by v_permlane32_swap (rows of matrix A live in lanes 0-31, of
 //      matrix B in lanes 32-63).
-// L rows go to LDS as they are produced, for the back solve.
+// L rows go to LDS as they are produced, for the back solve.  With
+// refinement on, three probe vectors x (the row's normals z, all ones,
+// alternating signs) are forward-solved beside the RHS, giving |L^{-1} x|^2 =
+// x^T P^{-1} x for the condition estimate (probe_estimate).
 __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 accA, f32x16 accB, float bpA,
                              float bpB, int it, float alpha, PairLds& s, int nnzA, int nnzB) {
   int lane = threadIdx.x & 63;
@@ -244,7 +258,22 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
   }
   bool bad = false;
   float inv_diag = 1.f;  // 1 / L[i][i]
-  float pii = 0.f;       // |row i of L|^2 = P_ii
+  const float zi = i < a.K ? normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
+                                        make_tag(TAG_LATENT, a.mode))
+                           : 0.f;
+  // condition probes (refinement only): L^{-1} D^{1/2} x for x = z, ones,
+  // alternating signs, D = diag(P)
+  const bool probe = a.refine.count != nullptr && (nnzA >= a.refine.min_nnz || nnzB >= a.refine.min_nnz);
+  float x1 = zi, x2 = i < a.K ? 1.f : 0.f, x3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
+  if (probe) {
+    const float dA = frag_diag(accA, lane), dB = frag_diag(accB, lane);
+    const float sd = sqrtf(-(half ? dB : dA));  // the fragments hold -P
+    x1 *= sd;
+    x2 *= sd;
+    x3 *= sd;
+  }
+  const int tri_i = i * (i + 1) / 2;
+  float* const Lh = s.Lf + half * kTri32;
 #pragma unroll
   for (int k0 = 0; k0 < 32; k0 += 4) {
     opaque(i);
@@ -260,14 +289,19 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
             make_float4(accB[4 * mm], accB[4 * mm + 1], accB[4 * mm + 2], accB[4 * mm + 3]);
       }
     }
-    s.Wb[half][i] = b;
+    s.Wb[half][0][i] = b;
+    if (probe) {
+      s.Wb[half][1][i] = x1;
+      s.Wb[half][2][i] = x2;
+      s.Wb[half][3][i] = x3;
+    }
     lds_wait();
     const float w0 = -s.WT[half][0][i], w1 = -s.WT[half][1][i], w2 = -s.WT[half][2][i], w3 = -s.WT[half][3][i];
     const float4 c0 = *reinterpret_cast<const float4*>(&s.WT[half][0][k0]);  // -D[.][0]
     const float4 c1 = *reinterpret_cast<const float4*>(&s.WT[half][1][k0]);
     const float4 c2 = *reinterpret_cast<const float4*>(&s.WT[half][2][k0]);
     const float4 c3 = *reinterpret_cast<const float4*>(&s.WT[half][3][k0]);
-    const float4 bd = *reinterpret_cast<const float4*>(&s.Wb[half][k0]);
+    const float4 bd = *reinterpret_cast<const float4*>(&s.Wb[half][0][k0]);
     // L_D = chol(D), D[t][u] = -c_u.(t), lower entries
     const float a00 = -c0.x;
     bool bp = !(a00 > 0.f);
@@ -294,14 +328,33 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
     const float p1 = (w1 - p0 * l10) * r1;
     const float p2 = (w2 - p0 * l20 - p1 * l21) * r2;
     const float p3 = (w3 - p0 * l30 - p1 * l31 - p2 * l32) * r3;
-    *reinterpret_cast<float4*>(&s.Lf[half][i][k0]) = make_float4(p0, p1, p2, p3);
-    if (k0 <= i) pii += p0 * p0 + p1 * p1 + p2 * p2 + p3 * p3;
+    if (k0 <= i) {
+      float* lr = Lh + tri_i + k0;
+      lr[0] = p0;
+      if (k0 + 1 <= i) lr[1] = p1;
+      if (k0 + 2 <= i) lr[2] = p2;
+      if (k0 + 3 <= i) lr[3] = p3;
+    }
     const int t = i - k0;
     const bool inb = t >= 0 && t < 4, below = t >= 4;
     const float yt = t == 0 ? y0 : (t == 1 ? y1 : (t == 2 ? y2 : y3));
     const float rt = t == 0 ? r0 : (t == 1 ? r1 : (t == 2 ? r2 : r3));
     b = inb ? yt : (below ? b - (p0 * y0 + p1 * y1 + p2 * y2 + p3 * y3) : b);
     inv_diag = inb ? rt : inv_diag;
+    if (probe) {
+      // the same elimination step for the probe vectors
+      auto fwd = [&](float x, float4 xd) {
+        const float v0 = xd.x * r0;
+        const float v1 = (xd.y - l10 * v0) * r1;
+        const float v2 = (xd.z - l20 * v0 - l21 * v1) * r2;
+        const float v3 = (xd.w - l30 * v0 - l31 * v1 - l32 * v2) * r3;
+        const float vt = t == 0 ? v0 : (t == 1 ? v1 : (t == 2 ? v2 : v3));
+        return inb ? vt : (below ? x - (p0 * v0 + p1 * v1 + p2 * v2 + p3 * v3) : x);
+      };
+      x1 = fwd(x1, *reinterpret_cast<const float4*>(&s.Wb[half][1][k0]));
+      x2 = fwd(x2, *reinterpret_cast<const float4*>(&s.Wb[half][2][k0]));
+      x3 = fwd(x3, *reinterpret_cast<const float4*>(&s.Wb[half][3][k0]));
+    }
     // 2. -P += p p^T over the rows below the block, both matrices
     if (k0 + 4 < 32) {
       const float q0 = below ? p0 : 0.f, q1 = below ? p1 : 0.f, q2 = below ? p2 : 0.f, q3 = below ? p3 : 0.f;
@@ -318,42 +371,32 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
     }
   }
   if (bad && active && i == 0) atomicMax(a.fail, row + 1);
-  const float zi = i < a.K ? normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
-                                        make_tag(TAG_LATENT, a.mode))
-                           : 0.f;
   b += zi;
   const float w = b;  // L^{-1} b + z
-  // condition probes (refinement only): L^{-T} x for x = z, ones, alternating signs
-  const bool probe = a.refine.count != nullptr && (nnzA >= a.refine.min_nnz || nnzB >= a.refine.min_nnz);
-  float y1 = zi, y2 = i < a.K ? 1.f : 0.f, y3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
   // back solve L^T u = w.  Lane i needs column i of L (L[j][i], j > i),
-  // i.e. entries of other lanes' rows: each lane reads its column from the
-  // rows in LDS once; the solved values are broadcast with readlane.
+  // i.e. entries of other lanes' rows: it reads them from the rows in LDS,
+  // eight steps' worth at a time; the solved values are broadcast with
+  // readlane.  (For lanes i > j the read lands in later rows and is unused.)
   lds_wait();
-  float Lc[32];
 #pragma unroll
-  for (int j = 0; j < 32; j++) Lc[j] = s.Lf[half][j][i];
-#pragma unroll
-  for (int j = 31; j >= 0; j--) {
+  for (int g = 31; g >= 0; g -= 8) {
     opaque(i);
-    const float xl = b * inv_diag;  // final value on lane j
-    const float x0 = rdlane(xl, j), x1 = rdlane(xl, 32 + j);
-    const float xj = half ? x1 : x0;
-    b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
-    if (probe) {
-      const float p1 = rdlane(y1 * inv_diag, j), q1 = rdlane(y1 * inv_diag, 32 + j);
-      const float p2 = rdlane(y2 * inv_diag, j), q2 = rdlane(y2 * inv_diag, 32 + j);
-      const float p3 = rdlane(y3 * inv_diag, j), q3 = rdlane(y3 * inv_diag, 32 + j);
-      const float v1 = half ? q1 : p1, v2 = half ? q2 : p2, v3 = half ? q3 : p3;
-      y1 = (i < j) ? y1 - Lc[j] * v1 : (i == j ? v1 : y1);
-      y2 = (i < j) ? y2 - Lc[j] * v2 : (i == j ? v2 : y2);
-      y3 = (i < j) ? y3 - Lc[j] * v3 : (i == j ? v3 : y3);
+    float Lc[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) Lc[u] = Lh[(g - u) * (g - u + 1) / 2 + i];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int j = g - u;
+      const float xl = b * inv_diag;  // final value on lane j
+      const float xa = rdlane(xl, j), xb = rdlane(xl, 32 + j);
+      const float xj = half ? xb : xa;
+      b = (i < j) ? b - Lc[u] * xj : (i == j ? xj : b);
     }
   }
   if (active) a.U[(size_t)row * 32 + i] = b;
   if (probe) {
-    const float est = probe_estimate<32>(i, a.K, pii, y1, y2, y3, zi);
-    refine_enqueue<32, 32>(a, active ? row : -1, half ? nnzB : nnzA, i, half * 32, est, &s.Lf[half][i][0]);
+    const float est = probe_estimate<32>(a.K, x1, x2, x3, zi);
+    refine_enqueue<32, 32>(a, active ? row : -1, half ? nnzB : nnzA, i, half * 32, est, Lh + tri_i);
   }
   if (!a.sse_partial) return 0.f;
   // train SSE share of this lane: (w^2 - u (Lambda u)) / alpha - 2 u c (+ sum r^2)
@@ -381,9 +424,9 @@ __device__ float finish_pair(const SampleArgs& a, int rowA, int rowB, f32x16 acc
 // first two index blocks are loaded while the current pair is multiplied and
 // finished, so a pair's stream starts with its indices already in registers.
 template <int NO>
-__global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
+__global__ void __launch_bounds__(64, 4) k_light32(SampleArgs a, BatchPlan p) {
   __shared__ PairLds s;
-  __shared__ BlkLds<NO> blk;
+  __shared__ BlkLds<NO, kBlkLight> blk;
   const Factors<NO> F = make_factors<NO, 32>(a.others);
   const int it = *a.iter;
   const float alpha = (float)(*a.alpha);
@@ -396,7 +439,7 @@ __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
   if (q < p.n_pair) {
     rng = p.pair_rng[q];
     rows = p.pair_row[q];
-    GramStream<NO, true>::first_blocks(a, rng, lane, b0, b1);
+    GramStream<NO, true, kBlkLight>::first_blocks(a, rng, lane, b0, b1);
   }
   for (; q < p.n_pair; q += gridDim.x) {
     const int qn = q + gridDim.x;
@@ -413,8 +456,8 @@ __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
       accB[r] = 0.f;
     }
     float bpA = 0.f, bpB = 0.f;
-    GramStream<NO, true>(a, F, blk, rng.x, rng.y, rng.z, rng.w).run(b0, b1, accA, bpA, accB, bpB);
-    if (qn < p.n_pair) GramStream<NO, true>::first_blocks(a, rng_n, lane, b0, b1);
+    GramStream<NO, true, kBlkLight>(a, F, blk, rng.x, rng.y, rng.z, rng.w).run(b0, b1, accA, bpA, accB, bpB);
+    if (qn < p.n_pair) GramStream<NO, true, kBlkLight>::first_blocks(a, rng_n, lane, b0, b1);
     sse += (double)finish_pair(a, rows.x, rows.y, accA, accB, bpA, bpB, it, alpha, s, rng.y - rng.x,
                                rng.w - rng.z);
     rng = rng_n;
@@ -425,7 +468,7 @@ __global__ void __launch_bounds__(64, 3) k_light32(SampleArgs a, BatchPlan p) {
 
 template <int NO>
 __global__ void __launch_bounds__(64) k_seg32(SampleArgs a, BatchPlan p) {
-  __shared__ BlkLds<NO> blk;
+  __shared__ BlkLds<NO, kBlkSeg> blk;
   const Factors<NO> F = make_factors<NO, 32>(a.others);
   const int lane = threadIdx.x;
   for (int sg = blockIdx.x; sg < p.n_seg; sg += gridDim.x) {
@@ -435,8 +478,8 @@ __global__ void __launch_bounds__(64) k_seg32(SampleArgs a, BatchPlan p) {
     float bp = 0.f, unused_bp = 0.f;
     const int4 r = make_int4(p.seg_beg[sg], p.seg_end[sg], 0, 0);
     BlkReg<NO> b0, b1;
-    GramStream<NO, false>::first_blocks(a, r, lane, b0, b1);
-    GramStream<NO, false>(a, F, blk, r.x, r.y, 0, 0).run(b0, b1, acc, bp, unused, unused_bp);
+    GramStream<NO, false, kBlkSeg>::first_blocks(a, r, lane, b0, b1);
+    GramStream<NO, false, kBlkSeg>(a, F, blk, r.x, r.y, 0, 0).run(b0, b1, acc, bp, unused, unused_bp);
     float* out = p.partial + (size_t)sg * kPartial;
 #pragma unroll
     for (int r = 0; r < 16; r++) out[r * 64 + lane] = acc[r];
@@ -503,12 +546,13 @@ static void launch_batched_t(const SampleArgs& a, const BatchPlan& p, hipStream_
     const int gs = batched_seg_grid(p);
     hipLaunchKernelGGL((k_seg32<NO>), dim3(gs), dim3(64), 0, ss, a, p);
     launch_seg_sum<kPartial>(p, ss);
+    // the heavy rows depend only on their segments: finished on the segment
+    // stream too, beside the light rows
+    if (gh > 0) hipLaunchKernelGGL(k_heavy32, dim3(gh), dim3(64), 0, ss, a, p, gl);
     if (fork) seg_record_join(seg);
   }
-  launch_refine_reset(a, s);
   if (gl > 0) hipLaunchKernelGGL((k_light32<NO>), dim3(gl), dim3(64), 0, s, a, p);
   if (fork) seg_wait_join(s, seg);
-  if (gh > 0) hipLaunchKernelGGL(k_heavy32, dim3(gh), dim3(64), 0, s, a, p, gl);
   launch_refine(a, s);
 }
 

@@ -183,14 +183,16 @@ struct GramStream64 {
 };
 
 struct Fin64Lds {
-  alignas(16) float WT[4][64];    // current block columns of -P (column-major)
-  alignas(16) float Wb[64];       // RHS
-  alignas(16) float Lf[64][68];   // L rows
-  alignas(16) float us[64];       // the new sample (SSE)
+  alignas(16) float WT[4][64];     // current block columns of -P (column-major)
+  alignas(16) float Wb[4][64];     // RHS and the three probe vectors
+  alignas(16) float Lf[64 * 65 / 2];  // L, lower triangle, row i at i (i + 1) / 2
+  alignas(16) float us[64];        // the new sample (SSE)
 };
 
 // Finish one row whose Gram is in g (row < 0: no row); returns this lane's
-// SSE share.  Lane i owns row i of P.
+// SSE share.  Lane i owns row i of P.  With refinement on, three probe
+// vectors (z, all ones, alternating signs) are forward-solved beside the RHS
+// for the condition estimate (probe_estimate).
 __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float alpha, Fin64Lds& s, int nnz) {
   int lane = threadIdx.x & 63;
   opaque(lane);
@@ -225,7 +227,22 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
   }
   bool bad = false;
   float inv_diag = 1.f;
-  float pii = 0.f;  // |row i of L|^2 = P_ii
+  const float zi = i < a.K ? normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
+                                        make_tag(TAG_LATENT, a.mode))
+                           : 0.f;
+  // condition probes (refinement only): L^{-1} D^{1/2} x for x = z, ones,
+  // alternating signs, D = diag(P)
+  const bool probe = a.refine.count != nullptr && nnz >= a.refine.min_nnz;
+  float x1 = zi, x2 = i < a.K ? 1.f : 0.f, x3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
+  if (probe) {
+    // row i < 32 of P is in block (0, 0), row i >= 32 in block (1, 1)
+    const float d00 = frag_diag(g.g00, lane), d11 = frag_diag(g.g11, lane);
+    const float sd = sqrtf(-(half ? d11 : d00));  // the fragments hold -P
+    x1 *= sd;
+    x2 *= sd;
+    x3 *= sd;
+  }
+  const int tri_i = i * (i + 1) / 2;
 #pragma unroll
   for (int k0 = 0; k0 < 64; k0 += 4) {
     opaque(i);
@@ -247,14 +264,19 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
         }
       }
     }
-    s.Wb[i] = b;
+    s.Wb[0][i] = b;
+    if (probe) {
+      s.Wb[1][i] = x1;
+      s.Wb[2][i] = x2;
+      s.Wb[3][i] = x3;
+    }
     lds_wait();
     const float w0 = -s.WT[0][i], w1 = -s.WT[1][i], w2 = -s.WT[2][i], w3 = -s.WT[3][i];
     const float4 c0 = *reinterpret_cast<const float4*>(&s.WT[0][k0]);
     const float4 c1 = *reinterpret_cast<const float4*>(&s.WT[1][k0]);
     const float4 c2 = *reinterpret_cast<const float4*>(&s.WT[2][k0]);
     const float4 c3 = *reinterpret_cast<const float4*>(&s.WT[3][k0]);
-    const float4 bd = *reinterpret_cast<const float4*>(&s.Wb[k0]);
+    const float4 bd = *reinterpret_cast<const float4*>(&s.Wb[0][k0]);
     const float a00 = -c0.x;
     bool bp = !(a00 > 0.f);
     const float r0 = __builtin_amdgcn_rsqf(bp ? 1.f : a00);
@@ -279,14 +301,33 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
     const float p1 = (w1 - p0 * l10) * r1;
     const float p2 = (w2 - p0 * l20 - p1 * l21) * r2;
     const float p3 = (w3 - p0 * l30 - p1 * l31 - p2 * l32) * r3;
-    *reinterpret_cast<float4*>(&s.Lf[i][k0]) = make_float4(p0, p1, p2, p3);
-    if (k0 <= i) pii += p0 * p0 + p1 * p1 + p2 * p2 + p3 * p3;
+    if (k0 <= i) {
+      float* lr = s.Lf + tri_i + k0;
+      lr[0] = p0;
+      if (k0 + 1 <= i) lr[1] = p1;
+      if (k0 + 2 <= i) lr[2] = p2;
+      if (k0 + 3 <= i) lr[3] = p3;
+    }
     const int t = i - k0;
     const bool inb = t >= 0 && t < 4, below = t >= 4;
     const float yt = t == 0 ? y0 : (t == 1 ? y1 : (t == 2 ? y2 : y3));
     const float rt = t == 0 ? r0 : (t == 1 ? r1 : (t == 2 ? r2 : r3));
     b = inb ? yt : (below ? b - (p0 * y0 + p1 * y1 + p2 * y2 + p3 * y3) : b);
     inv_diag = inb ? rt : inv_diag;
+    if (probe) {
+      // the same elimination step for the probe vectors
+      auto fwd = [&](float x, float4 xd) {
+        const float v0 = xd.x * r0;
+        const float v1 = (xd.y - l10 * v0) * r1;
+        const float v2 = (xd.z - l20 * v0 - l21 * v1) * r2;
+        const float v3 = (xd.w - l30 * v0 - l31 * v1 - l32 * v2) * r3;
+        const float vt = t == 0 ? v0 : (t == 1 ? v1 : (t == 2 ? v2 : v3));
+        return inb ? vt : (below ? x - (p0 * v0 + p1 * v1 + p2 * v2 + p3 * v3) : x);
+      };
+      x1 = fwd(x1, *reinterpret_cast<const float4*>(&s.Wb[1][k0]));
+      x2 = fwd(x2, *reinterpret_cast<const float4*>(&s.Wb[2][k0]));
+      x3 = fwd(x3, *reinterpret_cast<const float4*>(&s.Wb[3][k0]));
+    }
     // 2. -P += p p^T over the rows below the block
     if (k0 + 4 < 64) {
       const float q0 = below ? p0 : 0.f, q1 = below ? p1 : 0.f, q2 = below ? p2 : 0.f, q3 = below ? p3 : 0.f;
@@ -306,35 +347,29 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
     }
   }
   if (bad && active && i == 0) atomicMax(a.fail, row + 1);
-  const float zi = i < a.K ? normal_f32(a.seed, (uint32_t)(active ? row : 0), (uint32_t)i, (uint32_t)it,
-                                        make_tag(TAG_LATENT, a.mode))
-                           : 0.f;
   b += zi;
   const float w = b;  // L^{-1} b + z
-  // condition probes (refinement only): L^{-T} x for x = z, ones, alternating signs
-  const bool probe = a.refine.count != nullptr && nnz >= a.refine.min_nnz;
-  float y1 = zi, y2 = i < a.K ? 1.f : 0.f, y3 = i < a.K ? ((i & 1) ? -1.f : 1.f) : 0.f;
-  // back solve L^T u = w (column i of L from the rows in LDS; readlane broadcast)
+  // back solve L^T u = w: column i of L read from the rows in LDS, eight
+  // steps' worth at a time (lanes i > j read later rows, unused); readlane
+  // broadcast of the solved values
   lds_wait();
-  float Lc[64];
 #pragma unroll
-  for (int j = 0; j < 64; j++) Lc[j] = s.Lf[j][i];
-#pragma unroll
-  for (int j = 63; j >= 0; j--) {
+  for (int g = 63; g >= 0; g -= 8) {
     opaque(i);
-    const float xj = rdlane(b * inv_diag, j);
-    b = (i < j) ? b - Lc[j] * xj : (i == j ? xj : b);
-    if (probe) {
-      const float v1 = rdlane(y1 * inv_diag, j), v2 = rdlane(y2 * inv_diag, j), v3 = rdlane(y3 * inv_diag, j);
-      y1 = (i < j) ? y1 - Lc[j] * v1 : (i == j ? v1 : y1);
-      y2 = (i < j) ? y2 - Lc[j] * v2 : (i == j ? v2 : y2);
-      y3 = (i < j) ? y3 - Lc[j] * v3 : (i == j ? v3 : y3);
+    float Lc[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) Lc[u] = s.Lf[(g - u) * (g - u + 1) / 2 + i];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int j = g - u;
+      const float xj = rdlane(b * inv_diag, j);
+      b = (i < j) ? b - Lc[u] * xj : (i == j ? xj : b);
     }
   }
   if (active) a.U[(size_t)row * 64 + i] = b;
   if (probe) {
-    const float est = probe_estimate<64>(i, a.K, pii, y1, y2, y3, zi);
-    refine_enqueue<64, 64>(a, active ? row : -1, nnz, i, 0, est, &s.Lf[i][0]);
+    const float est = probe_estimate<64>(a.K, x1, x2, x3, zi);
+    refine_enqueue<64, 64>(a, active ? row : -1, nnz, i, 0, est, s.Lf + tri_i);
   }
   if (!a.sse_partial) return 0.f;
   s.us[i] = b;
@@ -356,7 +391,7 @@ __device__ float finish64(const SampleArgs& a, int row, Gram64 g, int it, float 
 }
 
 template <int NO>
-__global__ void __launch_bounds__(64, 2) k_light64(SampleArgs a, BatchPlan p) {
+__global__ void __launch_bounds__(64, 3) k_light64(SampleArgs a, BatchPlan p) {
   __shared__ Fin64Lds s;
   __shared__ BlkLds64<NO> blk;
   const Factors<NO> F = make_factors<NO, 64>(a.others);
@@ -459,12 +494,13 @@ static void launch64_t(const SampleArgs& a, const BatchPlan& p, hipStream_t s, c
     const int gs = batched_seg_grid(p);
     hipLaunchKernelGGL((k_seg64<NO>), dim3(gs), dim3(64), 0, ss, a, p);
     launch_seg_sum<kPartial64>(p, ss);
+    // the heavy rows depend only on their segments: finished on the segment
+    // stream too, beside the light rows
+    if (gh > 0) hipLaunchKernelGGL(k_heavy64, dim3(gh), dim3(64), 0, ss, a, p, gl);
     if (fork) seg_record_join(seg);
   }
-  launch_refine_reset(a, s);
   if (gl > 0) hipLaunchKernelGGL((k_light64<NO>), dim3(gl), dim3(64), 0, s, a, p);
   if (fork) seg_wait_join(s, seg);
-  if (gh > 0) hipLaunchKernelGGL(k_heavy64, dim3(gh), dim3(64), 0, s, a, p, gl);
   launch_refine(a, s);
 }
 

@@ -60,12 +60,13 @@ def _is_sparse_matrix(Y):
 
 
 def make_train_test(Y, ntest):
-    """Splits a sparse matrix Y into a train and a test matrix.
-       Y      scipy sparse matrix (coo_matrix, csr_matrix or csc_matrix)
-       ntest  either a float below 1.0 or integer.
-              if float, then indicates the ratio of test cells
-              if integer, then indicates the number of test cells
-       returns Ytrain, Ytest (type coo_matrix)
+    """Random train/test partition of the observed cells of a sparse matrix
+    (macau.pyx:109-130).
+
+    Y is a scipy COO, CSR or CSC matrix.  ntest below 1 is the fraction of
+    Y's nonzeros that go to the test side, ntest of 1 or more their count
+    (rounded).  The cells are drawn with numpy's global generator; both
+    results are COO matrices of Y's shape, train first.
     """
     if not _is_sparse_matrix(Y):
         raise TypeError("Unsupported Y type: %s" % type(Y))
@@ -84,12 +85,12 @@ def make_train_test(Y, ntest):
 
 
 def make_train_test_df(Y, ntest):
-    """Splits rows of dataframe Y into a train and a test dataframe.
-       Y      pandas dataframe
-       ntest  either a float below 1.0 or integer.
-              if float, then indicates the ratio of test cells
-              if integer, then indicates the number of test cells
-       returns Ytrain, Ytest (type coo_matrix)
+    """Random train/test partition of the rows of a DataFrame in the cell
+    layout (index columns, then the value column; macau.pyx:132-152).
+
+    ntest below 1 is the fraction of rows that go to the test side, ntest of
+    1 or more their count (rounded).  Returns (train, test) as row subsets of
+    Y, in the permuted order of numpy's global generator.
     """
     if type(Y) != pd.core.frame.DataFrame:
         raise TypeError("Y should be DataFrame.")
@@ -299,32 +300,34 @@ def macau(Y,
           verbose=True,
           seed=None,
           sampler="auto"):
-    """
-    Matrix and tensor factorization with side information.
-      Y          training data to factorize (sparse matrix or DataFrame)
-      Ytest      either:
-                 number below 1.0, how much training data to move to test
-                 sparse matrix or DataFrame of test data
-      side       list of side information matrices.
-                 If Y is matrix, then need 2 side matrices (first for rows, second for columns)
-      lambda_beta  initial precision (regularization) for the link matrix
-                   connecting side matrices to Y
-      num_latent   number of latent dimensions
-      precision    precision of observations of Y, can be
-                   scalar     - specifies the precision (1 / noise_variance)
-                   "adaptive" - automatically learn precision
-      burnin       number of Gibbs samples to drop (burn-in)
-      nsamples     number of Gibbs samples to collect
-      univariate   use univariate (coordinate-wise) sampler for the modes with
-                   side information (MacauOnePrior)
-      tol          error tolerance for conjugate gradient solver
-      sn_max       maximum signal-to-noise ratio for observation precision
-                   only used if precision="adaptive"
-      save_prefix  prefix for model files or None if not saving the model
-      verbose      whether to print output for each Gibbs iteration
-      seed         key of the counter-based random streams (None: clock)
-      sampler      "auto" (default: "batched" where available for num_latent),
-                   "batched" (split-BF16 MFMA) or "plain" (FP32)
+    """Gibbs sampling of a matrix or tensor factorization, with side
+    information on any mode (macau.pyx:337-451); returns a MacauResult.
+
+    Y           observed training cells: a scipy sparse matrix, or a
+                DataFrame whose last column holds the values and whose other
+                columns (2 to 10) the integer index of each mode.
+    Ytest       the test cells in the same form as Y, or a number below 1:
+                that fraction of Y is held out at random as the test set.
+    side        one entry per mode (rows first): None, a dense array or a
+                scipy sparse matrix with as many rows as that mode has items.
+    lambda_beta starting precision of the link matrix beta between a side
+                matrix and the latents (sampled afterwards).
+    num_latent  latent dimension K.
+    precision   observation noise: a number (fixed precision), "adaptive"
+                (precision sampled from the residuals, capped by sn_max) or
+                "probit" (binary cells, latent Gaussian augmentation).
+    burnin, nsamples
+                iterations discarded, then averaged into the predictions.
+    univariate  coordinate-wise (MacauOnePrior) updates for modes with side
+                information instead of the joint beta draw.
+    tol         relative residual at which the block CG solve of beta stops.
+    sn_max      largest signal-to-noise ratio "adaptive" may reach.
+    save_prefix write the posterior-mean latents, link matrices and priors
+                under this prefix at the end (None: write nothing).
+    verbose     print one status line per iteration.
+    seed        key of the counter-based random streams (None: from the clock).
+    sampler     "auto" (the split-BF16 MFMA kernels where they support
+                num_latent, else the FP32 ones), "batched" or "plain".
     """
     data = Data(Y, Ytest)
 
