@@ -183,7 +183,15 @@ def time_sampler(name, w, steps, warmup, seed, log):
         flagged, refined = ctypes.c_longlong(), ctypes.c_longlong()
         check(L.macau_get_refine_stats(h, ctypes.byref(flagged), ctypes.byref(refined)))
         its = warmup + steps + nprof
-        cg.append({"rows_flagged_per_iteration": flagged.value / its, "rows_refined_per_iteration": refined.value / its})
+        shapes = []
+        if name != "plain":
+            for m in range(nm):
+                k = np.zeros(4, dtype=np.int32)
+                tuned = int(L.macau_get_tuning(h, m, ptr(k)))
+                shapes.append({"seg_len": int(k[0]), "light_cap": int(k[1]), "heavy_cap": int(k[2]),
+                               "seg_cap": int(k[3]), "from_tuning_table": bool(tuned == 1)})
+        cg.append({"rows_flagged_per_iteration": flagged.value / its, "rows_refined_per_iteration": refined.value / its,
+                   "launch_shapes": shapes})
         log("%s: %d steps in %.3f s -> %.1f it/s; phases(ms) %s" %
             (name, steps, t1 - t0, steps / (t1 - t0), np.round(phase, 3).tolist()))
     finally:
@@ -616,7 +624,10 @@ def main():
     if tuning is not None:
         out["tuning"] = tuning
     if head.startswith("batched"):
-        out["refinement"] = dict(res[head]["cg"][-1], threshold="estimated condition number > 500, rows of >= max(K, 128) nonzeros")
+        st_ = dict(res[head]["cg"][-1])
+        out["launch_shapes"] = st_.pop("launch_shapes")
+        out["refinement"] = dict(st_, threshold="estimated condition number of the Jacobi-scaled precision > 500, "
+                                                "rows of >= max(K, 128) nonzeros")
     if "plain" in res and head != "plain":
         out["plain_iters_per_sec"] = 1.0 / res["plain"]["sec"]
         out["speedup_vs_plain"] = res["plain"]["sec"] / sec

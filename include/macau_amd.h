@@ -125,11 +125,12 @@ int macau_get_ftf_solver(macau_model* m, int mode);
 /* ---- refinement of ill-conditioned rows (batched sampler; no reference
  *      counterpart: it guards the reference's FP64 solve,
  *      latentprior.cpp:373-384, against the FP32 / split-BF16 arithmetic) ----
- * After each row's Cholesky the batched sampler estimates the row's
- * conditioning with three probe solves L^{-T} x beside the sample's back
- * solve (x = the row's normals, ones, alternating signs):
- * est = K max_i P_ii max_x |L^{-T} x|^2 / |x|^2, an estimate of kappa(P); a
- * row above `threshold` (default 500) with at least max(num_latent, 128)
+ * Beside each row's Cholesky the batched sampler estimates the row's
+ * conditioning with three probe solves y = L^{-1} D^{1/2} x (D = diag P;
+ * x = the row's normals, ones, alternating signs) done with the forward
+ * solve: est = K max_x |y|^2 / |x|^2, an estimate of the condition number
+ * of the Jacobi-scaled D^{-1/2} P D^{-1/2} (K for a diagonal P); a row
+ * above `threshold` (default 500) with at least max(num_latent, 128)
  * nonzeros (fewer: not probed) gets one step
  * of iterative refinement against the exact precision matrix (residual from
  * per-observation residuals, FP64 sums).  macau_set_refine (before

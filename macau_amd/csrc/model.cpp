@@ -222,9 +222,17 @@ Model::Model(int nm, int K, std::unique_ptr<INoiseModel> nz)
   if (nm < 2 || nm > kMaxOther + 1) throw std::runtime_error("number of modes must be in [2, 10]");
 }
 
+void Model::drop_graphs() {
+  for (int v = 0; v < 2; v++) {
+    if (graph_exec[v]) (void)hipGraphExecDestroy(graph_exec[v]);
+    if (graph[v]) (void)hipGraphDestroy(graph[v]);
+    graph_exec[v] = nullptr;
+    graph[v] = nullptr;
+  }
+}
+
 Model::~Model() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  drop_graphs();
   if (stream) {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
@@ -239,6 +247,8 @@ Model::~Model() {
   }
   for (auto& e : ev_fork) (void)hipEventDestroy(e);
   if (ev_join) (void)hipEventDestroy(ev_join);
+  if (ev_start) (void)hipEventDestroy(ev_start);
+  if (ev_plast) (void)hipEventDestroy(ev_plast);
   for (auto& e : ev_seg_fork) (void)hipEventDestroy(e);
   for (auto& e : ev_seg_join) (void)hipEventDestroy(e);
 }
@@ -267,13 +277,15 @@ void Model::init() {
     ev_fork.resize(nmodes);
     for (auto& e : ev_fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_start, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_plast, hipEventDisableTiming));
     ev_seg_fork.resize(nmodes);
     ev_seg_join.resize(nmodes);
     for (auto& e : ev_seg_fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : ev_seg_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
-  if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-  if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  drop_graphs();
+  prior_pending = false;
   d_rowptr.clear(); d_idx.clear(); d_r.clear(); d_U.clear(); rdev.clear();
   for (int m = 0; m < nmodes; m++) {
     const HostRows& h = rows[m];
@@ -297,6 +309,7 @@ void Model::init() {
     HIPCHK(hipStreamSynchronize(stream));  // host vectors r go out of scope
   }
   upload_scalar<int>(d_iter, 0, stream);
+  upload_scalar<int>(d_iter_prior, 0, stream);
   upload_scalar<int>(d_fail, 0, stream);
   d_eval_out.alloc(2 * sizeof(double));
   d_sse.alloc(sizeof(double));
@@ -391,7 +404,9 @@ RefineQueue make_refine_queue(std::vector<DevBuf>& B, int KP, int cap, float thr
 }
 
 void Model::set_iteration(int it) {
+  launch_pending_prior();
   upload_scalar<int>(d_iter, it, stream);
+  upload_scalar<int>(d_iter_prior, it, stream);
   host_iter = it;
 }
 
@@ -516,27 +531,60 @@ struct OnSideStream {
 };
 }  // namespace
 
-void Model::step_launch() {
+void Model::step_launch(bool defer_last, bool run_pending) {
   // The reference samples every mode, then updates every prior
   // (macau.cpp:120-128).  The update of mode m reads only mode m's new
-  // samples and its draws are keyed by (iteration, mode), so it is launched
-  // on the side stream as soon as mode m is sampled and runs while the next
-  // modes are sampled (for the last mode, while the noise and the test
-  // predictions are updated).  The streams join before the iteration
-  // counter advances, so the next iteration sees every update.
+  // samples and is read only by mode m's next sampling; its draws are keyed
+  // by (iteration, mode) through d_iter_prior.  So it is launched on the side
+  // stream as soon as mode m is sampled and runs while the next modes are
+  // sampled.  The last mode's update either runs beside the noise and the
+  // test predictions, or, with defer_last, is left for the next launch,
+  // where it runs beside the first modes' sampling and the last mode waits
+  // for it (run_pending): an iteration then ends with its evaluation instead
+  // of the last Normal-Wishart draw.  Same data dependencies, same draws,
+  // same results.
+  const int last = nmodes - 1;
+  if (run_pending) {
+    HIPCHK(hipEventRecord(ev_start, stream));
+    HIPCHK(hipStreamWaitEvent(side, ev_start, 0));
+    {
+      OnSideStream guard(*this);
+      priors[last]->update_prior(*this, last);
+    }
+    launch_advance_iter(d_iter_prior.as<int>(), side);
+    HIPCHK(hipEventRecord(ev_plast, side));
+  }
   for (int m = 0; m < nmodes; m++) {
+    if (m == last && run_pending) HIPCHK(hipStreamWaitEvent(stream, ev_plast, 0));
     noise->augment(*this, m);
     priors[m]->sample_latents(*this, m);
+    if (m == last && defer_last) break;
     HIPCHK(hipEventRecord(ev_fork[m], stream));
     HIPCHK(hipStreamWaitEvent(side, ev_fork[m], 0));
     OnSideStream guard(*this);
     priors[m]->update_prior(*this, m);
+    if (m == last) launch_advance_iter(d_iter_prior.as<int>(), stream);  // (the side stream, swapped in)
   }
   noise->update(*this);
   noise->evalModel(*this);
   HIPCHK(hipEventRecord(ev_join, side));
   HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   launch_advance_iter(d_iter.as<int>(), stream);
+}
+
+void Model::launch_pending_prior() {
+  if (!prior_pending) return;
+  const int last = nmodes - 1;
+  HIPCHK(hipEventRecord(ev_start, stream));
+  HIPCHK(hipStreamWaitEvent(side, ev_start, 0));
+  {
+    OnSideStream guard(*this);
+    priors[last]->update_prior(*this, last);
+  }
+  launch_advance_iter(d_iter_prior.as<int>(), side);
+  HIPCHK(hipEventRecord(ev_join, side));
+  HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+  prior_pending = false;
 }
 
 void Model::iterate(int n) {
@@ -548,18 +596,23 @@ void Model::iterate(int n) {
   }
   for (int i = 0; i < n; i++) {
     if (graphable) {
-      if (!graph_exec) {
+      // each graph defers its last prior update to the next launch; the
+      // update deferred by the last one is enqueued before iterate returns
+      const int v = prior_pending ? 1 : 0;
+      if (!graph_exec[v]) {
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        step_launch();
-        HIPCHK(hipStreamEndCapture(stream, &graph));
-        HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        step_launch(true, v == 1);
+        HIPCHK(hipStreamEndCapture(stream, &graph[v]));
+        HIPCHK(hipGraphInstantiate(&graph_exec[v], graph[v], nullptr, nullptr, 0));
       }
-      HIPCHK(hipGraphLaunch(graph_exec, stream));
+      HIPCHK(hipGraphLaunch(graph_exec[v], stream));
+      prior_pending = true;
     } else {
       step_launch();
     }
     host_iter++;
   }
+  launch_pending_prior();
 }
 
 std::vector<double> Model::profile(int n) {
@@ -578,6 +631,7 @@ std::vector<double> Model::profile(int n) {
     }
     for (int m = 0; m < nmodes; m++) {
       priors[m]->update_prior(*this, m);
+      if (m == nmodes - 1) launch_advance_iter(d_iter_prior.as<int>(), stream);
       HIPCHK(hipEventRecord(E[q++], stream));
     }
     noise->update(*this);
@@ -812,7 +866,7 @@ static NWArgs nw_args(Model& m, ILatentPrior& p, int mode, double nu) {
   a.WI = p.WI.as<double>();
   a.nu = nu;
   a.seed = m.seed;
-  a.iter = m.d_iter.as<int>();
+  a.iter = m.d_iter_prior.as<int>();
   a.mode = mode;
   a.ws = p.nw_ws.as<double>();
   a.mu = p.mu.as<double>();
@@ -1082,7 +1136,7 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   b.Lchol = Lchol.as<double>();
   b.lambda_beta = lambda_beta_d.as<double>();
   b.seed = m.seed;
-  b.iter = m.d_iter.as<int>();
+  b.iter = m.d_iter_prior.as<int>();
   b.mode = mode;
   launch_beta_tmp(b, tmp.as<double>(), s);
   launch_spmm(Ft(), tmp.as<double>(), K, Fty.as<double>(), nullptr, nullptr, s);
@@ -1123,7 +1177,7 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   // lambda_beta = sample_lambda_beta(beta, Lambda, nu0, mu0)
   launch_hyper_stats_f64(beta.as<double>(), K, nfeat, K, bbt.as<double>(), bbt_nblk, s);
   launch_lambda_beta(K, (double)nfeat, bbt.as<double>(), 1, Lambda.as<double>(), lambda_beta_d.as<double>(),
-                     m.seed, m.d_iter.as<int>(), mode, s);
+                     m.seed, m.d_iter_prior.as<int>(), mode, s);
 }
 
 double MacauPrior::getLinkNorm(Model& m) {
@@ -1277,7 +1331,7 @@ void MacauOnePrior::update_prior(Model& m, int mode) {  // macauoneprior.cpp:189
   b.batch_meta = dbatch.as<int4>();
   b.nbatch = (int)(batch_meta.size() / 4);
   b.seed = m.seed;
-  b.iter = m.d_iter.as<int>();
+  b.iter = m.d_iter_prior.as<int>();
   b.mode = mode;
   launch_beta_uni(b, s);
   // compute_uhat, with the per-row prior term lambda (mu + Uhat_n) for the

@@ -309,6 +309,7 @@ class Model {
   hipStream_t side = nullptr;  // prior updates overlapped with sampling (step_launch)
   std::vector<hipEvent_t> ev_fork;
   hipEvent_t ev_join = nullptr;
+  hipEvent_t ev_start = nullptr, ev_plast = nullptr;  // deferred last-mode prior update (step_launch)
   // the batched sampler's heavy-row segment kernels, beside its light rows
   hipStream_t seg_stream = nullptr;
   std::vector<hipEvent_t> ev_seg_fork, ev_seg_join;
@@ -316,6 +317,10 @@ class Model {
   std::vector<RowsDev> rdev;
   std::vector<BatchPlan> plans;
   std::vector<std::vector<DevBuf>> plan_bufs;
+  // d_iter keys the sampling, noise and evaluation draws; d_iter_prior the
+  // prior updates' (it advances after the last mode's prior update, which
+  // can run after d_iter has advanced: see step_launch)
+  DevBuf d_iter_prior;
   DevBuf d_iter, d_alpha, d_fail, d_eval_out, d_test_idx, d_test_vals, d_pred, d_pred_var, d_eval_partial,
       d_noise_partial, d_norm_partial, d_sse, d_r2_last, d_sse_partial;
   int sse_partial_n = 0;       // fused SSE partials of the batched sampler (last mode)
@@ -323,13 +328,17 @@ class Model {
   int eval_nblk = 0, noise_nblk = 0;
   bool initialized = false;
   int host_iter = 0;  // iterations launched so far
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
+  // iteration graphs: [0] without, [1] with the previous iteration's
+  // deferred last-mode prior update at its start
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  bool prior_pending = false;  // the last mode's prior update of the last launched iteration is deferred
+  void drop_graphs();
 
   FactorSet others_of(int mode) const;
   SampleArgs sample_args(int mode, const ILatentPrior& p);
-  // batched sampler: one refinement step for rows flagged by the pivot-ratio
-  // condition estimate (refine.hip); settings read at init
+  // batched sampler: one refinement step for rows flagged by the probe
+  // condition estimate (batched.h, refine.hip); settings read at init
   bool refine = true;
   float refine_thresh = kRefineThresh;
   int refine_min_nnz = -1;  // < 0: max(num_latent, kRefineMinNnz)
@@ -344,7 +353,11 @@ class Model {
   // times the candidate shapes of every mode, keeps the fastest, returns
   // (default ms, chosen ms) per mode and appends the choices to record_path
   std::vector<std::pair<double, double>> autotune(int reps, const std::string& record_path);
-  void step_launch();  // enqueue one Gibbs iteration
+  // enqueue one Gibbs iteration; defer_last leaves the last mode's prior
+  // update for the next launch (prior_pending), run_pending runs the one
+  // deferred by the previous launch at the start of this one
+  void step_launch(bool defer_last = false, bool run_pending = false);
+  void launch_pending_prior();  // enqueue the deferred prior update, if any
   double factor_norm(int mode);
   double host_eval_out(int which);
 

@@ -23,8 +23,10 @@
 // a refined row (the change is second order in the correction).
 //
 // The queue needs no separate reset launch: the last work-group of k_refine
-// to finish (a ticket counter beside the queue count) folds the half-sweep's
-// count into the running totals and clears the count for the next one.
+// to finish (a ticket counter beside the queue count, taken by the
+// work-groups that have rows to refine, or by work-group 0 alone when the
+// queue is empty) folds the half-sweep's count into the running totals and
+// clears the count for the next one.
 #include "batched.h"
 
 namespace mgpu {
@@ -42,6 +44,10 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(SampleArgs a) {
   const double alpha = *a.alpha;
   const int queued = *(volatile const int*)q.count;
   const int nq = queued < q.cap ? queued : q.cap;
+  // work-groups past the queued rows leave at once and take no ticket (with
+  // an empty queue only work-group 0 takes one and clears the count)
+  const int nact = nq < 1 ? 1 : (nq < (int)gridDim.x ? nq : (int)gridDim.x);
+  if ((int)blockIdx.x >= nact) return;
   for (int e = blockIdx.x; e < nq; e += gridDim.x) {
     const int row = q.row[e];
     const float* Lq = q.L + (size_t)e * KP * KP;
@@ -128,7 +134,7 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(SampleArgs a) {
   // one folds the count into the totals and clears the queue
   if (t == 0) {
     __threadfence();
-    if (atomicAdd(q.count + 1, 1) == (int)gridDim.x - 1) {
+    if (atomicAdd(q.count + 1, 1) == nact - 1) {
       q.totals[0] += queued;
       q.totals[1] += nq;
       *(volatile int*)q.count = 0;

@@ -131,9 +131,9 @@ std::vector<std::pair<double, double>> Model::autotune(int reps, const std::stri
   std::vector<std::pair<double, double>> out(nmodes, {0.0, 0.0});
   if (sampler != SAMPLER_BATCHED) return out;
   if (reps < 1) reps = 1;
-  // the iteration graph holds the old launch shapes
-  if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-  if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  // the iteration graphs hold the old launch shapes
+  drop_graphs();
+  launch_pending_prior();
   HIPCHK(hipStreamSynchronize(stream));
   std::vector<DevBuf> saved(nmodes);
   for (int m = 0; m < nmodes; m++) {

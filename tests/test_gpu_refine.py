@@ -1,5 +1,6 @@
 """One step of iterative refinement for ill-conditioned rows of the batched
-sampler (gated by the pivot-ratio condition estimate).
+sampler (gated by the probe condition estimate of the Jacobi-scaled
+precision matrix).
 
 Rows of 2000 to 27000 nonzeros whose factor columns are nearly collinear
 (one direction of the other mode's factors scaled by 0.03, condition numbers
@@ -54,6 +55,35 @@ def test_refinement_brings_ill_conditioned_heavy_rows_within_1e4(K):
     queued = degs >= max(K, 128)
     assert err[_lib.SAMPLER_BATCHED][queued].max() <= 1e-4
     np.testing.assert_array_equal(err[_lib.SAMPLER_BATCHED][~queued], err[_lib.SAMPLER_BATCHED_NOREFINE][~queued])
+
+
+@pytest.mark.parametrize("K,lo,hi", [(32, 0.1, 1.0), (64, 0.2, 1.5)])
+def test_spread_component_scales_are_not_queued(K, lo, hi):
+    """Rows whose latent components have very different scales but no weak
+    direction (kappa ~ 60-100, as a chain's factors develop them) stay out
+    of the queue: the probe estimate works on the Jacobi-scaled matrix, for
+    which such rows read ~K (an estimate with max_i P_ii in place of the
+    scaling reads 700-1100 here, above the threshold of 500).  Without a
+    queued row the refined sampler's output equals the unrefined one bit for
+    bit, and both are within 1e-4 of the oracle."""
+    announce("spread component scales K=%d" % K)
+    rng = np.random.default_rng(K + 7)
+    ncol = 30000
+    scale = np.exp(np.linspace(np.log(lo), np.log(hi), K))
+    V = f32(rng.standard_normal((ncol, K)) * scale * 0.5)
+    degs = [2000, 4000, 8000, 27000, 300, 150]
+    r = np.concatenate([np.full(d, i) for i, d in enumerate(degs)])
+    c = np.concatenate([rng.choice(ncol, d, replace=False) for d in degs])
+    y = np.sum(rng.standard_normal((len(degs), K))[r] * V[c], 1) + 0.3 * rng.standard_normal(len(r))
+    modes, mean, _ = O.matrix_rows(r, c, y, len(degs), ncol)
+    Lam, mu = spd(rng, K, 1.0), np.zeros(K)
+    ref = O.sample_latents(K, modes[0], mean, [V], 10.0, mu, Lam, SEED, 3, 0)
+    a = gpu_sample_latents(K, modes[0], mean, [V], 10.0, mu, Lam, SEED, 3, 0, _lib.SAMPLER_BATCHED)
+    b = gpu_sample_latents(K, modes[0], mean, [V], 10.0, mu, Lam, SEED, 3, 0, _lib.SAMPLER_BATCHED_NOREFINE)
+    np.testing.assert_array_equal(a, b)
+    err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print("K=%d spread scales: row errors %s" % (K, " ".join("%.1e" % e for e in err)))
+    assert err.max() <= 1e-4
 
 
 def test_refinement_counts_and_switch():
