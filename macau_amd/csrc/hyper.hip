@@ -610,6 +610,161 @@ __global__ void __launch_bounds__(kNWThreadsW) k_nw_draw_w(NWBatch b) {
   nw_draw_w(b.m[blockIdx.x], s);
 }
 
+// The same draw for 64 < K <= 128 with every matrix in LDS (nw_draw keeps
+// them in a global workspace there: its per-step barriers then wait on
+// global memory, ~2.5 ms at K = 128).  Triangles are packed: M = chol(J X J)
+// lower, row i at i (i + 1) / 2; the Bartlett factor C upper, column j at
+// j (j + 1) / 2.  Row i of Z = M^{-T} J C^T only needs column K - 1 - i of C
+// and is written over it, and B = J Z is then the packed lower triangle
+// read row by row; Lambda = B B^T takes M's place.  Every value goes through
+// the same operations in the same order as in nw_draw.  LDS:
+// (K (K + 1) + 3 K) doubles, 132 KB at K = 128.
+__device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
+
+__global__ void __launch_bounds__(kNWThreads) k_nw_draw_lds(NWBatch b) {
+  extern __shared__ double dyn[];
+  const NWArgs& a = b.m[blockIdx.x];
+  __shared__ int sflag;
+  const int K = a.K, KK = K * K, t = threadIdx.x, T2 = tri(K);
+  double* Mp = dyn;         // J X J, then M (packed lower), then Lambda (packed lower)
+  double* Cp = Mp + T2;     // C (packed upper, by columns), then Z rows, i.e. B (packed lower)
+  double* v_mu = Cp + T2;
+  double* v_z = v_mu + K;
+  double* v_nu = v_z + K;
+  if (t == 0) sflag = 0;
+  const int it = *a.iter;
+  const double N = a.N;
+  const double kappa_c = a.kappa + N, nu_c = a.nu + N;
+  const size_t stride = (size_t)(K + KK);
+  auto reduce = [&](int q) {  // the statistic q summed over the blocks, in block order
+    double sum = 0.0;
+    int bb = 0;
+    for (; bb + 8 <= a.nblk; bb += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = a.partial[(size_t)(bb + u) * stride + q];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sum += v[u];
+    }
+    for (; bb < a.nblk; bb++) sum += a.partial[(size_t)bb * stride + q];
+    return sum;
+  };
+  // 1. sums of the columns, mu_c
+  for (int q = t; q < K; q += kNWThreads) v_nu[q] = reduce(q);
+  __syncthreads();
+  for (int k = t; k < K; k += kNWThreads) v_mu[k] = (a.kappa * a.mu0[k] + v_nu[k]) / (a.kappa + N);
+  const double lb = a.lambda_beta ? *a.lambda_beta : 0.0;
+  __syncthreads();
+  // 2. J X J (lower: entry (r, c) = X[K-1-r][K-1-c]), the Bartlett factor, z
+  for (int e = t; e < T2; e += kNWThreads) {
+    int r = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (tri(r) > e) r--;
+    while (tri(r + 1) <= e) r++;
+    const int c = e - tri(r);
+    const int i = K - 1 - r, j = K - 1 - c, q = i + j * K;
+    double tq = a.WI[q];
+    if (a.BBt_partial) {
+      double sb = 0.0;
+      for (int bb = 0; bb < a.bbt_nblk; bb++) sb += a.BBt_partial[(size_t)bb * stride + K + q];
+      tq += lb * sb;
+    }
+    Mp[e] = tq + reduce(K + q) + a.kappa * a.mu0[i] * a.mu0[j] - kappa_c * v_mu[i] * v_mu[j];
+    // C[c'][j'] of packed slot e (column j' = r, row c' = c)
+    double cv;
+    if (c == r)
+      cv = sqrt(2.0 * gamma_f64(a.seed, 0.5 * (nu_c - c), (uint32_t)c, (uint32_t)it, make_tag(TAG_WISHART_CHI, a.mode)));
+    else
+      cv = normal_f64(a.seed, (uint32_t)c, (uint32_t)r, (uint32_t)it, make_tag(TAG_WISHART_Z, a.mode));
+    Cp[e] = cv;
+  }
+  for (int k = t; k < K; k += kNWThreads)
+    v_z[k] = normal_f64(a.seed, 0u, (uint32_t)k, (uint32_t)it, make_tag(TAG_NW_MU, a.mode));
+  __syncthreads();
+  // 3. M = chol(J X J), right-looking; the trailing update on a 16 x 16
+  //    thread grid
+  const int tx = t & 15, ty = t >> 4;
+  for (int k = 0; k < K; k++) {
+    if (t == 0) {
+      const double d = Mp[tri(k) + k];
+      if (!(d > 0.0)) sflag = 1;
+      Mp[tri(k) + k] = sqrt(d > 0.0 ? d : 1.0);
+    }
+    __syncthreads();
+    const double inv = 1.0 / Mp[tri(k) + k];
+    for (int i = k + 1 + t; i < K; i += kNWThreads) Mp[tri(i) + k] *= inv;
+    __syncthreads();
+    for (int i = k + 1 + ty; i < K; i += 16) {
+      const double lik = Mp[tri(i) + k];
+#pragma unroll 4
+      for (int j = k + 1 + tx; j <= i; j += 16) Mp[tri(i) + j] -= lik * Mp[tri(j) + k];
+    }
+    __syncthreads();
+  }
+  const bool ok = sflag == 0;
+  // 4. Z = M^{-T} R, R = J C^T, rows from the bottom; row i over column
+  //    K - 1 - i of C (slots tri(K-1-i) + c, c <= K - 1 - i)
+  for (int i = K - 1; i >= 0; i--) {
+    const int j = K - 1 - i;
+    for (int c = t; c <= j; c += kNWThreads) {
+      double sv = Cp[tri(j) + c];  // C[c][K-1-i]
+      // (unrolled so that the loads of eight steps are issued together; the
+      // subtractions keep their order)
+#pragma unroll 8
+      for (int k = i + 1; k + c <= K - 1; k++) sv -= Mp[tri(k) + i] * Cp[tri(K - 1 - k) + c];
+      Cp[tri(j) + c] = sv / Mp[tri(i) + i];
+    }
+    __syncthreads();
+  }
+  // B[i][c] = Z[K-1-i][c] = Cp[tri(i) + c] (c <= i).
+  // 5. Lambda = B B^T (packed lower, over M)
+  for (int e = t; e < T2; e += kNWThreads) {
+    int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (tri(i) > e) i--;
+    while (tri(i + 1) <= e) i++;
+    const int j = e - tri(i);
+    double sv = 0.0;
+#pragma unroll 8
+    for (int k = 0; k <= j; k++) sv += Cp[tri(i) + k] * Cp[tri(j) + k];
+    Mp[e] = sv;
+  }
+  __syncthreads();
+  // 6. mu = mu_c + B^{-T} z / sqrt(kappa_c)
+  for (int j = K - 1; j >= 0; j--) {
+    if (t == 0) v_z[j] = v_z[j] / Cp[tri(j) + j];
+    __syncthreads();
+    const double xj = v_z[j];
+    for (int i = t; i < j; i += kNWThreads) v_z[i] -= Cp[tri(j) + i] * xj;
+    __syncthreads();
+  }
+  const double isk = 1.0 / sqrt(kappa_c);
+  for (int k = t; k < K; k += kNWThreads) {
+    const double m = v_mu[k] + v_z[k] * isk;
+    a.mu[k] = m;
+    v_mu[k] = m;
+  }
+  __syncthreads();
+  // 7. outputs
+  const int KP = a.KP;
+  for (int q = t; q < KK; q += kNWThreads) {
+    const int i = q % K, j = q / K;
+    a.Lambda[q] = i >= j ? Mp[tri(i) + j] : Mp[tri(j) + i];
+    if (a.Lchol) a.Lchol[q] = i >= j ? Cp[tri(i) + j] : 0.0;
+  }
+  for (int q = t; q < KP * KP; q += kNWThreads) {
+    const int i = q % KP, j = q / KP;
+    float v = (i == j) ? 1.f : 0.f;
+    if (i < K && j < K) v = (float)(i >= j ? Mp[tri(i) + j] : Mp[tri(j) + i]);
+    a.Lambda_f[q] = v;
+  }
+  for (int i = t; i < KP; i += kNWThreads) {
+    double sv = 0.0;
+    if (i < K)
+      for (int j = 0; j < K; j++) sv += (i >= j ? Mp[tri(i) + j] : Mp[tri(j) + i]) * v_mu[j];
+    a.Lmu_f[i] = (float)sv;
+  }
+  if (t == 0 && !ok && a.fail) atomicMax(a.fail, kFailNormalWishart);
+}
+
 // one work-group per mode of the batch
 __global__ void __launch_bounds__(kNWThreads) k_nw_draw(NWBatch b) {
   extern __shared__ double dyn[];
@@ -628,6 +783,30 @@ void launch_nw_draw_batch(const NWBatch& b, hipStream_t s) {
   if (wave) {
     hipLaunchKernelGGL(k_nw_draw_w, dim3(b.n), dim3(kNWThreadsW), 0, s, b);
     return;
+  }
+  // 64 < K <= 128: the LDS-resident draw where the device grants a
+  // work-group that much LDS (checked once per device)
+  const int K = b.m[0].K;
+  if (K > kNWW && K <= 128) {
+    const size_t bytes = (size_t)(K * (K + 1) + 3 * K) * sizeof(double);
+    static int granted[64] = {0};  // per device: 0 unknown, 1 yes, -1 no
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
+      if (granted[dev] == 0) {
+        int optin = 0;
+        granted[dev] = -1;
+        if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess &&
+            (size_t)optin >= (size_t)(128 * 129 + 3 * 128) * sizeof(double) &&
+            hipFuncSetAttribute((const void*)k_nw_draw_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((size_t)(128 * 129 + 3 * 128) * sizeof(double))) == hipSuccess)
+          granted[dev] = 1;
+        (void)hipGetLastError();
+      }
+      if (granted[dev] == 1) {
+        hipLaunchKernelGGL(k_nw_draw_lds, dim3(b.n), dim3(kNWThreads), bytes, s, b);
+        return;
+      }
+    }
   }
   const bool lds = b.m[0].ws == nullptr;
   hipLaunchKernelGGL(k_nw_draw, dim3(b.n), dim3(kNWThreads), lds ? nw_lds_bytes(b.m[0].K) : 0, s, b);

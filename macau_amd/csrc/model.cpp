@@ -249,6 +249,7 @@ Model::~Model() {
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (ev_start) (void)hipEventDestroy(ev_start);
   if (ev_plast) (void)hipEventDestroy(ev_plast);
+  if (ev_noise) (void)hipEventDestroy(ev_noise);
   for (auto& e : ev_seg_fork) (void)hipEventDestroy(e);
   for (auto& e : ev_seg_join) (void)hipEventDestroy(e);
 }
@@ -279,6 +280,7 @@ void Model::init() {
     HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_start, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_plast, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_noise, hipEventDisableTiming));
     ev_seg_fork.resize(nmodes);
     ev_seg_join.resize(nmodes);
     for (auto& e : ev_seg_fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -565,7 +567,14 @@ void Model::step_launch(bool defer_last, bool run_pending) {
     priors[m]->update_prior(*this, m);
     if (m == last) launch_advance_iter(d_iter_prior.as<int>(), stream);  // (the side stream, swapped in)
   }
-  noise->update(*this);
+  // the noise precision reads only the sampled factors (or the SSE the last
+  // half-sweep fused): drawn on the side stream beside the evaluation
+  HIPCHK(hipEventRecord(ev_noise, stream));
+  HIPCHK(hipStreamWaitEvent(side, ev_noise, 0));
+  {
+    OnSideStream guard(*this);
+    noise->update(*this);
+  }
   noise->evalModel(*this);
   HIPCHK(hipEventRecord(ev_join, side));
   HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));

@@ -310,6 +310,7 @@ class Model {
   std::vector<hipEvent_t> ev_fork;
   hipEvent_t ev_join = nullptr;
   hipEvent_t ev_start = nullptr, ev_plast = nullptr;  // deferred last-mode prior update (step_launch)
+  hipEvent_t ev_noise = nullptr;  // the noise draw forked beside the evaluation
   // the batched sampler's heavy-row segment kernels, beside its light rows
   hipStream_t seg_stream = nullptr;
   std::vector<hipEvent_t> ev_seg_fork, ev_seg_join;
