@@ -72,6 +72,12 @@ MAX_FF_SIZE = 10000  # side information with at most this many features uses F^T
 def load_workload(name, seed):
     """Generic N-mode form: idx [nnz x nmodes], vals, test idx/vals, dims,
     settings, side (per mode: scipy COO or None), label."""
+    if name == "c1":
+        train, test, dims, settings = synthetic.c1(seed)
+        return dict(idx=np.stack([train[0], train[1]], 1), vals=train[2], tidx=np.stack([test[0], test[1]], 1),
+                    tvals=test[2], dims=dims, settings=settings, side=[None, None],
+                    label="BPMF 5000x2000, 500k observed (400k train / 100k test), uniform cells, K=10, "
+                          "fixed precision 5, 20 burn-in + 50 samples")
     if name in ("c2", "c2p"):
         train, test, dims, settings = synthetic.c2(seed)
         idx = np.stack([train[0], train[1]], 1)
@@ -239,26 +245,40 @@ _ORACLE_INPUTS = {}
 
 
 def oracle_inputs(w):
-    """The FP64 oracle's view of a matrix workload (train rows of both modes,
-    test cells in the reference's order), built once."""
+    """The FP64 oracle's view of a workload (train rows of every mode, test
+    cells in the reference's order, side information as CSR pairs), built
+    once."""
     import oracle as O  # the FP64 oracle: the CPU baseline and the parity checker
     key = id(w)
     if key not in _ORACLE_INPUTS:
-        r, c = w["idx"][:, 0], w["idx"][:, 1]
         dims = w["dims"]
-        modes, mean, nnz = O.matrix_rows(r, c, w["vals"], dims[0], dims[1])
-        ti, tv = O.matrix_test(w["tidx"][:, 0], w["tidx"][:, 1], w["tvals"], dims[0], dims[1])
-        _ORACLE_INPUTS[key] = (modes, mean, nnz, ti, tv)
+        if len(dims) == 2:
+            r, c = w["idx"][:, 0], w["idx"][:, 1]
+            modes, mean, nnz = O.matrix_rows(r, c, w["vals"], dims[0], dims[1])
+            ti, tv = O.matrix_test(w["tidx"][:, 0], w["tidx"][:, 1], w["tvals"], dims[0], dims[1])
+        else:
+            modes, mean, nnz = O.tensor_rows(w["idx"], w["vals"], dims)
+            ti, tv = O.tensor_test(w["tidx"], w["tvals"], dims)
+        side = [None if s is None else O.csr_pair(s.row, s.col, s.data, s.shape[0], s.shape[1])
+                for s in w["side"]]
+        _ORACLE_INPUTS[key] = (modes, mean, nnz, ti, tv, side)
     return _ORACLE_INPUTS[key]
 
 
 def oracle_run(w, seed, **kw):
     import oracle as O
     st = w["settings"]
-    modes, mean, nnz, ti, tv = oracle_inputs(w)
+    modes, mean, nnz, ti, tv, side = oracle_inputs(w)
     O.set_threads(host_threads())
-    return O.run(modes, mean, nnz, list(w["dims"]), st["num_latent"], ti, tv, adaptive=True, burnin=st["burnin"],
-                 nsamples=st["nsamples"], seed=seed, **kw)
+    adaptive = st["precision"] == "adaptive"
+    extra = {}
+    if any(s is not None for s in side):
+        extra = dict(side=side, lambda_beta=float(st["lambda_beta"]), tol=float(st["tol"]),
+                     use_ftf=[s is not None and w["side"][m].shape[1] <= MAX_FF_SIZE for m, s in enumerate(side)],
+                     uni=[s is not None and bool(st.get("univariate")) for s in side])
+    return O.run(modes, mean, nnz, list(w["dims"]), st["num_latent"], ti, tv, is_tensor=len(w["dims"]) > 2,
+                 adaptive=adaptive, precision=1.0 if adaptive else float(st["precision"]), burnin=st["burnin"],
+                 nsamples=st["nsamples"], seed=seed, **extra, **kw)
 
 
 def cpu_baseline(w, iters, seed, log):
@@ -270,8 +290,8 @@ def cpu_baseline(w, iters, seed, log):
     threads = O.get_threads()
     log("cpu oracle: %d iterations in %.2f s on %d threads" % (iters, dt, threads))
     return {"value": iters / dt, "unit": "iters/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
-            "sample": "%d full Gibbs iterations of the same 138000x27000 / 18M-train K=32 workload, FP64, "
-                      "OpenMP dynamic,2 rows (C restatement of the reference)" % iters}
+            "sample": "%d full Gibbs iterations of the same workload (%s), FP64, OpenMP dynamic,2 rows (C "
+                      "restatement of the reference)" % (iters, w["label"])}
 
 
 def fp64_errors(w, name, seed, iters, log):
@@ -481,7 +501,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "c3", "c3f", "c3u", "c4", "c2p", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c3f", "c3u", "c4", "c2p", "c5"], default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
@@ -523,7 +543,7 @@ def main():
     # oracle's C code runs without the GIL) while the GPU chains run, after
     # the timed region
     fp64 = {}
-    gate = args.workload == "c2" and not args.no_chain and not args.no_fp64
+    gate = args.workload in ("c1", "c2") and not args.no_chain and not args.no_fp64
     if gate:
         import threading
 
@@ -643,8 +663,10 @@ def main():
                                "oracle's on the same data and seed" % (st["burnin"] + st["nsamples"])
             out[key]["gate_met"] = bool(res[head]["rmse"] <= 1.005 * fp64["rmse"])
             out["fp64_error_after_2_iterations"] = {k: v["fp64_err"] for k, v in res.items()}
-    if not args.no_cpu and args.workload == "c2":
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_iters, args.seed, log)
+    if not args.no_cpu and args.workload in ("c1", "c2", "c3", "c4"):
+        # a bounded sample of the same workload on the host cores (10-30 s)
+        iters = {"c1": st["burnin"] + st["nsamples"], "c2": args.cpu_iters, "c3": 5, "c4": 2}[args.workload]
+        out["cpu_baseline"] = cpu_baseline(w, iters, args.seed, log)
     print(json.dumps(out), flush=True)
 
 

@@ -103,7 +103,7 @@ def test_plain_sampler_per_row_mean():
     assert row_rel_err(got, ref) < TOL_FP32
 
 
-@pytest.mark.parametrize("K,N", [(3, 50), (10, 7000), (32, 20000), (64, 3000), (100, 500)])
+@pytest.mark.parametrize("K,N", [(3, 50), (10, 7000), (32, 20000), (64, 3000), (65, 800), (100, 500), (128, 4000)])
 def test_normal_wishart(K, N):
     announce("normal-wishart K=%d N=%d" % (K, N))
     rng = np.random.default_rng(K)
