@@ -70,6 +70,10 @@ struct RefineQueue {
   float thresh = 0.f;
   int min_nnz = 0;
   long long* totals = nullptr;  // [0] rows flagged, [1] rows refined, over the chain
+  // short queues: per-chunk data terms (kRefineGrid x KP, FP64) and per-row
+  // tickets (kRefineGrid, zero between launches) of the split rows
+  double* part = nullptr;
+  int* row_ticket = nullptr;
 };
 constexpr int kRefineGrid = 256;
 constexpr int kRefineCap = 4096;

@@ -389,11 +389,13 @@ void Model::init() {
 
 RefineQueue make_refine_queue(std::vector<DevBuf>& B, int KP, int cap, float thresh, int min_nnz, hipStream_t s) {
   B.clear();
-  B.resize(4);
+  B.resize(6);
   upload(B[0], std::vector<int>{0, 0}, s);
   B[1].alloc((size_t)std::max(cap, 1) * sizeof(int));
   B[2].alloc((size_t)std::max(cap, 1) * KP * KP * sizeof(float));
   upload(B[3], std::vector<long long>{0, 0}, s);
+  B[4].alloc((size_t)kRefineGrid * KP * sizeof(double));
+  upload(B[5], std::vector<int>(kRefineGrid, 0), s);
   RefineQueue q;
   q.count = B[0].as<int>();
   q.cap = cap;
@@ -402,6 +404,8 @@ RefineQueue make_refine_queue(std::vector<DevBuf>& B, int KP, int cap, float thr
   q.thresh = thresh;
   q.min_nnz = min_nnz;
   q.totals = B[3].as<long long>();
+  q.part = B[4].as<double>();
+  q.row_ticket = B[5].as<int>();
   return q;
 }
 

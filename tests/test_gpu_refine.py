@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from _gpu_helpers import SEED, GpuModel, announce, f32, gpu_sample_latents, spd
+from _gpu_helpers import SEED, GpuModel, announce, f32, gpu_sample_latents, row_rel_err, spd
 from macau_amd import _lib
 from macau_amd._lib import check
 
@@ -110,3 +110,41 @@ def test_refinement_counts_and_switch():
                                                                                  d.value))
     assert stats[0] == (0, 0)
     assert stats[1][0] > 0 and stats[1][1] == stats[1][0]
+
+
+def test_refinement_of_many_rows_matches_oracle():
+    """More queued rows than half the refinement grid (300 rows of 150-500
+    nonzeros, all queued by a threshold every row exceeds): one work-group
+    per row.  The model's iterations 1-2 stay within 1e-4 of the FP64 oracle
+    and every queued row is counted as refined."""
+    announce("refinement of 300 rows per half-sweep")
+    L = _lib.lib()
+    K = 32
+    rng = np.random.default_rng(11)
+    nrow, ncol = 300, 3000
+    deg = rng.integers(150, 500, nrow)
+    r = np.repeat(np.arange(nrow), deg)
+    c = np.concatenate([rng.choice(ncol, d, replace=False) for d in deg])
+    U, V = rng.standard_normal((nrow, 6)), rng.standard_normal((ncol, 6))
+    y = np.sum(U[r] * V[c], 1) + 0.4 * rng.standard_normal(len(r))
+    rt, ct = rng.integers(0, nrow, 500), rng.integers(0, ncol, 500)
+    yt = np.sum(U[rt] * V[ct], 1)
+    m = GpuModel(2, K, 4.0)
+    m.data(np.stack([r, c], 1), y, [nrow, ncol], np.stack([rt, ct], 1), yt)
+    m.prior()
+    m.prior()
+    check(L.macau_set_refine(m.h, 1, 1.0001))  # every probed row is queued
+    m.setup(1, 2, SEED, _lib.SAMPLER_BATCHED)
+    modes, mean, nnz = O.matrix_rows(r, c, y, nrow, ncol)
+    ti, tv = O.matrix_test(rt, ct, yt, nrow, ncol)
+    for n_it in (1, 2):
+        m.iterate(1)
+        ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, precision=4.0, burnin=1, nsamples=2, seed=SEED,
+                    max_iters=n_it, want_U=True)
+        e0 = row_rel_err(m.latents(0, nrow), ref.U[0])
+        e1 = row_rel_err(m.latents(1, ncol), ref.U[1])
+        print("iteration %d: row errors %.2e (rows) %.2e (columns)" % (n_it, e0, e1))
+        assert e0 < 1e-4 and e1 < 1e-4
+    f, d = ctypes.c_longlong(), ctypes.c_longlong()
+    check(L.macau_get_refine_stats(m.h, ctypes.byref(f), ctypes.byref(d)))
+    assert f.value == d.value == 2 * nrow  # the rows mode, two iterations (columns have < 128 nonzeros)
