@@ -261,6 +261,11 @@ struct CsrDev {
   // (multi_row, slots multi_slot0[m] .. multi_slot0[m + 1] - 1) are summed
   // in slot order afterwards
   int nseg = 0;
+  // the first nlong segments (the split rows' segments, then the whole rows
+  // of >= kSpmmLongNnz nonzeros, longest first) take a wave each; the rest
+  // (whole rows, by decreasing length) go four to a wave, except the last
+  // nempty (rows without nonzeros), sixteen to a wave
+  int nlong = 0, nempty = 0;
   const int4* seg = nullptr;
   int nmulti = 0;
   const int* multi_row = nullptr;
@@ -297,6 +302,7 @@ void launch_spectral_solve(const double* Q, const double* Qt, const double* d, i
                            const double* B, double* Y, double* part, const SpectralPlan& p, double* X, int nrhs,
                            int* fail, hipStream_t s);
 constexpr int kSpmmSegNnz = 256;
+constexpr int kSpmmLongNnz = 32;
 // out[:, r] = sum_c A[r,c] in[:, c] (+ (*reg) * add[:, r]); vectors have ld = nr
 // (any nr; beyond 64 the columns are taken in further passes of 64);
 // reg is a device scalar, read only when add is non-null; nothing is done

@@ -84,6 +84,168 @@ __global__ void __launch_bounds__(kSpmmThreads)
   }
 }
 
+// The same for an even nr <= 32 with 16-byte gathers: lanes (grp, cp), grp
+// = lane / 16 a lane group, cp = lane % 16 a pair of components.  The work
+// list holds the split rows' segments first, then the whole rows by
+// decreasing length, so its first A.nlong segments (those of >= 32
+// nonzeros) form a prefix:
+//   * a long segment takes a whole wave, four nonzeros per gather step (one
+//     per group): each batch of 64 column indices is one coalesced load
+//     handed to the gathering lanes by ds_bpermute, a lane's 16 gathers of
+//     a batch are issued together, the next batch's indices load meanwhile,
+//     and the groups' sums are combined by two butterfly steps;
+//   * short segments (a transposed feature matrix: most features occur in a
+//     handful of rows or none) go four to a wave, one per group, each group
+//     walking its own segment 16 indices per load and 8 gathers at a time.
+// Past a segment's end a lane gathers row 0 (valid) with weight 0, and lanes
+// past nr read a valid pair and discard it, so no gather sits behind a
+// branch.  The row's regularisation operand is loaded before the gathers;
+// it and the output row are streamed with nontemporal accesses, which
+// leaves more of the L2 to the gathered table (C3: +1%).
+__device__ __forceinline__ void spmm_store(const CsrDev& A, int4 d, int nr, int cp, double ax, double ay, double rg,
+                                           double2 ad, double* __restrict__ out) {
+  if (d.w < 0) {
+    double* o = out + (size_t)d.x * nr + 2 * cp;
+    __builtin_nontemporal_store(ax + rg * ad.x, o);
+    __builtin_nontemporal_store(ay + rg * ad.y, o + 1);
+  }
+  else
+    *reinterpret_cast<double2*>(A.partial + (size_t)d.w * nr + 2 * cp) = make_double2(ax, ay);
+}
+
+__device__ __forceinline__ void spmm_long(const CsrDev& A, int sg, const double* __restrict__ in, int nr,
+                                          double* __restrict__ out, double rg, const double* __restrict__ add,
+                                          int lane) {
+  const int grp = lane >> 4, cp = lane & 15;
+  const bool live = 2 * cp < nr;
+  const int cpo = live ? 2 * cp : 0;
+  const int4 d = A.seg[sg];  // row, first nnz, end nnz, slot
+  double2 ad = make_double2(0.0, 0.0);
+  if (add && d.w < 0 && live) {
+    const double* a = add + (size_t)d.x * nr + 2 * cp;
+    ad = make_double2(__builtin_nontemporal_load(a), __builtin_nontemporal_load(a + 1));
+  }
+  int myc = d.y + lane < d.z ? A.cols[d.y + lane] : 0;
+  double myv = d.y + lane < d.z ? (A.vals ? A.vals[d.y + lane] : 1.0) : 0.0;
+  double ax = 0.0, ay = 0.0;
+  for (int base = d.y; base < d.z; base += 64) {
+    const int en = base + 64 + lane;
+    const int nc = en < d.z ? A.cols[en] : 0;
+    const double nv = en < d.z ? (A.vals ? A.vals[en] : 1.0) : 0.0;
+    double2 xs[16];
+    double vs[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = 4 * u + grp;
+      const int c = __shfl(myc, e, 64);
+      vs[u] = __shfl(myv, e, 64);
+      xs[u] = *reinterpret_cast<const double2*>(in + (size_t)c * nr + cpo);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      ax += vs[u] * xs[u].x;
+      ay += vs[u] * xs[u].y;
+    }
+    myc = nc;
+    myv = nv;
+  }
+  ax += __shfl_xor(ax, 16, 64);
+  ay += __shfl_xor(ay, 16, 64);
+  ax += __shfl_xor(ax, 32, 64);
+  ay += __shfl_xor(ay, 32, 64);
+  if (grp == 0 && live) spmm_store(A, d, nr, cp, ax, ay, rg, ad, out);
+}
+
+__device__ __forceinline__ void spmm_short4(const CsrDev& A, int first, const double* __restrict__ in, int nr,
+                                            double* __restrict__ out, double rg, const double* __restrict__ add,
+                                            int lane, int end) {
+  const int grp = lane >> 4, cp = lane & 15;
+  const bool live = 2 * cp < nr;
+  const int cpo = live ? 2 * cp : 0;
+  const int sg = first + grp;
+  const bool has = sg < end;
+  const int4 d = has ? A.seg[sg] : make_int4(0, 0, 0, -1);
+  double2 ad = make_double2(0.0, 0.0);
+  if (add && has && d.w < 0 && live) {
+    const double* a = add + (size_t)d.x * nr + 2 * cp;
+    ad = make_double2(__builtin_nontemporal_load(a), __builtin_nontemporal_load(a + 1));
+  }
+  const int len = d.z - d.y;
+  int mx = max(len, __shfl_xor(len, 16, 64));
+  mx = max(mx, __shfl_xor(mx, 32, 64));  // the longest of the wave's four segments
+  int myc = cp < len ? A.cols[d.y + cp] : 0;
+  double myv = cp < len ? (A.vals ? A.vals[d.y + cp] : 1.0) : 0.0;
+  double ax = 0.0, ay = 0.0;
+  for (int b = 0; b < mx; b += 16) {
+    const int en = b + 16 + cp;
+    const int nc = en < len ? A.cols[d.y + en] : 0;
+    const double nv = en < len ? (A.vals ? A.vals[d.y + en] : 1.0) : 0.0;
+    const int tc = min(16, mx - b);
+    for (int t0 = 0; t0 < tc; t0 += 8) {
+      double2 xs[8];
+      double vs[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int src = 16 * grp + t0 + u;  // t0 + u < 16
+        const int c = __shfl(myc, src, 64);
+        vs[u] = __shfl(myv, src, 64);
+        xs[u] = *reinterpret_cast<const double2*>(in + (size_t)c * nr + cpo);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        ax += vs[u] * xs[u].x;
+        ay += vs[u] * xs[u].y;
+      }
+    }
+    myc = nc;
+    myv = nv;
+  }
+  if (has && live) spmm_store(A, d, nr, cp, ax, ay, rg, ad, out);
+}
+
+// empty rows (the tail of the work list): out = reg * add, sixteen rows to
+// a wave, four per group, their loads issued together
+__device__ __forceinline__ void spmm_empty16(const CsrDev& A, int first, int nr, double* __restrict__ out, double rg,
+                                             const double* __restrict__ add, int lane) {
+  const int grp = lane >> 4, cp = lane & 15;
+  const bool live = 2 * cp < nr;
+  int row[4];
+  double2 ad[4];
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const int sg = first + 4 * h + grp;
+    row[h] = sg < A.nseg ? A.seg[sg].x : -1;
+  }
+#pragma unroll
+  for (int h = 0; h < 4; h++)
+    ad[h] = (add && row[h] >= 0 && live) ? *reinterpret_cast<const double2*>(add + (size_t)row[h] * nr + 2 * cp)
+                                         : make_double2(0.0, 0.0);
+#pragma unroll
+  for (int h = 0; h < 4; h++)
+    if (row[h] >= 0 && live)
+      *reinterpret_cast<double2*>(out + (size_t)row[h] * nr + 2 * cp) = make_double2(rg * ad[h].x, rg * ad[h].y);
+}
+
+// work units: the A.nlong long segments, then groups of four short ones,
+// then groups of sixteen empty rows
+__global__ void __launch_bounds__(kSpmmThreads)
+    k_spmm_hyb(CsrDev A, const double* __restrict__ in, int nr, double* __restrict__ out,
+               const double* __restrict__ reg, const double* __restrict__ add, const double* skip) {
+  if (skip && *skip != 0.0) return;
+  const int lane = threadIdx.x & 63;
+  const int w = (blockIdx.x * kSpmmThreads + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * kSpmmThreads) >> 6;
+  const double rg = add ? *reg : 0.0;
+  const int e0 = A.nseg - A.nempty;  // first empty row's entry
+  const int nshort = (e0 - A.nlong + 3) / 4;
+  const int nunit = A.nlong + nshort + (A.nempty + 15) / 16;
+  for (int u = w; u < nunit; u += nw) {
+    if (u < A.nlong) spmm_long(A, u, in, nr, out, rg, add, lane);
+    else if (u < A.nlong + nshort) spmm_short4(A, A.nlong + 4 * (u - A.nlong), in, nr, out, rg, add, lane, e0);
+    else spmm_empty16(A, e0 + 16 * (u - A.nlong - nshort), nr, out, rg, add, lane);
+  }
+}
+
 // rows split into several segments: a work-group per row sums the row's
 // partials over 8 strands of slots (slot s on strand s % 8), the strands
 // combined in strand order
@@ -122,7 +284,14 @@ static void launch_spmm_t(const CsrDev& A, const double* in, int nr, double* out
   if (A.nseg > 0) {
     int grid = (A.nseg + 3) / 4;
     if (grid > 16384) grid = 16384;
-    hipLaunchKernelGGL((k_spmm_seg<NP>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add, skip);
+    const uintptr_t al = (uintptr_t)in | (uintptr_t)out | (uintptr_t)add | (uintptr_t)A.partial;
+    if (NP == 32 && nr % 2 == 0 && (al & 15) == 0) {
+      const int nunit = A.nlong + (A.nseg - A.nempty - A.nlong + 3) / 4 + (A.nempty + 15) / 16;
+      int g = (nunit + 3) / 4;
+      if (g > 16384) g = 16384;
+      hipLaunchKernelGGL(k_spmm_hyb, dim3(g), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add, skip);
+    } else
+      hipLaunchKernelGGL((k_spmm_seg<NP>), dim3(grid), dim3(kSpmmThreads), 0, s, A, in, nr, out, reg, add, skip);
     if (A.nmulti > 0) {
       const int g2 = A.nmulti < 4096 ? A.nmulti : 4096;
       hipLaunchKernelGGL(k_spmm_fix, dim3(g2), dim3(kSpmmThreads), 0, s, A, nr, out, reg, add, skip);
@@ -262,9 +431,70 @@ __global__ void __launch_bounds__(DotShape<NRP>::THREADS)
   }
 }
 
+// The same sums at nr = 32 on the FP64 matrix cores: v_mfma_f64_16x16x4f64
+// with the features as the reduction index, four per instruction.  Lane
+// (q = l / 16, j = l % 16) loads components 2j and 2j + 1 of feature f + q
+// of a and of b as one 16-byte vector; the even components feed the tiles
+// of row (column) block 0 and the odd ones block 1, so the four tiles
+// (ib, jb) hold D[2i + ib][2j' + jb] and every load is a 256-byte row read
+// by 16 lanes.  Each wave takes a quarter of the block's columns; the four
+// waves' sums are added in wave order through LDS.
+__global__ void __launch_bounds__(256)
+    k_dots32(const double* __restrict__ a, const double* __restrict__ b, int n, int cols,
+             double* __restrict__ partial, const double* skip) {
+  if (skip && *skip != 0.0) return;
+  __shared__ double red[4][32 * 32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+  const int b0 = blockIdx.x * cols, b1 = min(n, b0 + cols);
+  const int per = (cols + 3) / 4;
+  const int c0 = min(b1, b0 + wv * per), c1 = min(b1, c0 + per);
+  typedef double dbl4v __attribute__((ext_vector_type(4)));
+  dbl4v d[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int v = 0; v < 2; v++) d[u][v] = dbl4v{0.0, 0.0, 0.0, 0.0};
+  // 32 features per trip: the 16 loads of a lane are issued together
+  // (clamped addresses, zeros selected past the end: no branch)
+  for (int f0 = c0; f0 < c1; f0 += 32) {
+    double2 xs[8], ys[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int fc = min(f0 + 4 * u + q, c1 - 1);
+      xs[u] = reinterpret_cast<const double2*>(a + (size_t)fc * 32)[j];
+      ys[u] = reinterpret_cast<const double2*>(b + (size_t)fc * 32)[j];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const bool ok = f0 + 4 * u + q < c1;
+      const double x0 = ok ? xs[u].x : 0.0, x1 = ok ? xs[u].y : 0.0;
+      const double y0 = ok ? ys[u].x : 0.0, y1 = ok ? ys[u].y : 0.0;
+      d[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y0, d[0][0], 0, 0, 0);
+      d[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y1, d[0][1], 0, 0, 0);
+      d[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y0, d[1][0], 0, 0, 0);
+      d[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, d[1][1], 0, 0, 0);
+    }
+  }
+  // slot v of lane (q, j) in tile (ib, jb) is D[2 (4v + q) + ib][2j + jb]
+#pragma unroll
+  for (int ib = 0; ib < 2; ib++)
+#pragma unroll
+    for (int jb = 0; jb < 2; jb++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) red[wv][(2 * (4 * v + q) + ib) + 32 * (2 * j + jb)] = d[ib][jb][v];
+  __syncthreads();
+  double* out = partial + (size_t)blockIdx.x * 1024;
+  for (int e = threadIdx.x; e < 1024; e += 256) out[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+}
+
 void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
                  const double* skip) {
   const int cols = (n + nblk - 1) / nblk;
+  if (nr == 32 && (((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
+    hipLaunchKernelGGL(k_dots32, dim3(nblk), dim3(256), 0, s, a, b, n, cols, partial, skip);
+    if (nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
+    return;
+  }
 #define MGPU_DOTS(P)                                                                                        \
   hipLaunchKernelGGL((k_dots<P>), dim3(nblk), dim3(DotShape<P>::THREADS), 0, s, a, b, nr, n, cols, partial, \
                      skip)
@@ -299,6 +529,10 @@ struct SmallView {
 // The small-step kernels run one wave (64 threads); their nr x nr operands
 // are moved with all loads of a thread issued together.
 constexpr int kSmallPer = (kMaxNr * kMaxNr + 63) / 64;
+// LDS workspace of a small solve: the LLT path's padded factor, or the
+// elimination's rows of U (32 x 64) and reciprocal pivots (32)
+constexpr int kSolveWork = 32 * 64 + 32;
+static_assert(kMaxNr * (kMaxNr + 1) <= kSolveWork, "workspace holds the padded factor");
 
 __device__ __forceinline__ void load_nn(const double* __restrict__ src, double* dst, int NN) {
   double v[kSmallPer];
@@ -410,7 +644,98 @@ __device__ __forceinline__ void small_solve_w(const double* M, const double* Q, 
   lds_wait();
 }
 
+// The same X = (M^{-1} Q)^T for nr <= 32 by Gaussian elimination of the
+// augmented [M | Q] without pivoting: lane j holds column j of M (j < NRP,
+// identity padding), lane 32 + j column j of Q, as NRP doubles in registers.
+// Step k stores row k of the reduced matrix in LDS (row k of U in M = L'U);
+// its entries right of the pivot are also the multipliers' numerators,
+// since the trailing block of the reduced matrix stays symmetric (only the
+// upper triangle is ever read).  Each step is one LDS round trip plus
+// NRP - k - 1 independent FMAs per lane, so the 32 steps cost a few
+// thousand cycles instead of the LLT path's chains of dependent LDS reads:
+// row k + 1 is updated first and its LDS round trip overlaps the rest of
+// step k, and the pivot's reciprocal is the hardware estimate refined by
+// two Newton steps.  The back substitution reads U's columns above the
+// pivot, which are all in LDS once the forward pass ends.  A pivot below
+// 1e-10 of its diagonal entry of M (or NaN) returns false before anything
+// is written: near a non-positive pivot the caller runs the LLT path, which
+// reproduces Eigen's stop at a non-positive pivot.  U holds NRP x 64 + NRP
+// doubles (the rows, then the pivot thresholds).
+template <int NRP>
+__device__ __forceinline__ bool small_solve_elim(const double* M, const double* Q, double* X, int nr, double* U) {
+  static_assert(NRP <= 32, "two columns per lane pair");
+  const int lane = threadIdx.x & 63, col = lane & 31;
+  const bool isq = lane >= 32;
+  double a[NRP];
+#pragma unroll
+  for (int r = 0; r < NRP; r++) {
+    double v = 0.0;
+    if (col < nr && r < nr) v = isq ? Q[r + col * nr] : M[r + col * nr];
+    else if (!isq && r == col) v = 1.0;
+    a[r] = v;
+  }
+  double* thr = U + NRP * 64;  // pivot thresholds 1e-10 M_kk
+  if (lane < NRP) thr[lane] = lane < nr ? 1e-10 * M[lane + lane * nr] : 1e-10;
+  U[lane] = a[0];
+  lds_wait();
+  double u[NRP], rp[NRP];
+#pragma unroll
+  for (int i = 0; i < NRP; i++) u[i] = U[i];
+  double th = thr[0];
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < NRP; k++) {
+    // u holds row k of the reduced matrix
+    bad |= !(u[k] > th);  // checked once after the loop: no branch per step
+    double inv = __builtin_amdgcn_rcp(u[k]);  // two Newton steps from the hardware estimate
+    inv = fma(inv, fma(-u[k], inv, 1.0), inv);
+    inv = fma(inv, fma(-u[k], inv, 1.0), inv);
+    rp[k] = inv;
+    const double t = a[k] * inv;
+    double un[NRP], thn = 0.0;
+    if (k + 1 < NRP) {
+      a[k + 1] -= u[k + 1] * t;
+      U[(k + 1) * 64 + lane] = a[k + 1];
+      // a wave's LDS operations are performed in order, so the next row's
+      // reads may follow its write at once; the fence only stops the
+      // compiler from moving them
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+      for (int i = k + 1; i < NRP; i++) un[i] = U[(k + 1) * 64 + i];
+      thn = thr[k + 1];
+    }
+#pragma unroll
+    for (int i = k + 2; i < NRP; i++) a[i] -= u[i] * t;
+    // apply this step's updates here rather than sink them to each a[i]'s
+    // next use (which would keep every step's row alive in registers)
+#pragma unroll
+    for (int i = k + 2; i < NRP; i++) asm volatile("" : "+v"(a[i]));
+#pragma unroll
+    for (int i = k + 1; i < NRP; i++) u[i] = un[i];
+    th = thn;
+  }
+  if (bad) return false;  // uniform: every lane read the same pivots
+#pragma unroll
+  for (int k = NRP - 1; k >= 0; k--) {
+    int o = 0;  // opaque zero offset: step k's loads wait for step k + 2 (two steps of lookahead)
+    opaque_after(o, k + 2 < NRP ? a[k + 2] : 0.0);
+    a[k] *= rp[k];
+#pragma unroll
+    for (int i = 0; i < k; i++) a[i] -= U[i * 64 + k + o] * a[k];
+  }
+  if (isq && col < nr) {
+#pragma unroll
+    for (int r = 0; r < NRP; r++)
+      if (r < nr) X[col + r * nr] = a[r];
+  }
+  lds_wait();
+  return true;
+}
+
 __device__ __forceinline__ void small_solve(const double* M, const double* Q, double* X, int nr, double* Ls) {
+  if (nr <= 8 && small_solve_elim<8>(M, Q, X, nr, Ls)) return;
+  if (nr > 8 && nr <= 16 && small_solve_elim<16>(M, Q, X, nr, Ls)) return;
+  if (nr > 16 && nr <= 32 && small_solve_elim<32>(M, Q, X, nr, Ls)) return;
   if (nr <= 8) small_solve_w<8>(M, Q, X, nr, Ls);
   else if (nr <= 16) small_solve_w<16>(M, Q, X, nr, Ls);
   else if (nr <= 32) small_solve_w<32>(M, Q, X, nr, Ls);
@@ -454,7 +779,7 @@ __global__ void k_cg_rtr(const double* partial, int nblk, int nr, double* small)
 __global__ void __launch_bounds__(64) k_cg_step_a(const double* partial, int nblk, int nr, double* small) {
   SmallView v(small, nr);
   if (v.flags[0] != 0.0) return;
-  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kMaxNr * (kMaxNr + 1)], Xs[kMaxNr * kMaxNr];
+  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kSolveWork], Xs[kMaxNr * kMaxNr];
   reduce_sym(partial, nblk, nr, M);
   load_nn(v.RtR, Q, nr * nr);
   load_nn(M, v.PtKP, nr * nr);
@@ -503,12 +828,89 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same updates at nr = 32 on the FP64 matrix cores: per 16 features a
+// wave forms C x^T (32 x 16) with v_mfma_f64_16x16x4f64, C the 32 x 32
+// coefficients (A or Psi) in registers as A-operand fragments loaded once.
+// Operand layouts (lane l, q = l / 16, j = l % 16): A[i][kk] on lane
+// (i = j, kk = q), B[kk][j] on lane (kk = q, j), D[4v + q][j] in slot v.
+// The reduction index is mapped k = 8 kk + ks over the 8 MFMA steps ks, so
+// a lane's B values are 8 contiguous components of its feature (64 B), and
+// the rows of C are permuted (row 4 (i % 4) + i / 4 of a 16-row block on
+// fragment row i) so that a lane's 4 output slots are 4 contiguous
+// components: every global access is a 16-byte vector.  No LDS.  The sums
+// over k run in the matrix core's order, so the result differs from the
+// kernel above in the last bits.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 ld4(const double* p) {
+  const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
+  return dbl4{a.x, a.y, b.x, b.y};
+}
+
+__device__ __forceinline__ void st4(double* p, dbl4 v) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+
+template <int MODE>  // 0: X += A P, R -= A KP;  1: P = R + Psi P
+__global__ void __launch_bounds__(256)
+    k_cg_update32(int n, const double* __restrict__ small, double* __restrict__ X, double* __restrict__ R,
+                  double* __restrict__ P, const double* __restrict__ KPm) {
+  constexpr int NN = 32 * 32;  // SmallView layout at nr = 32
+  if (small[5 * NN + 32] != 0.0) return;  // converged or failed earlier
+  const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
+  const double* __restrict__ C = small + (MODE == 0 ? 3 : 4) * NN;  // col-major
+  double ca[2][8];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) ca[rb][ks] = C[(16 * rb + 4 * (j & 3) + (j >> 2)) + 32 * (8 * q + ks)];
+  const int ntile = (n + 15) / 16;
+  const int w = (blockIdx.x * 256 + threadIdx.x) >> 6, nw = (gridDim.x * 256) >> 6;
+  for (int t = w; t < ntile; t += nw) {
+    const int f = 16 * t + j;
+    const bool ok = f < n;
+    const size_t o = (size_t)(ok ? f : 0) * 32;
+    auto prod = [&](const double* __restrict__ in, dbl4& d0, dbl4& d1) {
+      double b[8];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const dbl4 v = ok ? ld4(in + o + 8 * q + 4 * h) : dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < 4; u++) b[4 * h + u] = v[u];
+      }
+      d0 = dbl4{0.0, 0.0, 0.0, 0.0};
+      d1 = d0;
+#pragma unroll
+      for (int ks = 0; ks < 8; ks++) {
+        d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[0][ks], b[ks], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[1][ks], b[ks], d1, 0, 0, 0);
+      }
+    };
+    dbl4 d0, d1;
+    prod(P, d0, d1);
+    if (MODE == 0) {
+      dbl4 e0, e1;
+      prod(KPm, e0, e1);
+      if (ok) {
+        st4(X + o + 4 * q, ld4(X + o + 4 * q) + d0);
+        st4(X + o + 16 + 4 * q, ld4(X + o + 16 + 4 * q) + d1);
+        st4(R + o + 4 * q, ld4(R + o + 4 * q) - e0);
+        st4(R + o + 16 + 4 * q, ld4(R + o + 16 + 4 * q) - e1);
+      }
+    } else if (ok) {
+      st4(P + o + 4 * q, ld4(R + o + 4 * q) + d0);
+      st4(P + o + 16 + 4 * q, ld4(R + o + 16 + 4 * q) + d1);
+    }
+  }
+}
+
 // convergence test on diag(R R^T) and Psi = (RtR^{-1} RtR2)^T
 __global__ void __launch_bounds__(64) k_cg_step_b(const double* partial, int nblk, int nr, double* small,
                                                    double tolsq) {
   SmallView v(small, nr);
   if (v.flags[0] != 0.0) return;
-  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kMaxNr * (kMaxNr + 1)], Xs[kMaxNr * kMaxNr];
+  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kSolveWork], Xs[kMaxNr * kMaxNr];
   __shared__ int conv;
   if (threadIdx.x == 0) conv = 1;
   reduce_sym(partial, nblk, nr, Q);  // RtR2
@@ -560,6 +962,11 @@ void launch_cg_step_a(const CgState& c, hipStream_t s) {
 }
 
 void launch_cg_update_xr(const CgState& c, hipStream_t s) {
+  if (c.nr == 32) {
+    hipLaunchKernelGGL((k_cg_update32<0>), dim3((c.n + 63) / 64), dim3(256), 0, s, c.n, (const double*)c.small, c.X,
+                       c.R, c.P, (const double*)c.KPm);
+    return;
+  }
   hipLaunchKernelGGL((k_cg_update<0>), dim3(grid_for(c.n, 8)), dim3(256), 0, s, c.nr, c.n, (const double*)c.small,
                      c.X, c.R, c.P, (const double*)c.KPm);
 }
@@ -570,6 +977,11 @@ void launch_cg_step_b(const CgState& c, double tolsq, hipStream_t s) {
 }
 
 void launch_cg_update_p(const CgState& c, hipStream_t s) {
+  if (c.nr == 32) {
+    hipLaunchKernelGGL((k_cg_update32<1>), dim3((c.n + 63) / 64), dim3(256), 0, s, c.n, (const double*)c.small, c.X,
+                       c.R, c.P, (const double*)c.KPm);
+    return;
+  }
   hipLaunchKernelGGL((k_cg_update<1>), dim3(grid_for(c.n, 8)), dim3(256), 0, s, c.nr, c.n, (const double*)c.small,
                      c.X, c.R, c.P, (const double*)c.KPm);
 }

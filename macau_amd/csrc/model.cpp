@@ -958,8 +958,20 @@ static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBu
       seg.push_back(make_int4(r, b, std::min(b + kSpmmSegNnz, rp[r + 1]), slots++));
   }
   mslot0.push_back(slots);
+  const int nsplit = (int)seg.size();
+  // whole rows by decreasing length (ties by row): the long ones form a
+  // prefix after the split rows' segments
+  std::vector<int> whole;
   for (int r = 0; r < nrow; r++)
-    if (rp[r + 1] - rp[r] <= kSpmmSegNnz) seg.push_back(make_int4(r, rp[r], rp[r + 1], -1));
+    if (rp[r + 1] - rp[r] <= kSpmmSegNnz) whole.push_back(r);
+  std::stable_sort(whole.begin(), whole.end(),
+                   [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+  int nlong = nsplit, nempty = 0;
+  for (int r : whole) {
+    seg.push_back(make_int4(r, rp[r], rp[r + 1], -1));
+    if (rp[r + 1] - rp[r] >= kSpmmLongNnz) nlong++;
+    if (rp[r + 1] == rp[r]) nempty++;
+  }
   store.clear();
   store.resize(4);
   upload(store[0], seg, s);
@@ -968,6 +980,8 @@ static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBu
   store[3].alloc((size_t)std::max(slots, 1) * nrmax * sizeof(double));
   CsrDev c;
   c.nseg = (int)seg.size();
+  c.nlong = nlong;
+  c.nempty = nempty;
   c.seg = store[0].as<int4>();
   c.nmulti = (int)mrow.size();
   c.multi_row = store[1].as<int>();

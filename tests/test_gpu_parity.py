@@ -134,7 +134,7 @@ def test_blockcg_reference_known_answers(kat):
     np.testing.assert_allclose(X, np.array(c3["X"]), rtol=1e-6)
 
 
-@pytest.mark.parametrize("nrhs,binary", [(1, True), (8, False), (32, True), (40, True)])
+@pytest.mark.parametrize("nrhs,binary", [(1, True), (5, False), (8, False), (16, True), (20, False), (32, True), (32, False), (40, True)])
 def test_blockcg_vs_oracle(nrhs, binary):
     announce("blockcg nrhs=%d" % nrhs)
     rng = np.random.default_rng(nrhs)
