@@ -905,6 +905,99 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// X += A P and R -= A KP at nr = 32 as above, fused with the partial sums of
+// R^T R over the new R (what the convergence test and Psi need next): a
+// block takes `cols` features (a wave a quarter, 16 at a time), so its
+// partial has the layout and block count of the dot kernel's.  A wave
+// passes its 16 new rows of R through LDS (rows of 34 doubles: 16-byte
+// accesses without bank conflicts) from the update's output layout into the
+// dot kernel's operand layout: lane (q, j) takes components 2j, 2j + 1 of
+// feature 4s + q for the four steps s.  Features past the end enter the
+// sums as zero rows.
+__global__ void __launch_bounds__(256)
+    k_cg_update32_xr(int n, int cols, const double* __restrict__ small, double* __restrict__ X,
+                     double* __restrict__ R, const double* __restrict__ P, const double* __restrict__ KPm,
+                     double* __restrict__ partial) {
+  constexpr int NN = 32 * 32, LD = 34;
+  if (small[5 * NN + 32] != 0.0) return;  // converged or failed earlier
+  __shared__ double tile[4][16 * LD];
+  __shared__ double red[4][NN];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+  const double* __restrict__ C = small + 3 * NN;  // A, col-major
+  double ca[2][8];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) ca[rb][ks] = C[(16 * rb + 4 * (j & 3) + (j >> 2)) + 32 * (8 * q + ks)];
+  const int b0 = blockIdx.x * cols, b1 = min(n, b0 + cols);
+  const int per = ((cols + 3) / 4 + 15) / 16 * 16;
+  const int c0 = min(b1, b0 + wv * per), c1 = min(b1, c0 + per);
+  dbl4 dd[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int v = 0; v < 2; v++) dd[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
+  double* T = tile[wv];
+  for (int f0 = c0; f0 < c1; f0 += 16) {
+    const int f = f0 + j;
+    const bool ok = f < c1;
+    const size_t o = (size_t)(ok ? f : c0) * 32;
+    double bp[8], bk[8];
+    const dbl4 p0 = ld4(P + o + 8 * q), p1 = ld4(P + o + 8 * q + 4);
+    const dbl4 k0 = ld4(KPm + o + 8 * q), k1 = ld4(KPm + o + 8 * q + 4);
+    const dbl4 x0 = ld4(X + o + 4 * q), x1 = ld4(X + o + 16 + 4 * q);
+    const dbl4 r0 = ld4(R + o + 4 * q), r1 = ld4(R + o + 16 + 4 * q);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      bp[u] = p0[u];
+      bp[4 + u] = p1[u];
+      bk[u] = k0[u];
+      bk[4 + u] = k1[u];
+    }
+    dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, e0 = d0, e1 = d0;
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+      d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[0][ks], bp[ks], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[1][ks], bp[ks], d1, 0, 0, 0);
+      e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[0][ks], bk[ks], e0, 0, 0, 0);
+      e1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[1][ks], bk[ks], e1, 0, 0, 0);
+    }
+    dbl4 n0 = r0 - e0, n1 = r1 - e1;
+    if (ok) {
+      st4(X + o + 4 * q, x0 + d0);
+      st4(X + o + 16 + 4 * q, x1 + d1);
+      st4(R + o + 4 * q, n0);
+      st4(R + o + 16 + 4 * q, n1);
+    } else {
+      n0 = dbl4{0.0, 0.0, 0.0, 0.0};
+      n1 = n0;
+    }
+    // slot v of lane (q, j): components 4q + v and 16 + 4q + v of feature f0 + j
+    st4(T + j * LD + 4 * q, n0);
+    st4(T + j * LD + 16 + 4 * q, n1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // a wave's LDS accesses are performed in order
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      const double2 v = *reinterpret_cast<const double2*>(T + (4 * st + q) * LD + 2 * j);
+      dd[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, v.x, dd[0][0], 0, 0, 0);
+      dd[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, v.y, dd[0][1], 0, 0, 0);
+      dd[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, v.x, dd[1][0], 0, 0, 0);
+      dd[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, v.y, dd[1][1], 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // the next tile's writes follow these reads
+  }
+  // slot v of lane (q, j) in tile (ib, jb) is D[2 (4v + q) + ib][2j + jb]
+#pragma unroll
+  for (int ib = 0; ib < 2; ib++)
+#pragma unroll
+    for (int jb = 0; jb < 2; jb++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) red[wv][(2 * (4 * v + q) + ib) + 32 * (2 * j + jb)] = dd[ib][jb][v];
+  __syncthreads();
+  double* out = partial + (size_t)blockIdx.x * NN;
+  for (int e = threadIdx.x; e < NN; e += 256) out[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+}
+
 // convergence test on diag(R R^T) and Psi = (RtR^{-1} RtR2)^T
 __global__ void __launch_bounds__(64) k_cg_step_b(const double* partial, int nblk, int nr, double* small,
                                                    double tolsq) {
@@ -961,10 +1054,16 @@ void launch_cg_step_a(const CgState& c, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_step_a, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
 }
 
+// nr = 32: the update also leaves the reduced partial sums of R^T R, which
+// launch_cg_step_b then uses without a dot pass of its own
+static bool fused_xr(const CgState& c) { return c.nr == 32; }
+
 void launch_cg_update_xr(const CgState& c, hipStream_t s) {
-  if (c.nr == 32) {
-    hipLaunchKernelGGL((k_cg_update32<0>), dim3((c.n + 63) / 64), dim3(256), 0, s, c.n, (const double*)c.small, c.X,
-                       c.R, c.P, (const double*)c.KPm);
+  if (fused_xr(c)) {
+    const int cols = (c.n + c.nblk - 1) / c.nblk;
+    hipLaunchKernelGGL(k_cg_update32_xr, dim3(c.nblk), dim3(256), 0, s, c.n, cols, (const double*)c.small, c.X, c.R,
+                       (const double*)c.P, (const double*)c.KPm, c.partial);
+    if (c.nblk > 1) launch_partial_reduce(c.partial, c.nblk, 32 * 32, s);
     return;
   }
   hipLaunchKernelGGL((k_cg_update<0>), dim3(grid_for(c.n, 8)), dim3(256), 0, s, c.nr, c.n, (const double*)c.small,
@@ -972,7 +1071,7 @@ void launch_cg_update_xr(const CgState& c, hipStream_t s) {
 }
 
 void launch_cg_step_b(const CgState& c, double tolsq, hipStream_t s) {
-  launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s, done_flag(c));
+  if (!fused_xr(c)) launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s, done_flag(c));
   hipLaunchKernelGGL(k_cg_step_b, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small, tolsq);
 }
 
