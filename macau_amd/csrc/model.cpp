@@ -1089,10 +1089,12 @@ int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const d
   const double tolsq = tol * tol;
   double flags[3] = {0, 0, 0};
   const double* dflags = c.small + 5 * nr * nr + nr;
-  // 8 iterations per host round trip; kernels of iterations after
-  // convergence return immediately (device-side done flag)
-  auto launch_chunk = [&](hipStream_t st) {
-    for (int q = 0; q < 8; q++) {
+  // a chunk of iterations per host round trip: the first as long as the
+  // previous solve needed, less 2 and rounded down to 4 (8 without one), then
+  // 4 at a time; kernels of iterations after convergence return
+  // immediately (device-side done flag)
+  auto launch_chunk = [&](hipStream_t st, int len) {
+    for (int q = 0; q < len; q++) {
       launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, st, dflags);
       launch_spmm(Ft, c.inner, nr, c.KPm, reg, c.P, st, dflags);
       launch_cg_step_a(c, st);
@@ -1101,39 +1103,46 @@ int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const d
       launch_cg_update_p(c, st);
     }
   };
-  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> key;
   if (use_graph) {
     double tolsq_bits = tolsq;
     uintptr_t tb = 0;
     std::memcpy(&tb, &tolsq_bits, sizeof(tb));
-    std::vector<uintptr_t> key = {(uintptr_t)reg, (uintptr_t)c.X, (uintptr_t)c.small, (uintptr_t)nr, tb};
+    key = {(uintptr_t)reg, (uintptr_t)c.X, (uintptr_t)c.small, (uintptr_t)nr, tb};
     for (const CsrDev* A : {&F, &Ft}) {
       const uintptr_t f[] = {(uintptr_t)A->nrow, (uintptr_t)A->ncol, (uintptr_t)A->row_ptr, (uintptr_t)A->cols,
                              (uintptr_t)A->vals, (uintptr_t)A->nseg, (uintptr_t)A->seg, (uintptr_t)A->nmulti,
                              (uintptr_t)A->multi_row, (uintptr_t)A->multi_slot0, (uintptr_t)A->partial};
       key.insert(key.end(), std::begin(f), std::end(f));
     }
-    for (auto& ch : b.chunks)
-      if (ch.key == key) exec = ch.exec;
-    if (!exec) {
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_chunk(s);
-      HIPCHK(hipStreamEndCapture(s, &g));
-      HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(g);
-      b.chunks.push_back({key, exec});
-    }
   }
-  for (int it = 0; it < 100000; it += 8) {
-    if (exec)
-      HIPCHK(hipGraphLaunch(exec, s));
+  // the chunk's graph for this configuration and length, captured once
+  auto chunk_exec = [&](int len) -> hipGraphExec_t {
+    std::vector<uintptr_t> k = key;
+    k.push_back((uintptr_t)len);
+    for (auto& ch : b.chunks)
+      if (ch.key == k) return ch.exec;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_chunk(s, len);
+    HIPCHK(hipStreamEndCapture(s, &g));
+    HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    b.chunks.push_back({k, exec});
+    return exec;
+  };
+  int len = b.last_iters >= 8 ? std::max(4, (b.last_iters - 2) / 4 * 4) : 8;
+  for (int it = 0; it < 100000; it += len, len = 4) {
+    if (use_graph)
+      HIPCHK(hipGraphLaunch(chunk_exec(len), s));
     else
-      launch_chunk(s);
+      launch_chunk(s, len);
     HIPCHK(hipMemcpyAsync(flags, dflags, sizeof(flags), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (flags[0] != 0.0) break;
   }
+  b.last_iters = (int)flags[1];
   if (flags[2] != 0.0) throw std::runtime_error("block CG: Cholesky of a small system failed");
   launch_cg_finish(c, s);
   if (nr != ld) launch_rows_copy(c.X, nr, X, ld, 0, r0, nr, nfeat, s);

@@ -62,9 +62,11 @@ class Model;
 struct CgBuffers {
   DevBuf X, R, P, KP, B, inner, partial, small;
   int nrmax = 0;
+  int last_iters = 0;  // iterations of the previous solve (sizes the first chunk)
   void alloc(int nrmax, int nfeat, int nsamples);
-  // instantiated graphs of 8 CG iterations, one per launch configuration
-  // (the kernels' arguments: operators, buffers, regulariser, tolerance)
+  // instantiated graphs of a chunk of CG iterations, one per launch
+  // configuration (the kernels' arguments: operators, buffers, regulariser,
+  // tolerance) and chunk length
   struct Chunk {
     std::vector<uintptr_t> key;
     hipGraphExec_t exec = nullptr;
@@ -78,8 +80,8 @@ struct CgBuffers {
 // Solves X[r0:r0+nr, :] (F^T F + reg I) = B[r0:r0+nr, :] with B and X of
 // leading dimension ld (= total right-hand sides); reg is a device scalar.
 // Returns the iteration count of the reference's loop.  With use_graph the
-// 8 CG iterations launched per host check of the convergence flags are
-// captured once and replayed as a HIP graph.
+// chunks of CG iterations launched per host check of the convergence flags
+// are captured once per length and replayed as HIP graphs.
 int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
                    double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph = false);
 
@@ -125,7 +127,7 @@ class MacauPrior : public ILatentPrior {
   void saveModel(Model& m, int mode, const std::string& prefix) override;
   const float* prior_mean_rows() const override { return Lmu_rows.as<float>(); }
   // whether update_prior can be captured in a HIP graph: the block CG reads
-  // its convergence flags on the host every 8 iterations; the F^T F branch
+  // its convergence flags on the host after each chunk of iterations; the F^T F branch
   // launches a fixed sequence of kernels
   virtual bool capturable() const { return use_ftf; }
 
