@@ -474,7 +474,7 @@ def bench_c5(args, log):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3-f32acc (K 17-64) / f32 (K 8, 16, 128)",
+        "dtype": "bf16x4-f32acc" if all(k == "batched" for k in kinds) else "bf16x4-f32acc (batched) / f32 (plain)",
         "data": "synthetic",
         "config": {"workload": "64 independent BPMF instances, rows x cols log-uniform in [1000,50000] x "
                                "[500,10000], density log-uniform in [0.1%,5%], K in {8,16,32,64,128}, fixed "

@@ -418,7 +418,7 @@ int macau_sample_latents(int K, int nother, int nrows, const long long* row_ptr,
     if (!batched_supported(KP, nother)) throw std::runtime_error("batched sampler: unsupported latent size");
     if (!batched_fits(fs, nother, KP)) throw std::runtime_error("batched sampler: a factor matrix exceeds 2^31 bytes");
     BatchPlan p = build_batch_plan(rp, KP, pb, s);
-    if (sampler == SAMPLER_BATCHED) a.refine = make_refine_queue(qb, KP, std::min(std::max(nrows, 1), kRefineCap),
+    if (sampler == SAMPLER_BATCHED && KP <= 64) a.refine = make_refine_queue(qb, KP, std::min(std::max(nrows, 1), kRefineCap),
                                                                  kRefineThresh, std::max(K, kRefineMinNnz), s);
     launch_sample_batched(a, p, s);
   } else if (sampler == SAMPLER_PLAIN) {

@@ -124,6 +124,9 @@ struct SegStreams {
 void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg = {});
 // K in (32, 64] (KP = 64): one row per wave, three 32 x 32 Gram blocks
 void launch_sample_batched64(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg = {});
+// K in (64, 128] (KP = 128): one row per 256-thread work-group, ten 32 x 32
+// Gram blocks over four waves; no refinement queue (a.refine is ignored)
+void launch_sample_batched128(const SampleArgs& a, const BatchPlan& p, hipStream_t s, const SegStreams& seg = {});
 int batched64_light_grid(const BatchPlan& p);
 int batched64_heavy_grid(const BatchPlan& p);
 int batched_supported(int KP, int nother);

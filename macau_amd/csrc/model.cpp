@@ -355,7 +355,7 @@ void Model::init() {
   build_plans();
   rq = RefineQueue();
   refine_bufs.clear();
-  if (sampler == SAMPLER_BATCHED && refine) {
+  if (sampler == SAMPLER_BATCHED && refine && KP <= 64) {  // no refinement step for K > 64
     const int maxrows = *std::max_element(dims.begin(), dims.end());
     rq = make_refine_queue(refine_bufs, KP, std::min(maxrows, kRefineCap), refine_thresh,
                            refine_min_nnz < 0 ? std::max(num_latent, kRefineMinNnz) : refine_min_nnz, stream);
@@ -523,7 +523,7 @@ SampleArgs Model::sample_args(int mode, const ILatentPrior& p) {
     a.sse_partial = d_sse_partial.as<double>();
     a.r2 = d_r2_last.as<float>();
   }
-  if (sampler == SAMPLER_BATCHED && refine) a.refine = rq;
+  if (sampler == SAMPLER_BATCHED && refine && KP <= 64) a.refine = rq;
   return a;
 }
 

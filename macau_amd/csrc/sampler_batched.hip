@@ -43,7 +43,7 @@ namespace mgpu {
 constexpr int kPartial = 1024 + 64;  // floats per segment partial: fragment + RHS lanes
 
 int batched_supported(int KP, int nother) {
-  return (KP == 32 || KP == 64) && nother >= 1 && nother <= kMaxBatchedOther;
+  return (KP == 32 || KP == 64 || KP == 128) && nother >= 1 && nother <= kMaxBatchedOther;
 }
 
 bool batched_fits(const FactorSet& f, int nother, int KP) {
@@ -521,12 +521,12 @@ __global__ void __launch_bounds__(64, 4) k_heavy32(SampleArgs a, BatchPlan p, in
 }
 
 int batched_light_grid(const BatchPlan& p) {
-  if (p.KP == 64) return batched64_light_grid(p);
+  if (p.KP >= 64) return batched64_light_grid(p);  // one row per work-group
   const int cap = p.light_cap > 0 ? p.light_cap : kDefaultLightCap;
   return p.n_pair < cap ? p.n_pair : cap;
 }
 int batched_heavy_grid(const BatchPlan& p) {
-  if (p.KP == 64) return batched64_heavy_grid(p);
+  if (p.KP >= 64) return batched64_heavy_grid(p);
   const int npairs = (p.n_heavy + 1) / 2;
   const int cap = p.heavy_cap > 0 ? p.heavy_cap : kDefaultHeavyCap;
   return npairs < cap ? npairs : cap;
@@ -562,6 +562,7 @@ void launch_sample_batched(const SampleArgs& a, const BatchPlan& p, hipStream_t 
   if (a.KP != p.KP || !batched_supported(a.KP, a.rows.nother))
     throw std::runtime_error("batched sampler: plan and latent size disagree or are unsupported");
   if (a.KP == 64) return launch_sample_batched64(a, p, s, seg);
+  if (a.KP == 128) return launch_sample_batched128(a, p, s, seg);
   switch (a.rows.nother) {
     case 1: return launch_batched_t<1>(a, p, s, seg);
     case 2: return launch_batched_t<2>(a, p, s, seg);

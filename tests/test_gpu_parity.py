@@ -409,7 +409,7 @@ BATCHED = _lib.SAMPLER_BATCHED
 TOL_BATCHED = 1e-4
 
 
-@pytest.mark.parametrize("K", [4, 10, 16, 17, 24, 32, 33, 48, 64])
+@pytest.mark.parametrize("K", [4, 10, 16, 17, 24, 32, 33, 48, 64, 65, 100, 128])
 def test_batched_sampler_matrix(K):
     announce("batched sampler matrix K=%d" % K)
     # heavy rows above the 2048-nnz segment length, rows at the boundary, an
@@ -422,7 +422,7 @@ def test_batched_sampler_matrix(K):
     assert err < TOL_BATCHED
 
 
-@pytest.mark.parametrize("K", [32, 64])
+@pytest.mark.parametrize("K", [32, 64, 128])
 def test_batched_sampler_item_side_and_tensor(K):
     announce("batched sampler item side / tensor K=%d" % K)
     rng, modes, mean, V, Lam, mu = _matrix_case(2, K, 3000, 200, 60000)
@@ -442,7 +442,7 @@ def test_batched_sampler_item_side_and_tensor(K):
         assert row_rel_err(got, ref) < TOL_BATCHED
 
 
-@pytest.mark.parametrize("K", [32, 64])
+@pytest.mark.parametrize("K", [32, 64, 128])
 def test_batched_sampler_per_row_mean(K):
     announce("batched sampler per-row mean K=%d" % K)
     rng, modes, mean, V, Lam, mu = _matrix_case(8, K, 400, 3000, 8000, heavy=(2500,))
@@ -452,7 +452,8 @@ def test_batched_sampler_per_row_mean(K):
     assert row_rel_err(got, ref) < TOL_BATCHED
 
 
-@pytest.mark.parametrize("precision,K", [(5.0, 32), ("adaptive", 32), ("adaptive", 64), ("adaptive", 10)])
+@pytest.mark.parametrize("precision,K", [(5.0, 32), ("adaptive", 32), ("adaptive", 64), ("adaptive", 10),
+                                         ("adaptive", 128)])
 def test_batched_model_matches_oracle(precision, K):
     announce("batched model precision=%s K=%d" % (precision, K))
     nrow, ncol = 3000, 3000
