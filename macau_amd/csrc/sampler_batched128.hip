@@ -449,7 +449,7 @@ __device__ inline void write_sse128(const SampleArgs& a, double acc, int slot) {
 }
 
 template <int NO>
-__global__ void __launch_bounds__(kThreads128) k_light128(SampleArgs a, BatchPlan p) {
+__global__ void __launch_bounds__(kThreads128, 2) k_light128(SampleArgs a, BatchPlan p) {
   __shared__ Lds128 s;
   const Factors<NO> F = make_factors<NO, 128>(a.others);
   const int it = *a.iter;
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(kThreads128) k_seg128(SampleArgs a, BatchPlan 
   }
 }
 
-__global__ void __launch_bounds__(kThreads128) k_heavy128(SampleArgs a, BatchPlan p, int sse_slot0) {
+__global__ void __launch_bounds__(kThreads128, 2) k_heavy128(SampleArgs a, BatchPlan p, int sse_slot0) {
   __shared__ Lds128 s;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int it = *a.iter;
