@@ -136,7 +136,8 @@ int macau_get_ftf_solver(macau_model* m, int mode);
  * per-observation residuals, FP64 sums).  macau_set_refine (before
  * macau_init) switches it off (on = 0) or sets the threshold (> 0);
  * macau_get_refine_stats reports the rows flagged and refined so far (a
- * half-sweep refines at most 4096 rows). */
+ * half-sweep refines at most 4096 rows).  num_latent above 64 is sampled
+ * without probes or refinement. */
 int macau_set_refine(macau_model* m, int on, double threshold);
 int macau_get_refine_stats(macau_model* m, long long* flagged, long long* refined);
 
