@@ -680,62 +680,167 @@ __global__ void __launch_bounds__(kNWThreads) k_nw_draw_lds(NWBatch b) {
   for (int k = t; k < K; k += kNWThreads)
     v_z[k] = normal_f64(a.seed, 0u, (uint32_t)k, (uint32_t)it, make_tag(TAG_NW_MU, a.mode));
   __syncthreads();
-  // 3. M = chol(J X J), right-looking; the trailing update on a 16 x 16
-  //    thread grid
+  // 3. M = chol(J X J), right-looking in blocks of 4 columns, two barriers
+  //    per block: every thread factors the 4 x 4 diagonal block itself
+  //    (pivots <= 0 flagged and replaced by 1, as column by column), thread
+  //    t solves row k0 + 4 + t of the block column, thread 0 stores the
+  //    diagonal block's factor after the barrier (once no thread reads the
+  //    old entries), and the rank-4 trailing update runs on a 16 x 16
+  //    thread grid, each row's entries loaded before any is stored
   const int tx = t & 15, ty = t >> 4;
-  for (int k = 0; k < K; k++) {
-    if (t == 0) {
-      const double d = Mp[tri(k) + k];
-      if (!(d > 0.0)) sflag = 1;
-      Mp[tri(k) + k] = sqrt(d > 0.0 ? d : 1.0);
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int nb = K - k0 < 4 ? K - k0 : 4;
+    double L4[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (c >= nb) break;
+      double d = Mp[tri(k0 + c) + k0 + c];
+#pragma unroll
+      for (int m = 0; m < c; m++) d -= L4[c][m] * L4[c][m];
+      if (t == 0 && !(d > 0.0)) sflag = 1;
+      L4[c][c] = sqrt(d > 0.0 ? d : 1.0);
+#pragma unroll
+      for (int r = c + 1; r < 4; r++) {
+        if (r >= nb) break;
+        double v = Mp[tri(k0 + r) + k0 + c];
+#pragma unroll
+        for (int m = 0; m < c; m++) v -= L4[r][m] * L4[c][m];
+        L4[r][c] = v / L4[c][c];
+      }
+    }
+    for (int i = k0 + nb + t; i < K; i += kNWThreads) {
+      double l[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) l[c] = c < nb ? Mp[tri(i) + k0 + c] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        if (c >= nb) break;
+#pragma unroll
+        for (int m = 0; m < c; m++) l[c] -= l[m] * L4[c][m];
+        l[c] = l[c] / L4[c][c];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (c < nb) Mp[tri(i) + k0 + c] = l[c];
     }
     __syncthreads();
-    const double inv = 1.0 / Mp[tri(k) + k];
-    for (int i = k + 1 + t; i < K; i += kNWThreads) Mp[tri(i) + k] *= inv;
-    __syncthreads();
-    for (int i = k + 1 + ty; i < K; i += 16) {
-      const double lik = Mp[tri(i) + k];
-#pragma unroll 4
-      for (int j = k + 1 + tx; j <= i; j += 16) Mp[tri(i) + j] -= lik * Mp[tri(j) + k];
+    if (t == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c <= r; c++)
+          if (r < nb) Mp[tri(k0 + r) + k0 + c] = L4[r][c];
+    }
+    const int j0 = k0 + nb;
+    for (int i = j0 + ty; i < K; i += 16) {
+      double li[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) li[c] = c < nb ? Mp[tri(i) + k0 + c] : 0.0;
+      double x[8], s4[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int j = j0 + tx + 16 * u;
+        const bool in = j <= i;
+        x[u] = in ? Mp[tri(i) + j] : 0.0;
+        double sv = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) sv += (in && c < nb) ? li[c] * Mp[tri(j) + k0 + c] : 0.0;
+        s4[u] = sv;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int j = j0 + tx + 16 * u;
+        if (j <= i) Mp[tri(i) + j] = x[u] - s4[u];
+      }
     }
     __syncthreads();
   }
   const bool ok = sflag == 0;
-  // 4. Z = M^{-T} R, R = J C^T, rows from the bottom; row i over column
-  //    K - 1 - i of C (slots tri(K-1-i) + c, c <= K - 1 - i)
-  for (int i = K - 1; i >= 0; i--) {
-    const int j = K - 1 - i;
-    for (int c = t; c <= j; c += kNWThreads) {
-      double sv = Cp[tri(j) + c];  // C[c][K-1-i]
-      // (unrolled so that the loads of eight steps are issued together; the
-      // subtractions keep their order)
-#pragma unroll 8
-      for (int k = i + 1; k + c <= K - 1; k++) sv -= Mp[tri(k) + i] * Cp[tri(K - 1 - k) + c];
-      Cp[tri(j) + c] = sv / Mp[tri(i) + i];
+  // 4. Z = M^{-T} R, R = J C^T, rows from the bottom, right-looking in
+  //    blocks of 4 rows: row i of R lives over column K - 1 - i of C (slots
+  //    tri(K-1-i) + c, c <= K - 1 - i).  For a block of rows b0..b0+nb-1,
+  //    thread c solves the block's 4 x 4 upper system for its column c
+  //    (rows i with c <= K - 1 - i), then every row i' < b0 subtracts
+  //    sum_k M_ki' Z_k over the block's rows k with c <= K - 1 - k (16 x 16
+  //    thread grid, each row's entries loaded before any store)
+  for (int b1 = K; b1 > 0; b1 -= 4) {
+    const int b0 = b1 - 4 > 0 ? b1 - 4 : 0, nb = b1 - b0;
+    for (int c = t; c <= K - 1 - b0; c += kNWThreads) {
+      double z[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 3; q >= 0; q--) {
+        const int i = b0 + q;
+        if (q >= nb || c > K - 1 - i) continue;
+        double v = Cp[tri(K - 1 - i) + c];
+#pragma unroll
+        for (int q2 = q + 1; q2 < 4; q2++)
+          if (q2 < nb && c <= K - 1 - (b0 + q2)) v -= Mp[tri(b0 + q2) + i] * z[q2];
+        z[q] = v / Mp[tri(i) + i];
+        Cp[tri(K - 1 - i) + c] = z[q];
+      }
+    }
+    __syncthreads();
+    const int cmax = K - 1 - b0;  // widest row of the block
+    for (int i2 = ty; i2 < b0; i2 += 16) {
+      double m[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) m[q] = q < nb ? Mp[tri(b0 + q) + i2] : 0.0;
+      const int j2 = K - 1 - i2;
+      double x[8], s4[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int c = tx + 16 * u;
+        const bool in = c <= cmax;
+        x[u] = in ? Cp[tri(j2) + c] : 0.0;
+        double sv = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          sv += (in && q < nb && c <= K - 1 - (b0 + q)) ? m[q] * Cp[tri(K - 1 - (b0 + q)) + c] : 0.0;
+        s4[u] = sv;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int c = tx + 16 * u;
+        if (c <= cmax) Cp[tri(j2) + c] = x[u] - s4[u];
+      }
     }
     __syncthreads();
   }
   // B[i][c] = Z[K-1-i][c] = Cp[tri(i) + c] (c <= i).
-  // 5. Lambda = B B^T (packed lower, over M)
+  // 5. Lambda = B B^T (packed lower, over M), four partial sums per entry
   for (int e = t; e < T2; e += kNWThreads) {
     int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
     while (tri(i) > e) i--;
     while (tri(i + 1) <= e) i++;
     const int j = e - tri(i);
-    double sv = 0.0;
-#pragma unroll 8
-    for (int k = 0; k <= j; k++) sv += Cp[tri(i) + k] * Cp[tri(j) + k];
-    Mp[e] = sv;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = 0;
+    for (; k + 4 <= j + 1; k += 4) {
+      s0 += Cp[tri(i) + k] * Cp[tri(j) + k];
+      s1 += Cp[tri(i) + k + 1] * Cp[tri(j) + k + 1];
+      s2 += Cp[tri(i) + k + 2] * Cp[tri(j) + k + 2];
+      s3 += Cp[tri(i) + k + 3] * Cp[tri(j) + k + 3];
+    }
+    for (; k <= j; k++) s0 += Cp[tri(i) + k] * Cp[tri(j) + k];
+    Mp[e] = (s0 + s1) + (s2 + s3);
+  }
+  // 6. mu = mu_c + B^{-T} z / sqrt(kappa_c): back substitution on wave 0,
+  //    lane l holding z_l and z_{l+64}, the solved value broadcast by a
+  //    shuffle (no barrier per step)
+  if (t < 64) {
+    double z0 = v_z[t], z1 = t + 64 < K ? v_z[t + 64] : 0.0;
+    for (int j = K - 1; j >= 0; j--) {
+      const double zj = j >= 64 ? __shfl(z1, j - 64, 64) : __shfl(z0, j, 64);
+      const double xj = zj / Cp[tri(j) + j];
+      if (t == j) z0 = xj;
+      if (t + 64 == j) z1 = xj;
+      if (t < j) z0 -= Cp[tri(j) + t] * xj;
+      if (t + 64 < j) z1 -= Cp[tri(j) + t + 64] * xj;
+    }
+    v_z[t] = z0;
+    if (t + 64 < K) v_z[t + 64] = z1;
   }
   __syncthreads();
-  // 6. mu = mu_c + B^{-T} z / sqrt(kappa_c)
-  for (int j = K - 1; j >= 0; j--) {
-    if (t == 0) v_z[j] = v_z[j] / Cp[tri(j) + j];
-    __syncthreads();
-    const double xj = v_z[j];
-    for (int i = t; i < j; i += kNWThreads) v_z[i] -= Cp[tri(j) + i] * xj;
-    __syncthreads();
-  }
   const double isk = 1.0 / sqrt(kappa_c);
   for (int k = t; k < K; k += kNWThreads) {
     const double m = v_mu[k] + v_z[k] * isk;
