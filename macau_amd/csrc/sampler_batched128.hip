@@ -13,20 +13,21 @@
 //     BF16 hi + lo and stores them in LDS; the ten 32 x 32 blocks (I, J),
 //     I >= J, of G are then shared out, each block taking the operands of
 //     component blocks I and J from LDS (HiHi^T + HiLo^T + LoHi^T + LoLo^T):
-//       wave 0: (0,0) (1,0) (1,1)    wave 1: (2,2) (3,2) (3,3)
-//       wave 2: (2,0) (2,1)          wave 3: (3,0) (3,1)
+//       wave 0: (0,0) (1,0) (1,1)    wave 1: (2,0) (2,1) (2,2)
+//       wave 2: (3,0) (3,1)          wave 3: (3,2) (3,3)
 //     The chunk's gathers are issued before the previous chunk's MFMAs, and
 //     its index loads one chunk earlier still.
-//   * Epilogue: waves 0 and 1 hold rows 0-63 and 64-127 of -P in the
-//     accumulator layout (wave 1 receives blocks (2,0) (2,1) (3,0) (3,1)
-//     through LDS) and factor it in 4-column steps as the K = 64 epilogue
-//     does within one wave: the step's columns and the RHS go through LDS
-//     (double-buffered by step parity), every row lane forms the 4 x 4
-//     diagonal factor, its own row of L and the forward solve, and the
-//     trailing rank-4 update runs on v_mfma_f32_32x32x2_f32 over the blocks
-//     still holding unfactored columns; while the first 64 columns are being
-//     factored wave 1 takes wave 0's update operands from LDS (a second
-//     barrier per step).  L is kept as a packed triangle in LDS; wave 0
+//   * Epilogue: each wave keeps its blocks of -P in the accumulator layout
+//     and the matrix is factored in 4-column steps as the K = 64 epilogue
+//     does within one wave: the owners of the step's block column publish
+//     its 4 columns to LDS (double-buffered by step parity); lanes of waves
+//     0 and 1 own rows 0-63 and 64-127 and each forms the 4 x 4 diagonal
+//     factor, its own row of L, the forward solve and its multipliers, which
+//     go to LDS; after a second barrier every wave applies the rank-4
+//     update to its blocks that still hold unfactored columns on
+//     v_mfma_f32_32x32x2_f32, its operands read from LDS (at most 6 MFMAs
+//     per wave and step).  L is kept as a packed triangle in LDS (over the
+//     Gram operands' buffers, three work-groups per CU); wave 0
 //     does the back solve with two rows per lane (readlane broadcasts).
 //   * No condition probes: refinement is not offered at this size.
 #include <stdexcept>
@@ -39,19 +40,18 @@ constexpr int kPartial128 = 10 * 1024 + 256;  // floats per segment partial: 10 
 constexpr int kThreads128 = 256;
 
 // Gram blocks of each wave (block row, block column; -1: none)
-__constant__ int kBI128[4][3] = {{0, 1, 1}, {2, 3, 3}, {2, 2, -1}, {3, 3, -1}};
-__constant__ int kBJ128[4][3] = {{0, 0, 1}, {2, 2, 3}, {0, 1, -1}, {0, 1, -1}};
+__constant__ int kBI128[4][3] = {{0, 1, 1}, {2, 2, 2}, {3, 3, -1}, {3, 3, -1}};
+__constant__ int kBJ128[4][3] = {{0, 0, 1}, {0, 1, 2}, {0, 1, -1}, {2, 3, -1}};
 
 struct Lds128 {
   union {
-    uint4 op[2][4][64][2];  // Gram operands: [buffer][component block][lane][hi, lo]
-    float hand[4][1024];    // blocks (2,0) (2,1) (3,0) (3,1) handed to wave 1
+    uint4 op[2][4][64][2];    // Gram operands: [buffer][component block][lane][hi, lo]
+    float Lf[128 * 129 / 2];  // the epilogue's L, row i at i (i + 1) / 2 (the operands are dead by then)
   } u;
   float csh[128];           // RHS sums c (alpha c + Lambda mu is the RHS)
   float WT[2][4][128];      // the step's columns of -P (double-buffered by step parity)
   float Wb[2][128];         // the step's RHS values
-  float xch[4][64];         // wave 0's update operands for wave 1's blocks (2|3, 0|1)
-  float Lf[128 * 129 / 2];  // L, row i at i (i + 1) / 2
+  float qsh[4][128];        // the step's multipliers of every row (zero at and above the block)
   float hb[2][64];          // rows 64-127: b and 1 / L_ii, for the back solve on wave 0
   float us[128];            // the sampled row (SSE)
   float ws[64];             // rows 64-127: L^{-1} b + z (SSE)
@@ -157,54 +157,29 @@ __device__ double finish128(const SampleArgs& a, int row, Gram128 g, int it, flo
     const float c = g.bp + __shfl_xor(g.bp, 32, 64);
     if (lane < 32) s.csh[32 * w + lane] = c;
   }
-  // waves 2 and 3 hand their blocks to wave 1 (the operand buffers are free
-  // once every wave has passed this barrier)
-  __syncthreads();
-  if (w >= 2) {
+  // each wave keeps its Gram blocks as -(Lambda + alpha G)
+  const int nb = w < 2 ? 3 : 2;
+  int BI[3], BJ[3];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      s.u.hand[2 * (w - 2)][r * 64 + lane] = g.g[0][r];
-      s.u.hand[2 * (w - 2) + 1][r * 64 + lane] = g.g[1][r];
+  for (int q = 0; q < 3; q++) {
+    BI[q] = kBI128[w][q];
+    BJ[q] = kBJ128[w][q];
+  }
+  f32x16 e[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    e[q] = g.g[q];
+    if (q >= nb) continue;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const float4 l4 =
+          *reinterpret_cast<const float4*>(a.Lambda_f + (32 * BJ[q] + c32) * 128 + 32 * BI[q] + 8 * m + 4 * half);
+      const float el[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+      for (int t = 0; t < 4; t++) e[q][4 * m + t] = -fmaf(alpha, e[q][4 * m + t], el[t]);
     }
   }
-  __syncthreads();
-  // e[]: wave 0 blocks (0,0) (1,0) (1,1); wave 1 blocks (2,0) (2,1) (2,2)
-  // (3,0) (3,1) (3,2) (3,3), as -(Lambda + alpha G)
-  f32x16 e[7];
-  if (w == 0) {
-    e[0] = g.g[0];
-    e[1] = g.g[1];
-    e[2] = g.g[2];
-  } else if (w == 1) {
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      e[0][r] = s.u.hand[0][r * 64 + lane];
-      e[1][r] = s.u.hand[1][r * 64 + lane];
-      e[3][r] = s.u.hand[2][r * 64 + lane];
-      e[4][r] = s.u.hand[3][r * 64 + lane];
-    }
-    e[2] = g.g[0];
-    e[5] = g.g[1];
-    e[6] = g.g[2];
-  }
-  const int nb = w == 0 ? 3 : 7;
-  if (w < 2) {
-#pragma unroll
-    for (int q = 0; q < 7; q++) {
-      if (q >= nb) break;
-      // block (I, J) of this wave's q
-      const int I = w == 0 ? (q == 0 ? 0 : 1) : (q < 3 ? 2 : 3);
-      const int J = w == 0 ? (q == 2 ? 1 : 0) : (q < 3 ? q : q - 3);
-#pragma unroll
-      for (int m = 0; m < 4; m++) {
-        const float4 l4 =
-            *reinterpret_cast<const float4*>(a.Lambda_f + (32 * J + c32) * 128 + 32 * I + 8 * m + 4 * half);
-        const float el[4] = {l4.x, l4.y, l4.z, l4.w};
-#pragma unroll
-        for (int t = 0; t < 4; t++) e[q][4 * m + t] = -fmaf(alpha, e[q][4 * m + t], el[t]);
-      }
-    }
-  }
+  __syncthreads();  // csh complete
   const int i = 64 * w + lane;  // this lane's row (waves 0 and 1)
   const bool rl = w < 2;
   float c = 0.f, b = 0.f, inv_diag = 1.f, zi = 0.f;
@@ -226,28 +201,23 @@ __device__ double finish128(const SampleArgs& a, int row, Gram128 g, int it, flo
   for (int k0 = 0; k0 < 128; k0 += 4) {
     opaque(lane);
     const int J0 = k0 >> 5, cc = k0 & 31, pb = (k0 >> 2) & 1;
-    // 1. publish columns k0..k0+3 of -P (block column J0) and the RHS
-    if (rl) {
-      if ((unsigned)(c32 - cc) < 4u) {
-        const int u = c32 - cc;
+    // 1. publish columns k0..k0+3 of -P (the owners of block column J0) and
+    // the RHS
+    if ((unsigned)(c32 - cc) < 4u) {
+      const int u = c32 - cc;
 #pragma unroll
-        for (int q = 0; q < 7; q++) {
-          if (q >= nb) break;
-          const int I = w == 0 ? (q == 0 ? 0 : 1) : (q < 3 ? 2 : 3);
-          const int J = w == 0 ? (q == 2 ? 1 : 0) : (q < 3 ? q : q - 3);
-          if (J != J0) continue;
+      for (int q = 0; q < 3; q++) {
+        if (q >= nb || BJ[q] != J0) continue;
 #pragma unroll
-          for (int m = 0; m < 4; m++)
-            *reinterpret_cast<float4*>(&s.WT[pb][u][32 * I + 8 * m + 4 * half]) =
-                make_float4(e[q][4 * m], e[q][4 * m + 1], e[q][4 * m + 2], e[q][4 * m + 3]);
-        }
+        for (int m = 0; m < 4; m++)
+          *reinterpret_cast<float4*>(&s.WT[pb][u][32 * BI[q] + 8 * m + 4 * half]) =
+              make_float4(e[q][4 * m], e[q][4 * m + 1], e[q][4 * m + 2], e[q][4 * m + 3]);
       }
-      s.Wb[pb][i] = b;
     }
+    if (rl) s.Wb[pb][i] = b;
     __syncthreads();
     // 2. the 4 x 4 diagonal factor, this row's entries of L and the forward
-    // solve
-    float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+    // solve; the rows' multipliers go to LDS for the update
     if (rl) {
       const float w0 = -s.WT[pb][0][i], w1 = -s.WT[pb][1][i], w2 = -s.WT[pb][2][i], w3 = -s.WT[pb][3][i];
       const float4 c0 = *reinterpret_cast<const float4*>(&s.WT[pb][0][k0]);
@@ -280,7 +250,7 @@ __device__ double finish128(const SampleArgs& a, int row, Gram128 g, int it, flo
       const float p2 = (w2 - p0 * l20 - p1 * l21) * r2;
       const float p3 = (w3 - p0 * l30 - p1 * l31 - p2 * l32) * r3;
       if (k0 <= i) {
-        float* lr = s.Lf + tri_i + k0;
+        float* lr = s.u.Lf + tri_i + k0;
         lr[0] = p0;
         if (k0 + 1 <= i) lr[1] = p1;
         if (k0 + 2 <= i) lr[2] = p2;
@@ -292,69 +262,26 @@ __device__ double finish128(const SampleArgs& a, int row, Gram128 g, int it, flo
       const float rt = t == 0 ? r0 : (t == 1 ? r1 : (t == 2 ? r2 : r3));
       b = inb ? yt : (below ? b - (p0 * y0 + p1 * y1 + p2 * y2 + p3 * y3) : b);
       inv_diag = inb ? rt : inv_diag;
-      q0 = below ? p0 : 0.f;
-      q1 = below ? p1 : 0.f;
-      q2 = below ? p2 : 0.f;
-      q3 = below ? p3 : 0.f;
+      if (k0 + 4 < 128) {
+        s.qsh[0][i] = below ? p0 : 0.f;
+        s.qsh[1][i] = below ? p1 : 0.f;
+        s.qsh[2][i] = below ? p2 : 0.f;
+        s.qsh[3][i] = below ? p3 : 0.f;
+      }
     }
     if (k0 + 4 >= 128) break;
-    // 3. -P += p p^T over the blocks that still hold unfactored columns.
-    // swap(q0, q1): .x = (q0 | q1 of the wave's rows 0-31), .y = (q0 | q1 of
-    // its rows 32-63), the MFMA operand layout of its two block rows
-    const auto s01 = __builtin_amdgcn_permlane32_swap(__float_as_uint(q0), __float_as_uint(q1), false, false);
-    const auto s23 = __builtin_amdgcn_permlane32_swap(__float_as_uint(q2), __float_as_uint(q3), false, false);
-    const float t01 = __uint_as_float(s01[0]), u01 = __uint_as_float(s01[1]);
-    const float t23 = __uint_as_float(s23[0]), u23 = __uint_as_float(s23[1]);
-    const int jmin = (k0 + 4) >> 5;  // first block column with unfactored columns
-    if (jmin < 2) {
-      if (w == 0) {
-        s.xch[0][lane] = t01;
-        s.xch[1][lane] = t23;
-        s.xch[2][lane] = u01;
-        s.xch[3][lane] = u23;
-      }
-      __syncthreads();
-    }
-    if (w == 0) {
-      if (jmin == 0) {
-        e[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(t01, t01, e[0], 0, 0, 0);
-        e[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(t23, t23, e[0], 0, 0, 0);
-        e[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(u01, t01, e[1], 0, 0, 0);
-        e[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(u23, t23, e[1], 0, 0, 0);
-      }
-      if (jmin <= 1) {
-        e[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(u01, u01, e[2], 0, 0, 0);
-        e[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(u23, u23, e[2], 0, 0, 0);
-      }
-    } else if (w == 1) {
-      // block rows 2 (t) and 3 (u); block columns 0, 1 from wave 0
-      float x01 = 0.f, x23 = 0.f, y01 = 0.f, y23 = 0.f;
-      if (jmin < 2) {
-        x01 = s.xch[0][lane];
-        x23 = s.xch[1][lane];
-        y01 = s.xch[2][lane];
-        y23 = s.xch[3][lane];
-      }
-      if (jmin == 0) {
-        e[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(t01, x01, e[0], 0, 0, 0);
-        e[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(t23, x23, e[0], 0, 0, 0);
-        e[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(u01, x01, e[3], 0, 0, 0);
-        e[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(u23, x23, e[3], 0, 0, 0);
-      }
-      if (jmin <= 1) {
-        e[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(t01, y01, e[1], 0, 0, 0);
-        e[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(t23, y23, e[1], 0, 0, 0);
-        e[4] = __builtin_amdgcn_mfma_f32_32x32x2f32(u01, y01, e[4], 0, 0, 0);
-        e[4] = __builtin_amdgcn_mfma_f32_32x32x2f32(u23, y23, e[4], 0, 0, 0);
-      }
-      if (jmin <= 2) {
-        e[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(t01, t01, e[2], 0, 0, 0);
-        e[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(t23, t23, e[2], 0, 0, 0);
-        e[5] = __builtin_amdgcn_mfma_f32_32x32x2f32(u01, t01, e[5], 0, 0, 0);
-        e[5] = __builtin_amdgcn_mfma_f32_32x32x2f32(u23, t23, e[5], 0, 0, 0);
-      }
-      e[6] = __builtin_amdgcn_mfma_f32_32x32x2f32(u01, u01, e[6], 0, 0, 0);
-      e[6] = __builtin_amdgcn_mfma_f32_32x32x2f32(u23, u23, e[6], 0, 0, 0);
+    __syncthreads();
+    // 3. -P += p p^T over this wave's blocks that still hold unfactored
+    // columns; the MFMA operand of block row X on lane l is the multiplier
+    // l / 32 (or 2 + l / 32) of row 32 X + l % 32
+    const int jmin = (k0 + 4) >> 5;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (q >= nb || BJ[q] < jmin) continue;
+      const float a01 = s.qsh[half][32 * BI[q] + c32], a23 = s.qsh[2 + half][32 * BI[q] + c32];
+      const float b01 = s.qsh[half][32 * BJ[q] + c32], b23 = s.qsh[2 + half][32 * BJ[q] + c32];
+      e[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a01, b01, e[q], 0, 0, 0);
+      e[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a23, b23, e[q], 0, 0, 0);
     }
   }
   if (bad && active && i == 0) atomicMax(a.fail, row + 1);  // the pivots are the same on every row lane
@@ -376,8 +303,8 @@ __device__ double finish128(const SampleArgs& a, int row, Gram128 g, int it, flo
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int j = gq - u;
-        La[u] = s.Lf[j * (j + 1) / 2 + lane];
-        Lb[u] = s.Lf[j * (j + 1) / 2 + ((64 + lane) <= j ? 64 + lane : j)];
+        La[u] = s.u.Lf[j * (j + 1) / 2 + lane];
+        Lb[u] = s.u.Lf[j * (j + 1) / 2 + ((64 + lane) <= j ? 64 + lane : j)];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
@@ -393,7 +320,7 @@ __device__ double finish128(const SampleArgs& a, int row, Gram128 g, int it, flo
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int j = gq - u;
-        La[u] = s.Lf[j * (j + 1) / 2 + (lane <= j ? lane : j)];
+        La[u] = s.u.Lf[j * (j + 1) / 2 + (lane <= j ? lane : j)];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
@@ -449,7 +376,7 @@ __device__ inline void write_sse128(const SampleArgs& a, double acc, int slot) {
 }
 
 template <int NO>
-__global__ void __launch_bounds__(kThreads128, 2) k_light128(SampleArgs a, BatchPlan p) {
+__global__ void __launch_bounds__(kThreads128, 3) k_light128(SampleArgs a, BatchPlan p) {
   __shared__ Lds128 s;
   const Factors<NO> F = make_factors<NO, 128>(a.others);
   const int it = *a.iter;
@@ -489,7 +416,7 @@ __global__ void __launch_bounds__(kThreads128) k_seg128(SampleArgs a, BatchPlan 
   }
 }
 
-__global__ void __launch_bounds__(kThreads128, 2) k_heavy128(SampleArgs a, BatchPlan p, int sse_slot0) {
+__global__ void __launch_bounds__(kThreads128, 3) k_heavy128(SampleArgs a, BatchPlan p, int sse_slot0) {
   __shared__ Lds128 s;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int it = *a.iter;
