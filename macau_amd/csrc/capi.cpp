@@ -608,8 +608,9 @@ int macau_solve_blockcg(int nrhs, int nrow, int ncol, long long nnz, const int* 
   CgBuffers cg;
   cg.alloc(nrhs, ncol, nrow);
   HIPCHK(hipStreamSynchronize(s));
+  const CsrDev fd = F.Fd(), ftd = F.Ftd();
   const int it = blockcg_device(F.F(), F.Ft(), dreg.as<double>(), dB.as<double>(), nrhs, 0, nrhs, tol,
-                                dX.as<double>(), cg, s);
+                                dX.as<double>(), cg, s, false, &fd, &ftd);
   HIPCHK(hipMemcpyAsync(X, dX.as<void>(), (size_t)nrhs * ncol * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (iters) *iters = it;

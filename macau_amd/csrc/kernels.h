@@ -274,7 +274,27 @@ struct CsrDev {
   const int* multi_row = nullptr;
   const int* multi_slot0 = nullptr;
   double* partial = nullptr;  // slots x nr
+  // optional dense block of a binary F (launch_spmm with nr = 32 only): the
+  // ndp (a multiple of 16, at most kDenseMaxCols) densest columns of F in bit
+  // form; dense_role 1 on F (rows), 2 on its transpose (the dense columns
+  // are rows there)
+  int dense_role = 0;
+  int ndp = 0, nd = 0;        // padded and real count of dense columns
+  int nfrow = 0;              // rows of F
+  int ndw = 0;                // words per row of row_bits: ndp / 32 rounded up
+  int nrw = 0;                // words per dense column of col_bits: (nfrow + 31) / 32
+  const int* dcol = nullptr;  // ndp column ids of F (padding: column 0, no bits)
+  const uint32_t* row_bits = nullptr;  // role 1: bit j of row r set when F[r][dcol[j]] = 1
+  const int* rem_ptr = nullptr;        // role 1: F without the dense columns (binary CSR)
+  const int* rem_cols = nullptr;
+  const int* rem_fix = nullptr;        // role 1: the first kDenseRem of rem_cols per row (padding: column 0)
+  const uint32_t* col_bits = nullptr;  // role 2: bit r of dense column j at word j * nrw + r / 32
+  int nkc = 0;                // role 2: chunks of kDenseRowChunk rows of F per dense column
+  int dslot0 = 0;             // role 2: partial slot of (column j, chunk k) = dslot0 + j * nkc + k
 };
+constexpr int kDenseMaxCols = 288;
+constexpr int kDenseRowChunk = 256;
+constexpr int kDenseRem = 8;
 
 // --- F^T F branch of MacauPrior::sample_beta (latentprior.cpp:280-289) ---
 // fail-flag value set when F^T F + lambda_beta I is not positive definite
@@ -324,7 +344,7 @@ void launch_sumsq_f64(const double* x, size_t n, double* partial, int nblk, hipS
 // when the launch completes (nblk = dots_blocks(n))
 int dots_blocks(int n);
 void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
-                 const double* skip = nullptr);
+                 const double* skip = nullptr, bool reduce = true);
 
 struct CgState {
   int nr = 0, n = 0;

@@ -6,12 +6,29 @@
 // row-major [nrhs x nfeat] Eigen matrices, linop.h:249-266).  The SpMM
 // F P^T (A_mul_Bx<N>, linop.h:374-403) gathers one such column per nonzero
 // with NP lanes, 64/NP nonzeros per wave-step: an HBM/L2 gather-bound loop.
+#include <algorithm>
+#include <stdexcept>
+
 #include "common.h"
 #include "kernels.h"
 
 namespace mgpu {
 
 constexpr int kSpmmThreads = 256;
+
+// four doubles: an FP64 matrix-core accumulator, or a 32-byte vector access
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 ld4(const double* p) {
+  const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
+  return dbl4{a.x, a.y, b.x, b.y};
+}
+
+__device__ __forceinline__ void st4(double* p, dbl4 v) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+
 
 // one wave per row (matrices without a work list)
 template <int NP>
@@ -228,13 +245,12 @@ __device__ __forceinline__ void spmm_empty16(const CsrDev& A, int first, int nr,
 
 // work units: the A.nlong long segments, then groups of four short ones,
 // then groups of sixteen empty rows
-__global__ void __launch_bounds__(kSpmmThreads)
-    k_spmm_hyb(CsrDev A, const double* __restrict__ in, int nr, double* __restrict__ out,
-               const double* __restrict__ reg, const double* __restrict__ add, const double* skip) {
-  if (skip && *skip != 0.0) return;
+__device__ __forceinline__ void hyb_units(const CsrDev& A, const double* __restrict__ in, int nr,
+                                          double* __restrict__ out, const double* __restrict__ reg,
+                                          const double* __restrict__ add, int block, int nblock) {
   const int lane = threadIdx.x & 63;
-  const int w = (blockIdx.x * kSpmmThreads + threadIdx.x) >> 6;
-  const int nw = (gridDim.x * kSpmmThreads) >> 6;
+  const int w = (block * kSpmmThreads + threadIdx.x) >> 6;
+  const int nw = (nblock * kSpmmThreads) >> 6;
   const double rg = add ? *reg : 0.0;
   const int e0 = A.nseg - A.nempty;  // first empty row's entry
   const int nshort = (e0 - A.nlong + 3) / 4;
@@ -244,6 +260,13 @@ __global__ void __launch_bounds__(kSpmmThreads)
     else if (u < A.nlong + nshort) spmm_short4(A, A.nlong + 4 * (u - A.nlong), in, nr, out, rg, add, lane, e0);
     else spmm_empty16(A, e0 + 16 * (u - A.nlong - nshort), nr, out, rg, add, lane);
   }
+}
+
+__global__ void __launch_bounds__(kSpmmThreads)
+    k_spmm_hyb(CsrDev A, const double* __restrict__ in, int nr, double* __restrict__ out,
+               const double* __restrict__ reg, const double* __restrict__ add, const double* skip) {
+  if (skip && *skip != 0.0) return;
+  hyb_units(A, in, nr, out, reg, add, blockIdx.x, gridDim.x);
 }
 
 // rows split into several segments: a work-group per row sums the row's
@@ -278,14 +301,230 @@ __global__ void __launch_bounds__(kSpmmThreads)
   }
 }
 
+
+// F P for a binary F with a dense column block, nr = 32 (dense_role 1): the
+// dense columns through FP64 matrix-core products, the other nonzeros by
+// gathers.  A work-group (4 waves) takes 32 rows of F; wave (tile, half)
+// owns rows 16 tile .. + 15 and components 16 half .. + 15 of them.  The
+// work-group first copies the in-rows of the dense columns into LDS as
+// [half][j][16] (a k-step's four rows of 16 doubles: two bank halves, each
+// read by two rows, the minimum for 64 lanes x 8 bytes).  MFMA step s
+// multiplies the 16 x 4 bit block of the tile's rows x dense columns
+// 4 s .. 4 s + 3 (lane l: row l % 16, column 4 s + l / 16, a bit of the
+// row's words) by the 4 x 16 block of in-rows (lane l: column 4 s + l / 16,
+// component l % 16).  Output slot v of lane l is row 4 v + l / 16,
+// component l % 16.  The first kDenseRem remaining nonzeros of each row
+// come from a fixed-width table, so their gathers are two loads deep and
+// overlap the copy; rows with more take further batches after the
+// products.  All loads that need no earlier load are issued before the
+// convergence flag is tested.  Sums run in the matrix core's order, then
+// the remaining nonzeros in CSR order.
+constexpr int kDenseStage = kDenseMaxCols * 16 / 256;  // 16-byte copies per thread
+
+__global__ void __launch_bounds__(256)
+    k_spmm_dense_f(CsrDev A, const double* __restrict__ in, double* __restrict__ out, const double* skip) {
+  __shared__ double sP[2 * kDenseMaxCols * 16];
+  const double sk = skip ? *skip : 0.0;
+  const int ndp = A.ndp;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
+  const int half = wv & 1;
+  const int r0 = blockIdx.x * 32 + (wv >> 1) * 16;
+  const int ra = min(r0 + m, A.nfrow - 1);  // rows past the end read the last row and are not stored
+  // loads at clamped addresses, unused values dropped afterwards: no branch
+  // around a load, so they are all in flight together
+  uint32_t wd[kDenseMaxCols / 32];
+#pragma unroll
+  for (int u = 0; u < kDenseMaxCols / 32; u++) wd[u] = A.row_bits[(size_t)ra * A.ndw + min(u, A.ndw - 1)];
+  int col[kDenseStage];
+#pragma unroll
+  for (int u = 0; u < kDenseStage; u++) col[u] = A.dcol[min((int)threadIdx.x + 256 * u, ndp * 16 - 1) >> 4];
+  // the lane's output rows 4 v + l / 16
+  int rb[4], re[4];
+  int4 rc[4][kDenseRem / 4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int row = min(r0 + 4 * v + kq, A.nfrow - 1);
+    rb[v] = A.rem_ptr[row];
+    re[v] = A.rem_ptr[row + 1];
+#pragma unroll
+    for (int t = 0; t < kDenseRem / 4; t++) rc[v][t] = reinterpret_cast<const int4*>(A.rem_fix + (size_t)row * kDenseRem)[t];
+  }
+  asm volatile("" ::: "memory");  // the loads above stay ahead of the flag's test
+  if (sk != 0.0) return;
+  double2 cv[kDenseStage];
+#pragma unroll
+  for (int u = 0; u < kDenseStage; u++) {
+    const int e = threadIdx.x + 256 * u;
+    cv[u] = reinterpret_cast<const double2*>(in + (size_t)col[u] * 32)[e & 15];
+  }
+  double rv[4][kDenseRem];
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+#pragma unroll
+    for (int t = 0; t < kDenseRem / 4; t++) {
+      rv[v][4 * t + 0] = in[(size_t)rc[v][t].x * 32 + 16 * half + m];
+      rv[v][4 * t + 1] = in[(size_t)rc[v][t].y * 32 + 16 * half + m];
+      rv[v][4 * t + 2] = in[(size_t)rc[v][t].z * 32 + 16 * half + m];
+      rv[v][4 * t + 3] = in[(size_t)rc[v][t].w * 32 + 16 * half + m];
+    }
+  // the copies' loads are all issued before the first of their guarded stores
+#pragma unroll
+  for (int u = 0; u < kDenseStage; u++) asm volatile("" ::"v"(cv[u].x), "v"(cv[u].y));
+#pragma unroll
+  for (int u = 0; u < kDenseStage; u++) {
+    const int e = threadIdx.x + 256 * u, j = e >> 4, p = e & 15;
+    if (e < ndp * 16) *reinterpret_cast<double2*>(sP + ((size_t)(p >> 3) * ndp + j) * 16 + 2 * (p & 7)) = cv[u];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) re[v] = r0 + 4 * v + kq < A.nfrow ? re[v] : rb[v];
+#pragma unroll
+  for (int u = 0; u < kDenseMaxCols / 32; u++) wd[u] = u < A.ndw ? wd[u] : 0u;
+  __syncthreads();
+  const double* sB = sP + (size_t)half * ndp * 16 + m;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  // 16 dense columns (4 steps) at a time: their LDS reads, then the products
+#pragma unroll
+  for (int g = 0; g < kDenseMaxCols / 16; g++) {
+    if (16 * g >= ndp) break;
+    double b[4];
+#pragma unroll
+    for (int st = 0; st < 4; st++) b[st] = sB[(16 * g + 4 * st + kq) * 16];
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      const double a = (wd[g >> 1] >> (16 * (g & 1) + 4 * st + kq)) & 1u ? 1.0 : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[st], acc, 0, 0, 0);
+    }
+  }
+  double x[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    x[v] = acc[v];
+#pragma unroll
+    for (int t = 0; t < kDenseRem; t++)
+      if (rb[v] + t < re[v]) x[v] += rv[v][t];
+  }
+  // rows with more: batches of kDenseRem for every row of the wave at once
+  for (int e0 = kDenseRem;; e0 += kDenseRem) {
+    bool left = false;
+#pragma unroll
+    for (int v = 0; v < 4; v++) left |= rb[v] + e0 < re[v];
+    if (!__any(left)) break;
+    int c2[4][kDenseRem];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int t = 0; t < kDenseRem; t++) c2[v][t] = A.rem_cols[max(min(rb[v] + e0 + t, re[v] - 1), 0)];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int t = 0; t < kDenseRem; t++) rv[v][t] = in[(size_t)c2[v][t] * 32 + 16 * half + m];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int t = 0; t < kDenseRem; t++)
+        if (rb[v] + e0 + t < re[v]) x[v] += rv[v][t];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int row = r0 + 4 * v + kq;
+    if (row < A.nfrow) out[(size_t)row * 32 + 16 * half + m] = x[v];
+  }
+}
+
+// partial sums of F^T Y over the dense columns of a binary F, nr = 32
+// (dense_role 2 on F^T).  Work-group (group g, half h, chunk c) copies the
+// kDenseRowChunk rows of Y in chunk c, components 16 h .. 16 h + 15, into
+// LDS ([row][16]: a step's four rows fall two to a bank half), and its wave
+// w multiplies the 16 x kDenseRowChunk bit block of dense columns
+// 16 (4 g + w) .. + 15 x those rows of F (lane l: column l % 16, row
+// 4 s + l / 16 of step s, from the column's bit words) by them (lane l: row
+// 4 s + l / 16, component l % 16), writing slot dslot0 + j nkc + c of each
+// dense column j; k_spmm_fix adds the chunks in order (and reg * add).  Rows
+// past the end have no bits; their copies read the last row.
+__device__ __forceinline__ void dense_ft_block(const CsrDev& A, const double* __restrict__ in, int bid,
+                                               double* sY) {
+  constexpr int KS = kDenseRowChunk / 4, NW = kDenseRowChunk / 32;
+  constexpr int CP = kDenseRowChunk * 8 / 256;  // 16-byte copies per thread
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
+  const int kc = bid % A.nkc, half = (bid / A.nkc) & 1, g = bid / A.nkc >> 1;
+  const int d0 = (4 * g + wv) * 16;
+  const bool active = d0 < A.ndp;  // a uniform per wave
+  const int rb = kc * kDenseRowChunk;
+  double2 cv[CP];
+#pragma unroll
+  for (int u = 0; u < CP; u++) {
+    const int e = threadIdx.x + 256 * u, row = min(rb + (e >> 3), A.nfrow - 1);
+    cv[u] = reinterpret_cast<const double2*>(in + (size_t)row * 32 + 16 * half)[e & 7];
+  }
+  uint32_t wd[NW];
+#pragma unroll
+  for (int u = 0; u < NW; u++)
+    wd[u] = A.col_bits[(size_t)min(d0 + m, A.ndp - 1) * A.nrw + min(rb / 32 + u, A.nrw - 1)];
+#pragma unroll
+  for (int u = 0; u < CP; u++) {
+    const int e = threadIdx.x + 256 * u;
+    *reinterpret_cast<double2*>(sY + (e >> 3) * 16 + 2 * (e & 7)) = cv[u];
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int u = 0; u < NW; u++) wd[u] = rb / 32 + u < A.nrw ? wd[u] : 0u;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int gs = 0; gs < KS / 8; gs++) {
+    double b[8];
+#pragma unroll
+    for (int st = 0; st < 8; st++) b[st] = sY[(32 * gs + 4 * st + kq) * 16 + m];
+#pragma unroll
+    for (int st = 0; st < 8; st++) {
+      const double a = (wd[gs] >> (4 * st + kq)) & 1u ? 1.0 : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[st], acc, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int j = d0 + 4 * v + kq;
+    if (j < A.nd) A.partial[((size_t)A.dslot0 + (size_t)j * A.nkc + kc) * 32 + 16 * half + m] = acc[v];
+  }
+}
+
+// F^T Y + reg add for the dense-block plan of F^T in one launch: the first
+// ndense work-groups take the dense columns' chunk sums (above), the rest
+// the other rows' work units (k_spmm_hyb); k_spmm_fix then completes the
+// split rows and the dense columns
+__global__ void __launch_bounds__(256)
+    k_spmm_dense_ft(CsrDev A, const double* __restrict__ in, int nr, double* __restrict__ out,
+                    const double* __restrict__ reg, const double* __restrict__ add, int ndense, const double* skip) {
+  __shared__ double sY[kDenseRowChunk * 16];
+  if (skip && *skip != 0.0) return;
+  if ((int)blockIdx.x < ndense) dense_ft_block(A, in, blockIdx.x, sY);
+  else hyb_units(A, in, nr, out, reg, add, blockIdx.x - ndense, gridDim.x - ndense);
+}
+
 template <int NP>
 static void launch_spmm_t(const CsrDev& A, const double* in, int nr, double* out, const double* reg,
                           const double* add, hipStream_t s, const double* skip) {
+  const bool aligned = (((uintptr_t)in | (uintptr_t)out | (uintptr_t)add | (uintptr_t)A.partial) & 15) == 0;
+  if (A.dense_role == 1 && nr == 32 && !add && aligned) {
+    hipLaunchKernelGGL(k_spmm_dense_f, dim3((A.nfrow + 31) / 32), dim3(256), 0, s, A, in, out, skip);
+    return;
+  }
+  if (A.dense_role == 2 && nr == 32 && aligned && A.nd > 0) {
+    const int ndense = (A.ndp / 16 + 3) / 4 * 2 * A.nkc;  // four column tiles per work-group
+    const int nunit = A.nlong + (A.nseg - A.nempty - A.nlong + 3) / 4 + (A.nempty + 15) / 16;
+    const int g = A.nseg > 0 ? std::min((nunit + 3) / 4, 16384) : 0;
+    hipLaunchKernelGGL(k_spmm_dense_ft, dim3(ndense + g), dim3(256), 0, s, A, in, nr, out, reg, add, ndense, skip);
+    if (A.nmulti > 0) {
+      const int g2 = A.nmulti < 4096 ? A.nmulti : 4096;
+      hipLaunchKernelGGL(k_spmm_fix, dim3(g2), dim3(kSpmmThreads), 0, s, A, nr, out, reg, add, skip);
+    }
+    return;
+  }
+  if (A.dense_role != 0) throw std::logic_error("dense-block SpMM plan used with nr != 32 or unaligned vectors");
   if (A.nseg > 0) {
     int grid = (A.nseg + 3) / 4;
     if (grid > 16384) grid = 16384;
-    const uintptr_t al = (uintptr_t)in | (uintptr_t)out | (uintptr_t)add | (uintptr_t)A.partial;
-    if (NP == 32 && nr % 2 == 0 && (al & 15) == 0) {
+    if (NP == 32 && nr % 2 == 0 && aligned) {
       const int nunit = A.nlong + (A.nseg - A.nempty - A.nlong + 3) / 4 + (A.nempty + 15) / 16;
       int g = (nunit + 3) / 4;
       if (g > 16384) g = 16384;
@@ -488,11 +727,11 @@ __global__ void __launch_bounds__(256)
 }
 
 void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
-                 const double* skip) {
+                 const double* skip, bool reduce) {
   const int cols = (n + nblk - 1) / nblk;
   if (nr == 32 && (((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
     hipLaunchKernelGGL(k_dots32, dim3(nblk), dim3(256), 0, s, a, b, n, cols, partial, skip);
-    if (nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
+    if (reduce && nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
     return;
   }
 #define MGPU_DOTS(P)                                                                                        \
@@ -503,7 +742,7 @@ void launch_dots(const double* a, const double* b, int nr, int n, double* partia
   else if (nr <= 32) MGPU_DOTS(32);
   else MGPU_DOTS(40);
 #undef MGPU_DOTS
-  if (nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
+  if (reduce && nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
 }
 
 // ------------------------------------------------------------------
@@ -525,6 +764,14 @@ struct SmallView {
   }
 };
 
+// The small steps run on one wave: the LDS accesses of a wave are performed
+// in order, so a wait for them and a compiler fence order its lanes' LDS
+// stores before the following loads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  lds_wait();
+}
+
 // reduce partials -> lower-symmetric matrix in LDS
 // The small-step kernels run one wave (64 threads); their nr x nr operands
 // are moved with all loads of a thread issued together.
@@ -534,13 +781,15 @@ constexpr int kSmallPer = (kMaxNr * kMaxNr + 63) / 64;
 constexpr int kSolveWork = 32 * 64 + 32;
 static_assert(kMaxNr * (kMaxNr + 1) <= kSolveWork, "workspace holds the padded factor");
 
-__device__ __forceinline__ void load_nn(const double* __restrict__ src, double* dst, int NN) {
-  double v[kSmallPer];
+__device__ __forceinline__ void gather_nn(const double* __restrict__ src, double (&v)[kSmallPer], int NN) {
 #pragma unroll
   for (int u = 0; u < kSmallPer; u++) {
     const int q = threadIdx.x + 64 * u;
     v[u] = q < NN ? src[q] : 0.0;
   }
+}
+
+__device__ __forceinline__ void scatter_nn(const double (&v)[kSmallPer], double* dst, int NN) {
 #pragma unroll
   for (int u = 0; u < kSmallPer; u++) {
     const int q = threadIdx.x + 64 * u;
@@ -548,9 +797,15 @@ __device__ __forceinline__ void load_nn(const double* __restrict__ src, double* 
   }
 }
 
+__device__ __forceinline__ void load_nn(const double* __restrict__ src, double* dst, int NN) {
+  double v[kSmallPer];
+  gather_nn(src, v, NN);
+  scatter_nn(v, dst, NN);
+}
+
 // M = the symmetric matrix whose lower triangle is partial[0] (makeSymmetric,
 // linop.cpp:241-243; the dots leave their total in the first partial)
-__device__ void reduce_sym(const double* partial, int nblk, int nr, double* M) {
+__device__ __forceinline__ void reduce_sym(const double* partial, int nblk, int nr, double* M) {
   (void)nblk;
   const int NN = nr * nr;
   double v[kSmallPer];
@@ -573,7 +828,7 @@ __device__ void reduce_sym(const double* partial, int nblk, int nr, double* M) {
     const int q = threadIdx.x + 64 * u;
     if (q < NN) M[q] = v[u];
   }
-  __syncthreads();
+  wave_sync();
 }
 
 // X = (M^{-1} Q)^T with M factored as Eigen's unblocked LLT does it
@@ -755,6 +1010,7 @@ __global__ void k_cg_norms(const double* partial, int nblk, int nr, double* smal
     v.flags[0] = 0.0;  // done
     v.flags[1] = 0.0;  // iterations
     v.flags[2] = 0.0;  // failure
+    v.flags[3] = 0.0;  // work-group count of k_cg_reduce_step (as an unsigned int)
   }
 }
 
@@ -776,16 +1032,105 @@ __global__ void k_cg_rtr(const double* partial, int nblk, int nr, double* small)
   reduce_sym(partial, nblk, nr, v.RtR);
 }
 
+// LDS of the small steps
+struct StepLds {
+  double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kSolveWork], Xs[kMaxNr * kMaxNr];
+  int conv;
+};
+
+// A = (PtKP)^{-1} RtR with PtKP from the reduced dot partials, on one wave
+__device__ __forceinline__ void cg_step_a_wave(const double* partial, int nr, double* small, StepLds& sh) {
+  SmallView v(small, nr);
+  double rt[kSmallPer];
+  gather_nn(v.RtR, rt, nr * nr);  // in flight with the partials' loads
+  reduce_sym(partial, 1, nr, sh.M);
+  scatter_nn(rt, sh.Q, nr * nr);
+  load_nn(sh.M, v.PtKP, nr * nr);
+  wave_sync();
+  small_solve(sh.M, sh.Q, sh.Xs, nr, sh.L);
+  load_nn(sh.Xs, v.A, nr * nr);
+}
+
+// convergence test on diag(R R^T) and Psi = (RtR^{-1} RtR2)^T, on one wave
+__device__ __forceinline__ void cg_step_b_wave(const double* partial, int nr, double* small, double tolsq,
+                                               StepLds& sh) {
+  SmallView v(small, nr);
+  double rt[kSmallPer];
+  gather_nn(v.RtR, rt, nr * nr);  // in flight with the partials' loads
+  if (threadIdx.x == 0) sh.conv = 1;
+  reduce_sym(partial, 1, nr, sh.Q);  // RtR2
+  if ((int)threadIdx.x < nr && !(sh.Q[threadIdx.x * (nr + 1)] < tolsq)) atomicAnd(&sh.conv, 0);
+  wave_sync();
+  if (sh.conv) {
+    if (threadIdx.x == 0) v.flags[0] = 1.0;
+    return;
+  }
+  scatter_nn(rt, sh.M, nr * nr);
+  wave_sync();
+  small_solve(sh.M, sh.Q, sh.Xs, nr, sh.L);
+  load_nn(sh.Xs, v.Psi, nr * nr);
+  load_nn(sh.Q, v.RtR, nr * nr);  // R R' = R2 R2' (linop.h:329)
+  if (threadIdx.x == 0) v.flags[1] += 1.0;
+}
+
 __global__ void __launch_bounds__(64) k_cg_step_a(const double* partial, int nblk, int nr, double* small) {
+  (void)nblk;
   SmallView v(small, nr);
   if (v.flags[0] != 0.0) return;
-  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kSolveWork], Xs[kMaxNr * kMaxNr];
-  reduce_sym(partial, nblk, nr, M);
-  load_nn(v.RtR, Q, nr * nr);
-  load_nn(M, v.PtKP, nr * nr);
+  __shared__ StepLds sh;
+  cg_step_a_wave(partial, nr, small, sh);
+}
+
+// The dot partials' sum in a fixed order (16 interleaved strands of blocks,
+// combined in strand order; a work-group owns 16 of the nr x nr entries),
+// then, in the work-group that finishes last, the small step on its first
+// wave (STEP 0: A as k_cg_step_a, 1: convergence and Psi as k_cg_step_b):
+// one launch instead of two.  The work-groups count themselves on the
+// spare flag (reset by k_cg_norms); the last one resets it.  Its release
+// fence writes back only the few partial sums this kernel wrote.
+template <int STEP>
+__global__ void __launch_bounds__(256)
+    k_cg_reduce_step(double* partial, int nblk, int nr, double* small, double tolsq) {
+  constexpr int kStrands = 16, kEntries = 256 / kStrands;
+  SmallView v(small, nr);
+  if (v.flags[0] != 0.0) return;  // converged: the same on every work-group
+  const int width = nr * nr;
+  __shared__ double red[kStrands][kEntries];
+  __shared__ StepLds sh;
+  __shared__ unsigned last;
+  const int e = threadIdx.x % kEntries, g = threadIdx.x / kEntries;
+  const int q = blockIdx.x * kEntries + e;
+  double acc = 0.0;
+  if (q < width) {
+    int b = g;
+    for (; b + 7 * kStrands < nblk; b += 8 * kStrands) {
+      double w[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) w[u] = partial[(size_t)(b + kStrands * u) * width + q];
+#pragma unroll
+      for (int u = 0; u < 8; u++) acc += w[u];
+    }
+    for (; b < nblk; b += kStrands) acc += partial[(size_t)b * width + q];
+  }
+  red[g][e] = acc;
   __syncthreads();
-  small_solve(M, Q, Xs, nr, L);
-  load_nn(Xs, v.A, nr * nr);
+  if (g == 0 && q < width) {  // the first 16 lanes of wave 0
+    double t = red[0][e];
+#pragma unroll
+    for (int h = 1; h < kStrands; h++) t += red[h][e];
+    partial[q] = t;
+  }
+  unsigned* ticket = reinterpret_cast<unsigned*>(v.flags + 3);
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // after wave 0's stores of partial[q]
+    last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last || threadIdx.x >= 64) return;  // the rest of the work-group ends
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (STEP == 0) cg_step_a_wave(partial, nr, small, sh);
+  else cg_step_b_wave(partial, nr, small, tolsq, sh);
 }
 
 template <int MODE>  // 0: X += A P, R -= A KP;  1: P = R + Psi P
@@ -840,18 +1185,6 @@ __global__ void __launch_bounds__(256)
 // components: every global access is a 16-byte vector.  No LDS.  The sums
 // over k run in the matrix core's order, so the result differs from the
 // kernel above in the last bits.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ dbl4 ld4(const double* p) {
-  const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
-  return dbl4{a.x, a.y, b.x, b.y};
-}
-
-__device__ __forceinline__ void st4(double* p, dbl4 v) {
-  reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
-  reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
-}
-
 template <int MODE>  // 0: X += A P, R -= A KP;  1: P = R + Psi P
 __global__ void __launch_bounds__(256)
     k_cg_update32(int n, const double* __restrict__ small, double* __restrict__ X, double* __restrict__ R,
@@ -870,15 +1203,25 @@ __global__ void __launch_bounds__(256)
   for (int t = w; t < ntile; t += nw) {
     const int f = 16 * t + j;
     const bool ok = f < n;
+    // every operand of the tile is loaded before the first MFMA, with no
+    // branch around the loads (a lane past n reads feature 0: its column of
+    // the product is its own and is not stored)
     const size_t o = (size_t)(ok ? f : 0) * 32;
-    auto prod = [&](const double* __restrict__ in, dbl4& d0, dbl4& d1) {
-      double b[8];
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const dbl4 v = ok ? ld4(in + o + 8 * q + 4 * h) : dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int u = 0; u < 4; u++) b[4 * h + u] = v[u];
-      }
+    const dbl4 p0 = ld4(P + o + 8 * q), p1 = ld4(P + o + 8 * q + 4);
+    dbl4 k0 = {0.0, 0.0, 0.0, 0.0}, k1 = k0, x0 = k0, x1 = k0;
+    if (MODE == 0) {
+      k0 = ld4(KPm + o + 8 * q);
+      k1 = ld4(KPm + o + 8 * q + 4);
+      x0 = ld4(X + o + 4 * q);
+      x1 = ld4(X + o + 16 + 4 * q);
+    }
+    const dbl4 r0 = ld4(R + o + 4 * q), r1 = ld4(R + o + 16 + 4 * q);
+    // one wait for all of them: keeps the compiler from sinking the D-layout
+    // loads into the store branch (a second round trip per tile)
+    asm volatile("" ::"v"(p0), "v"(p1), "v"(r0), "v"(r1));
+    if (MODE == 0) asm volatile("" ::"v"(k0), "v"(k1), "v"(x0), "v"(x1));
+    auto prod = [&](dbl4 v0, dbl4 v1, dbl4& d0, dbl4& d1) {
+      const double b[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
       d0 = dbl4{0.0, 0.0, 0.0, 0.0};
       d1 = d0;
 #pragma unroll
@@ -888,19 +1231,19 @@ __global__ void __launch_bounds__(256)
       }
     };
     dbl4 d0, d1;
-    prod(P, d0, d1);
+    prod(p0, p1, d0, d1);
     if (MODE == 0) {
       dbl4 e0, e1;
-      prod(KPm, e0, e1);
+      prod(k0, k1, e0, e1);
       if (ok) {
-        st4(X + o + 4 * q, ld4(X + o + 4 * q) + d0);
-        st4(X + o + 16 + 4 * q, ld4(X + o + 16 + 4 * q) + d1);
-        st4(R + o + 4 * q, ld4(R + o + 4 * q) - e0);
-        st4(R + o + 16 + 4 * q, ld4(R + o + 16 + 4 * q) - e1);
+        st4(X + o + 4 * q, x0 + d0);
+        st4(X + o + 16 + 4 * q, x1 + d1);
+        st4(R + o + 4 * q, r0 - e0);
+        st4(R + o + 16 + 4 * q, r1 - e1);
       }
     } else if (ok) {
-      st4(P + o + 4 * q, ld4(R + o + 4 * q) + d0);
-      st4(P + o + 16 + 4 * q, ld4(R + o + 16 + 4 * q) + d1);
+      st4(P + o + 4 * q, r0 + d0);
+      st4(P + o + 16 + 4 * q, r1 + d1);
     }
   }
 }
@@ -947,6 +1290,10 @@ __global__ void __launch_bounds__(256)
     const dbl4 k0 = ld4(KPm + o + 8 * q), k1 = ld4(KPm + o + 8 * q + 4);
     const dbl4 x0 = ld4(X + o + 4 * q), x1 = ld4(X + o + 16 + 4 * q);
     const dbl4 r0 = ld4(R + o + 4 * q), r1 = ld4(R + o + 16 + 4 * q);
+    // one wait for all eight loads: keeps the compiler from sinking the X and
+    // R loads into the store branch (a second round trip per tile)
+    asm volatile("" ::"v"(p0), "v"(p1), "v"(k0), "v"(k1));
+    asm volatile("" ::"v"(x0), "v"(x1), "v"(r0), "v"(r1));
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       bp[u] = p0[u];
@@ -998,28 +1345,13 @@ __global__ void __launch_bounds__(256)
   for (int e = threadIdx.x; e < NN; e += 256) out[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
 }
 
-// convergence test on diag(R R^T) and Psi = (RtR^{-1} RtR2)^T
 __global__ void __launch_bounds__(64) k_cg_step_b(const double* partial, int nblk, int nr, double* small,
                                                    double tolsq) {
+  (void)nblk;
   SmallView v(small, nr);
   if (v.flags[0] != 0.0) return;
-  __shared__ double M[kMaxNr * kMaxNr], Q[kMaxNr * kMaxNr], L[kSolveWork], Xs[kMaxNr * kMaxNr];
-  __shared__ int conv;
-  if (threadIdx.x == 0) conv = 1;
-  reduce_sym(partial, nblk, nr, Q);  // RtR2
-  for (int i = threadIdx.x; i < nr; i += blockDim.x)
-    if (!(Q[i + i * nr] < tolsq)) atomicAnd(&conv, 0);
-  __syncthreads();
-  if (conv) {
-    if (threadIdx.x == 0) v.flags[0] = 1.0;
-    return;
-  }
-  load_nn(v.RtR, M, nr * nr);
-  __syncthreads();
-  small_solve(M, Q, Xs, nr, L);
-  load_nn(Xs, v.Psi, nr * nr);
-  load_nn(Q, v.RtR, nr * nr);  // R R' = R2 R2' (linop.h:329)
-  if (threadIdx.x == 0) v.flags[1] += 1.0;
+  __shared__ StepLds sh;
+  cg_step_b_wave(partial, nr, small, tolsq, sh);
 }
 
 __global__ void k_cg_finish(int nr, int n, const double* small, double* X) {
@@ -1049,9 +1381,21 @@ void launch_cg_init(const CgState& c, const double* B, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_rtr, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
 }
 
+// more than one partial: the sum and the small step are one launch
+static bool fold_steps(const CgState& c) { return c.nblk > 1; }
+
+static void launch_reduce_step(const CgState& c, int step, double tolsq, hipStream_t s) {
+  const dim3 grid((c.nr * c.nr + 15) / 16);
+  if (step == 0)
+    hipLaunchKernelGGL(k_cg_reduce_step<0>, grid, dim3(256), 0, s, c.partial, c.nblk, c.nr, c.small, tolsq);
+  else
+    hipLaunchKernelGGL(k_cg_reduce_step<1>, grid, dim3(256), 0, s, c.partial, c.nblk, c.nr, c.small, tolsq);
+}
+
 void launch_cg_step_a(const CgState& c, hipStream_t s) {
-  launch_dots(c.P, c.KPm, c.nr, c.n, c.partial, c.nblk, s, done_flag(c));
-  hipLaunchKernelGGL(k_cg_step_a, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
+  launch_dots(c.P, c.KPm, c.nr, c.n, c.partial, c.nblk, s, done_flag(c), !fold_steps(c));
+  if (fold_steps(c)) launch_reduce_step(c, 0, 0.0, s);
+  else hipLaunchKernelGGL(k_cg_step_a, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
 }
 
 // nr = 32: the update also leaves the reduced partial sums of R^T R, which
@@ -1063,7 +1407,7 @@ void launch_cg_update_xr(const CgState& c, hipStream_t s) {
     const int cols = (c.n + c.nblk - 1) / c.nblk;
     hipLaunchKernelGGL(k_cg_update32_xr, dim3(c.nblk), dim3(256), 0, s, c.n, cols, (const double*)c.small, c.X, c.R,
                        (const double*)c.P, (const double*)c.KPm, c.partial);
-    if (c.nblk > 1) launch_partial_reduce(c.partial, c.nblk, 32 * 32, s);
+    if (c.nblk > 1 && !fold_steps(c)) launch_partial_reduce(c.partial, c.nblk, 32 * 32, s);
     return;
   }
   hipLaunchKernelGGL((k_cg_update<0>), dim3(grid_for(c.n, 8)), dim3(256), 0, s, c.nr, c.n, (const double*)c.small,
@@ -1071,8 +1415,9 @@ void launch_cg_update_xr(const CgState& c, hipStream_t s) {
 }
 
 void launch_cg_step_b(const CgState& c, double tolsq, hipStream_t s) {
-  if (!fused_xr(c)) launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s, done_flag(c));
-  hipLaunchKernelGGL(k_cg_step_b, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small, tolsq);
+  if (!fused_xr(c)) launch_dots(c.R, c.R, c.nr, c.n, c.partial, c.nblk, s, done_flag(c), !fold_steps(c));
+  if (fold_steps(c)) launch_reduce_step(c, 1, tolsq, s);
+  else hipLaunchKernelGGL(k_cg_step_b, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small, tolsq);
 }
 
 void launch_cg_update_p(const CgState& c, hipStream_t s) {

@@ -945,25 +945,39 @@ CsrDev MacauPrior::Ft() const {
 // nnz-balanced SpMM work list of a CSR matrix (see CsrDev): rows longer
 // than kSpmmSegNnz are cut into segments whose partial sums are added in
 // segment order; those segments come first so the long work starts early
-static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBuf>& store, hipStream_t s) {
+// Rows listed in dense_rows (ascending) are left to the dense-block kernel:
+// they get no segment and are appended to the split rows with nkc slots
+// each, after all other slots (the dense kernel writes them).
+static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBuf>& store, hipStream_t s,
+                        const std::vector<int>* dense_rows = nullptr, int nkc = 0) {
   const int nrow = (int)rp.size() - 1;
+  std::vector<char> is_dense(nrow, 0);
+  if (dense_rows)
+    for (int r : *dense_rows) is_dense[r] = 1;
   std::vector<int4> seg;
   std::vector<int> mrow, mslot0;
   int slots = 0;
   for (int r = 0; r < nrow; r++) {
-    if (rp[r + 1] - rp[r] <= kSpmmSegNnz) continue;
+    if (is_dense[r] || rp[r + 1] - rp[r] <= kSpmmSegNnz) continue;
     mrow.push_back(r);
     mslot0.push_back(slots);
     for (int b = rp[r]; b < rp[r + 1]; b += kSpmmSegNnz)
       seg.push_back(make_int4(r, b, std::min(b + kSpmmSegNnz, rp[r + 1]), slots++));
   }
+  const int dslot0 = slots;
+  if (dense_rows)
+    for (int r : *dense_rows) {
+      mrow.push_back(r);
+      mslot0.push_back(slots);
+      slots += nkc;
+    }
   mslot0.push_back(slots);
   const int nsplit = (int)seg.size();
   // whole rows by decreasing length (ties by row): the long ones form a
   // prefix after the split rows' segments
   std::vector<int> whole;
   for (int r = 0; r < nrow; r++)
-    if (rp[r + 1] - rp[r] <= kSpmmSegNnz) whole.push_back(r);
+    if (!is_dense[r] && rp[r + 1] - rp[r] <= kSpmmSegNnz) whole.push_back(r);
   std::stable_sort(whole.begin(), whole.end(),
                    [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
   int nlong = nsplit, nempty = 0;
@@ -987,6 +1001,113 @@ static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBu
   c.multi_row = store[1].as<int>();
   c.multi_slot0 = store[2].as<int>();
   c.partial = store[3].as<double>();
+  c.dslot0 = dslot0;
+  c.nkc = nkc;
+  return c;
+}
+
+// The dense column block of a binary F (or one whose values are all 1): columns with a nonzero in at least
+// every 64th row and more than kSpmmSegNnz nonzeros, the densest
+// kDenseMaxCols of them, when there are at least 16; none when one of their
+// entries is repeated (a bit cannot count it twice).
+void MacauPrior::build_dense(int nrmax, hipStream_t s) {
+  Fd_plan = CsrDev();
+  Ftd_plan = CsrDev();
+  dense_bufs.clear();
+  Ftd_plan_bufs.clear();
+  // values all 1 multiply like a binary F
+  const bool ones = binary || std::all_of(F_vals.begin(), F_vals.end(), [](double v) { return v == 1.0; });
+  if (!ones || nrmax < 32) return;
+  std::vector<int> cand;
+  for (int j = 0; j < nfeat; j++) {
+    const long c = Ft_rowptr[j + 1] - Ft_rowptr[j];
+    if (c * 64 >= nrow && c > kSpmmSegNnz) cand.push_back(j);
+  }
+  if (cand.size() < 16) return;
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+    return Ft_rowptr[a + 1] - Ft_rowptr[a] > Ft_rowptr[b + 1] - Ft_rowptr[b];
+  });
+  if ((int)cand.size() > kDenseMaxCols) cand.resize(kDenseMaxCols);
+  std::sort(cand.begin(), cand.end());
+  const int nd = (int)cand.size(), ndp = (nd + 15) / 16 * 16;
+  const int ndw = (ndp + 31) / 32, nrw = (nrow + 31) / 32;
+  std::vector<int> pos(nfeat, -1);
+  for (int j = 0; j < nd; j++) pos[cand[j]] = j;
+  std::vector<int> dcol(ndp, cand[0]);
+  std::copy(cand.begin(), cand.end(), dcol.begin());
+  std::vector<uint32_t> rbits((size_t)nrow * ndw, 0u), cbits((size_t)ndp * nrw, 0u);
+  std::vector<int> rptr(nrow + 1, 0), rcols;
+  for (int r = 0; r < nrow; r++) {
+    for (int e = F_rowptr[r]; e < F_rowptr[r + 1]; e++) {
+      const int j = pos[F_cols[e]];
+      if (j < 0) {
+        rcols.push_back(F_cols[e]);
+        continue;
+      }
+      uint32_t& w = rbits[(size_t)r * ndw + j / 32];
+      if (w & (1u << (j % 32))) return;  // repeated entry
+      w |= 1u << (j % 32);
+      cbits[(size_t)j * nrw + r / 32] |= 1u << (r % 32);
+    }
+    rptr[r + 1] = (int)rcols.size();
+  }
+  std::vector<int> rfix((size_t)nrow * kDenseRem, 0);
+  for (int r = 0; r < nrow; r++)
+    for (int t = 0; t < kDenseRem && rptr[r] + t < rptr[r + 1]; t++) rfix[(size_t)r * kDenseRem + t] = rcols[rptr[r] + t];
+  if (rcols.empty()) rcols.push_back(0);
+  dense_bufs.resize(6);
+  upload(dense_bufs[5], rfix, s);
+  upload(dense_bufs[0], dcol, s);
+  upload(dense_bufs[1], rbits, s);
+  upload(dense_bufs[2], rptr, s);
+  upload(dense_bufs[3], rcols, s);
+  upload(dense_bufs[4], cbits, s);
+  CsrDev d;
+  d.ndp = ndp;
+  d.nd = nd;
+  d.nfrow = nrow;
+  d.ndw = ndw;
+  d.nrw = nrw;
+  d.dcol = dense_bufs[0].as<int>();
+  Fd_plan = d;
+  Fd_plan.dense_role = 1;
+  Fd_plan.row_bits = dense_bufs[1].as<uint32_t>();
+  Fd_plan.rem_ptr = dense_bufs[2].as<int>();
+  Fd_plan.rem_cols = dense_bufs[3].as<int>();
+  Fd_plan.rem_fix = dense_bufs[5].as<int>();
+  const int nkc = (nrow + kDenseRowChunk - 1) / kDenseRowChunk;
+  Ftd_plan = spmm_plan(Ft_rowptr, nrmax, Ftd_plan_bufs, s, &cand, nkc);
+  Ftd_plan.dense_role = 2;
+  Ftd_plan.ndp = ndp;
+  Ftd_plan.nd = nd;
+  Ftd_plan.nfrow = nrow;
+  Ftd_plan.ndw = ndw;
+  Ftd_plan.nrw = nrw;
+  Ftd_plan.dcol = d.dcol;
+  Ftd_plan.col_bits = dense_bufs[4].as<uint32_t>();
+}
+
+CsrDev MacauPrior::Fd() const {
+  if (!Fd_plan.dense_role) return CsrDev();
+  CsrDev c = Fd_plan;
+  const CsrDev base = F();
+  c.nrow = base.nrow;
+  c.ncol = base.ncol;
+  c.row_ptr = base.row_ptr;
+  c.cols = base.cols;
+  c.vals = nullptr;
+  return c;
+}
+
+CsrDev MacauPrior::Ftd() const {
+  if (!Ftd_plan.dense_role) return CsrDev();
+  CsrDev c = Ftd_plan;
+  const CsrDev base = Ft();
+  c.nrow = base.nrow;
+  c.ncol = base.ncol;
+  c.row_ptr = base.row_ptr;
+  c.cols = base.cols;
+  c.vals = nullptr;
   return c;
 }
 
@@ -999,6 +1120,7 @@ void MacauPrior::upload_side(int nrmax, hipStream_t s) {
   upload(dFt_vals, Ft_vals, s);
   F_plan = spmm_plan(F_rowptr, nrmax, F_plan_bufs, s);
   Ft_plan = spmm_plan(Ft_rowptr, nrmax, Ft_plan_bufs, s);
+  build_dense(nrmax, s);
 }
 
 void MacauPrior::init(Model& m, int mode) {
@@ -1065,8 +1187,13 @@ CgBuffers::~CgBuffers() {
     if (c.exec) (void)hipGraphExecDestroy(c.exec);
 }
 
-int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
-                   double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph) {
+int blockcg_device(const CsrDev& F0, const CsrDev& Ft0, const double* reg, const double* B, int ld, int r0, int nr,
+                   double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph, const CsrDev* Fd,
+                   const CsrDev* Ftd) {
+  // 32 right-hand sides: the dense-block products when F has them
+  const bool dense = nr == 32 && Fd && Ftd && Fd->dense_role == 1 && Ftd->dense_role == 2;
+  const CsrDev& F = dense ? *Fd : F0;
+  const CsrDev& Ft = dense ? *Ftd : Ft0;
   if (nr > b.nrmax || nr > 40) throw std::runtime_error("BlockCG only available for up to 40 RHSs.");
   const int nfeat = F.ncol;
   CgState c;
@@ -1110,9 +1237,11 @@ int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const d
     std::memcpy(&tb, &tolsq_bits, sizeof(tb));
     key = {(uintptr_t)reg, (uintptr_t)c.X, (uintptr_t)c.small, (uintptr_t)nr, tb};
     for (const CsrDev* A : {&F, &Ft}) {
-      const uintptr_t f[] = {(uintptr_t)A->nrow, (uintptr_t)A->ncol, (uintptr_t)A->row_ptr, (uintptr_t)A->cols,
-                             (uintptr_t)A->vals, (uintptr_t)A->nseg, (uintptr_t)A->seg, (uintptr_t)A->nmulti,
-                             (uintptr_t)A->multi_row, (uintptr_t)A->multi_slot0, (uintptr_t)A->partial};
+      const uintptr_t f[] = {(uintptr_t)A->nrow,       (uintptr_t)A->ncol,      (uintptr_t)A->row_ptr,
+                             (uintptr_t)A->cols,       (uintptr_t)A->vals,      (uintptr_t)A->nseg,
+                             (uintptr_t)A->seg,        (uintptr_t)A->nmulti,    (uintptr_t)A->multi_row,
+                             (uintptr_t)A->multi_slot0, (uintptr_t)A->partial,  (uintptr_t)A->dense_role,
+                             (uintptr_t)A->dcol,       (uintptr_t)A->row_bits,  (uintptr_t)A->col_bits};
       key.insert(key.end(), std::begin(f), std::end(f));
     }
   }
@@ -1195,15 +1324,16 @@ void MacauPrior::update_prior(Model& m, int mode) {  // latentprior.cpp:254-264
   } else {
     // solve_blockcg(beta, F, lambda_beta, Ft_y, tol, 32, 8) (linop.h:206-226)
     const int blocksize = 32, excess = 8;
+    const CsrDev fd = Fd(), ftd = Ftd();
     if (K <= excess + blocksize) {
       last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, 0, K, tol,
-                                     beta.as<double>(), cg, s, m.use_graph);
+                                     beta.as<double>(), cg, s, m.use_graph, &fd, &ftd);
     } else {
       for (int i = 0; i < K; i += blocksize) {
         int nr = blocksize;
         if (i + blocksize + excess >= K) nr = K - i;
         last_cg_iters = blockcg_device(F(), Ft(), lambda_beta_d.as<double>(), Fty.as<double>(), K, i, nr, tol,
-                                       beta.as<double>(), cg, s, m.use_graph);
+                                       beta.as<double>(), cg, s, m.use_graph, &fd, &ftd);
       }
     }
   }

@@ -83,7 +83,8 @@ struct CgBuffers {
 // chunks of CG iterations launched per host check of the convergence flags
 // are captured once per length and replayed as HIP graphs.
 int blockcg_device(const CsrDev& F, const CsrDev& Ft, const double* reg, const double* B, int ld, int r0, int nr,
-                   double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph = false);
+                   double tol, double* X, CgBuffers& b, hipStream_t s, bool use_graph = false,
+                   const CsrDev* Fd = nullptr, const CsrDev* Ftd = nullptr);
 
 // Work list of the batched sampler for one mode (rows sorted by nnz,
 // heavy rows split into segments).
@@ -159,6 +160,14 @@ class MacauPrior : public ILatentPrior {
   void upload_side(int nrmax, hipStream_t s);
   CsrDev F_plan, Ft_plan;
   std::vector<DevBuf> F_plan_bufs, Ft_plan_bufs;
+  // the same products with a dense column block (binary F with at least 16
+  // columns holding a nonzero in every 64th row or more; 32 right-hand
+  // sides): dense_role 0 when there is none
+  CsrDev Fd() const;
+  CsrDev Ftd() const;
+  void build_dense(int nrmax, hipStream_t s);
+  CsrDev Fd_plan, Ftd_plan;
+  std::vector<DevBuf> dense_bufs, Ftd_plan_bufs;
 };
 
 // MacauOnePrior<SparseDoubleFeat> (macauoneprior.h:8-51): diagonal prior

@@ -154,6 +154,36 @@ def test_blockcg_vs_oracle(nrhs, binary):
     assert abs(it - itr) <= 2
 
 
+@pytest.mark.parametrize("ones", [False, True])
+def test_blockcg_dense_block_vs_oracle(ones):
+    """32 right-hand sides over a binary F with 40 dense columns (a third of
+    the rows each) and 0-20 sparse nonzeros per row: the dense-block SpMM
+    products, with rows whose remaining nonzeros take one, two or three
+    batches; nrow is not a multiple of the 32-row tiles or the 256-row
+    chunks.  ones: the same F given as values 1.0."""
+    announce("blockcg dense block ones=%s" % ones)
+    rng = np.random.default_rng(77)
+    nrow, ncol, nd = 1500, 3000, 40
+    dense = rng.choice(ncol, nd, replace=False)
+    rows, cols = [], []
+    for r in range(nrow):
+        d = dense[rng.random(nd) < 0.33]
+        k = rng.integers(0, 21)
+        sp = rng.choice(np.setdiff1d(np.arange(ncol), dense)[:1200], k, replace=False)
+        for c in np.concatenate([d, sp]):
+            rows.append(r)
+            cols.append(int(c))
+    r = np.array(rows, dtype=np.int32)
+    c = np.array(cols, dtype=np.int32)
+    v = np.ones(len(r)) if ones else None
+    B = rng.standard_normal((32, ncol))
+    X, it = gpu_blockcg(r, c, v, nrow, ncol, 5.0, B, 1e-6)
+    F, Ft = O.csr_pair(r, c, np.ones(len(r)), nrow, ncol)
+    Xr, itr = O.blockcg(F, Ft, 5.0, B.T, 1e-6)
+    assert rel_err(X, Xr.T) < 1e-6
+    assert abs(it - itr) <= 2
+
+
 # ------------------------------------------------------------ model level
 def _bpmf_problem(seed, nrow, ncol, K, density, noise):
     rng = np.random.default_rng(seed)
