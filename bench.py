@@ -482,6 +482,7 @@ def bench_c5(args, log):
         "instance_iterations_per_sec": len(ws) / sec,
         "row_updates_per_sec": rows / sec,
         "launch": mode,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "graph_ms_per_step": sec_graph * 1e3,
         "streams_ms_per_step": sec_streams * 1e3,
         "one_by_one_ms_per_step": seq * 1e3,
@@ -521,6 +522,11 @@ def main():
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
     if args.workload == "c5":
+        # 64 models with a graph and streams each: with 16 hardware queues per
+        # process (HIP's default is 4) they serialise least (batch it/s with
+        # 4, 8, 12, 16, 32 queues: 24.7, 23.8, 24.8, 27.2, 12.8); read when the
+        # library first touches the GPU, so set before that
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         return bench_c5(args, log)
     t0 = time.perf_counter()
     w = load_workload(args.workload, args.seed)
