@@ -1100,18 +1100,20 @@ __global__ void __launch_bounds__(256)
   __shared__ unsigned last;
   const int e = threadIdx.x % kEntries, g = threadIdx.x / kEntries;
   const int q = blockIdx.x * kEntries + e;
+  // a strand's (at most 512 / 16) partials loaded at once, at clamped
+  // addresses, and added in block order
+  constexpr int kPer = 512 / kStrands;
   double acc = 0.0;
-  if (q < width) {
-    int b = g;
-    for (; b + 7 * kStrands < nblk; b += 8 * kStrands) {
-      double w[8];
+  {
+    const int qc = min(q, width - 1);
+    double w[kPer];
 #pragma unroll
-      for (int u = 0; u < 8; u++) w[u] = partial[(size_t)(b + kStrands * u) * width + q];
+    for (int u = 0; u < kPer; u++) w[u] = partial[(size_t)min(g + kStrands * u, nblk - 1) * width + qc];
 #pragma unroll
-      for (int u = 0; u < 8; u++) acc += w[u];
-    }
-    for (; b < nblk; b += kStrands) acc += partial[(size_t)b * width + q];
+    for (int u = 0; u < kPer; u++)
+      if (g + kStrands * u < nblk) acc += w[u];
   }
+  if (q >= width) acc = 0.0;
   red[g][e] = acc;
   __syncthreads();
   if (g == 0 && q < width) {  // the first 16 lanes of wave 0
@@ -1385,6 +1387,7 @@ void launch_cg_init(const CgState& c, const double* B, hipStream_t s) {
 static bool fold_steps(const CgState& c) { return c.nblk > 1; }
 
 static void launch_reduce_step(const CgState& c, int step, double tolsq, hipStream_t s) {
+  if (c.nblk > 512) throw std::logic_error("k_cg_reduce_step sums at most 512 partials");
   const dim3 grid((c.nr * c.nr + 15) / 16);
   if (step == 0)
     hipLaunchKernelGGL(k_cg_reduce_step<0>, grid, dim3(256), 0, s, c.partial, c.nblk, c.nr, c.small, tolsq);
