@@ -1216,10 +1216,10 @@ int blockcg_device(const CsrDev& F0, const CsrDev& Ft0, const double* reg, const
   const double tolsq = tol * tol;
   double flags[3] = {0, 0, 0};
   const double* dflags = c.small + 5 * nr * nr + nr;
-  // a chunk of iterations per host round trip: the first as long as the
-  // previous solve needed, less 2 and rounded down to 4 (8 without one), then
-  // 4 at a time; kernels of iterations after convergence return
-  // immediately (device-side done flag)
+  // a chunk of iterations per host round trip: the first one iteration
+  // shorter than the previous solve needed (8 without one), then 2 at a
+  // time; kernels of iterations after convergence return immediately
+  // (device-side done flag)
   auto launch_chunk = [&](hipStream_t st, int len) {
     for (int q = 0; q < len; q++) {
       launch_spmm(F, c.P, nr, c.inner, nullptr, nullptr, st, dflags);
@@ -1261,8 +1261,8 @@ int blockcg_device(const CsrDev& F0, const CsrDev& Ft0, const double* reg, const
     b.chunks.push_back({k, exec});
     return exec;
   };
-  int len = b.last_iters >= 8 ? std::max(4, (b.last_iters - 2) / 4 * 4) : 8;
-  for (int it = 0; it < 100000; it += len, len = 4) {
+  int len = b.last_iters >= 8 ? b.last_iters - 1 : 8;
+  for (int it = 0; it < 100000; it += len, len = 2) {
     if (use_graph)
       HIPCHK(hipGraphLaunch(chunk_exec(len), s));
     else
