@@ -291,7 +291,12 @@ struct CsrDev {
   const uint32_t* col_bits = nullptr;  // role 2: bit r of dense column j at word j * nrw + r / 32
   int nkc = 0;                // role 2: chunks of kDenseRowChunk rows of F per dense column
   int dslot0 = 0;             // role 2: partial slot of (column j, chunk k) = dslot0 + j * nkc + k
+  // role 2: the first row at or after r with a nonzero (nrow past the last);
+  // rows without nonzeros and none within kEmptyReach rows either way get
+  // no work unit: their output (reg * add) is formed where it is read
+  const int* next_nonempty = nullptr;
 };
+constexpr int kEmptyReach = 31;
 constexpr int kDenseMaxCols = 288;
 constexpr int kDenseRowChunk = 256;
 constexpr int kDenseRem = 8;
@@ -344,7 +349,8 @@ void launch_sumsq_f64(const double* x, size_t n, double* partial, int nblk, hipS
 // when the launch completes (nblk = dots_blocks(n))
 int dots_blocks(int n);
 void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
-                 const double* skip = nullptr, bool reduce = true);
+                 const double* skip = nullptr, bool reduce = true, const int* nxt = nullptr,
+                 const double* reg = nullptr);
 
 struct CgState {
   int nr = 0, n = 0;
@@ -356,6 +362,11 @@ struct CgState {
   double* partial = nullptr;
   int nblk = 0;
   double* small = nullptr;  // RtR, RtR2, PtKP, A, Psi, norms, flags
+  // nr = 32 with a dense-block plan: the next_nonempty list of F^T and the
+  // regularisation; a group of at most 32 features with no nonzero in F^T
+  // takes KP = reg * P where it is read (KP is not written there)
+  const int* nxt = nullptr;
+  const double* reg = nullptr;
 };
 void launch_cg_init(const CgState& c, const double* B, hipStream_t s);
 void launch_cg_step_a(const CgState& c, hipStream_t s);   // A from PtKP partials

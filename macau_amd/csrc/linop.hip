@@ -680,7 +680,8 @@ __global__ void __launch_bounds__(DotShape<NRP>::THREADS)
 // waves' sums are added in wave order through LDS.
 __global__ void __launch_bounds__(256)
     k_dots32(const double* __restrict__ a, const double* __restrict__ b, int n, int cols,
-             double* __restrict__ partial, const double* skip) {
+             double* __restrict__ partial, const double* skip, const int* __restrict__ nxt,
+             const double* __restrict__ reg) {
   if (skip && *skip != 0.0) return;
   __shared__ double red[4][32 * 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
@@ -694,14 +695,29 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int v = 0; v < 2; v++) d[u][v] = dbl4v{0.0, 0.0, 0.0, 0.0};
   // 32 features per trip: the 16 loads of a lane are issued together
-  // (clamped addresses, zeros selected past the end: no branch)
+  // (clamped addresses, zeros selected past the end: no branch).  With nxt,
+  // a trip whose features have no nonzero in F^T takes b = reg * a (b is
+  // K P there, not written), and nxt for the next trip loads ahead.
+  const double lam = nxt ? *reg : 0.0;
+  int e = nxt ? nxt[min(c0, n)] : 0;
   for (int f0 = c0; f0 < c1; f0 += 32) {
+    const bool emp = nxt && e >= min(f0 + 32, c1);  // the same on every lane
+    if (nxt) e = nxt[min(f0 + 32, n)];
     double2 xs[8], ys[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int fc = min(f0 + 4 * u + q, c1 - 1);
       xs[u] = reinterpret_cast<const double2*>(a + (size_t)fc * 32)[j];
-      ys[u] = reinterpret_cast<const double2*>(b + (size_t)fc * 32)[j];
+    }
+    if (emp) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) ys[u] = make_double2(lam * xs[u].x, lam * xs[u].y);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int fc = min(f0 + 4 * u + q, c1 - 1);
+        ys[u] = reinterpret_cast<const double2*>(b + (size_t)fc * 32)[j];
+      }
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -727,13 +743,14 @@ __global__ void __launch_bounds__(256)
 }
 
 void launch_dots(const double* a, const double* b, int nr, int n, double* partial, int nblk, hipStream_t s,
-                 const double* skip, bool reduce) {
+                 const double* skip, bool reduce, const int* nxt, const double* reg) {
   const int cols = (n + nblk - 1) / nblk;
   if (nr == 32 && (((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
-    hipLaunchKernelGGL(k_dots32, dim3(nblk), dim3(256), 0, s, a, b, n, cols, partial, skip);
+    hipLaunchKernelGGL(k_dots32, dim3(nblk), dim3(256), 0, s, a, b, n, cols, partial, skip, nxt, reg);
     if (reduce && nblk > 1) launch_partial_reduce(partial, nblk, nr * nr, s);
     return;
   }
+  if (nxt) throw std::logic_error("launch_dots: the reg * a shortcut needs nr = 32 and aligned vectors");
 #define MGPU_DOTS(P)                                                                                        \
   hipLaunchKernelGGL((k_dots<P>), dim3(nblk), dim3(DotShape<P>::THREADS), 0, s, a, b, nr, n, cols, partial, \
                      skip)
@@ -1262,7 +1279,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_cg_update32_xr(int n, int cols, const double* __restrict__ small, double* __restrict__ X,
                      double* __restrict__ R, const double* __restrict__ P, const double* __restrict__ KPm,
-                     double* __restrict__ partial) {
+                     double* __restrict__ partial, const int* __restrict__ nxt, const double* __restrict__ reg) {
   constexpr int NN = 32 * 32, LD = 34;
   if (small[5 * NN + 32] != 0.0) return;  // converged or failed earlier
   __shared__ double tile[4][16 * LD];
@@ -1283,13 +1300,26 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int v = 0; v < 2; v++) dd[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
   double* T = tile[wv];
+  // with nxt, a tile whose features have no nonzero in F^T takes
+  // KP = reg * P (not written there); nxt for the next tile loads ahead
+  const double lam = nxt ? *reg : 0.0;
+  int e = nxt ? nxt[min(c0, n)] : 0;
   for (int f0 = c0; f0 < c1; f0 += 16) {
     const int f = f0 + j;
     const bool ok = f < c1;
+    const bool emp = nxt && e >= min(f0 + 16, c1);  // the same on every lane
+    if (nxt) e = nxt[min(f0 + 16, n)];
     const size_t o = (size_t)(ok ? f : c0) * 32;
     double bp[8], bk[8];
     const dbl4 p0 = ld4(P + o + 8 * q), p1 = ld4(P + o + 8 * q + 4);
-    const dbl4 k0 = ld4(KPm + o + 8 * q), k1 = ld4(KPm + o + 8 * q + 4);
+    dbl4 k0, k1;
+    if (emp) {
+      k0 = lam * p0;
+      k1 = lam * p1;
+    } else {
+      k0 = ld4(KPm + o + 8 * q);
+      k1 = ld4(KPm + o + 8 * q + 4);
+    }
     const dbl4 x0 = ld4(X + o + 4 * q), x1 = ld4(X + o + 16 + 4 * q);
     const dbl4 r0 = ld4(R + o + 4 * q), r1 = ld4(R + o + 16 + 4 * q);
     // one wait for all eight loads: keeps the compiler from sinking the X and
@@ -1396,7 +1426,7 @@ static void launch_reduce_step(const CgState& c, int step, double tolsq, hipStre
 }
 
 void launch_cg_step_a(const CgState& c, hipStream_t s) {
-  launch_dots(c.P, c.KPm, c.nr, c.n, c.partial, c.nblk, s, done_flag(c), !fold_steps(c));
+  launch_dots(c.P, c.KPm, c.nr, c.n, c.partial, c.nblk, s, done_flag(c), !fold_steps(c), c.nxt, c.reg);
   if (fold_steps(c)) launch_reduce_step(c, 0, 0.0, s);
   else hipLaunchKernelGGL(k_cg_step_a, dim3(1), dim3(64), 0, s, c.partial, 1, c.nr, c.small);
 }
@@ -1409,7 +1439,7 @@ void launch_cg_update_xr(const CgState& c, hipStream_t s) {
   if (fused_xr(c)) {
     const int cols = (c.n + c.nblk - 1) / c.nblk;
     hipLaunchKernelGGL(k_cg_update32_xr, dim3(c.nblk), dim3(256), 0, s, c.n, cols, (const double*)c.small, c.X, c.R,
-                       (const double*)c.P, (const double*)c.KPm, c.partial);
+                       (const double*)c.P, (const double*)c.KPm, c.partial, c.nxt, c.reg);
     if (c.nblk > 1 && !fold_steps(c)) launch_partial_reduce(c.partial, c.nblk, 32 * 32, s);
     return;
   }

@@ -948,12 +948,17 @@ CsrDev MacauPrior::Ft() const {
 // Rows listed in dense_rows (ascending) are left to the dense-block kernel:
 // they get no segment and are appended to the split rows with nkc slots
 // each, after all other slots (the dense kernel writes them).
+// Rows flagged in omit get no work at all.
 static CsrDev spmm_plan(const std::vector<int>& rp, int nrmax, std::vector<DevBuf>& store, hipStream_t s,
-                        const std::vector<int>* dense_rows = nullptr, int nkc = 0) {
+                        const std::vector<int>* dense_rows = nullptr, int nkc = 0,
+                        const std::vector<char>* omit = nullptr) {
   const int nrow = (int)rp.size() - 1;
   std::vector<char> is_dense(nrow, 0);
   if (dense_rows)
     for (int r : *dense_rows) is_dense[r] = 1;
+  if (omit)
+    for (int r = 0; r < nrow; r++)
+      if ((*omit)[r]) is_dense[r] = 2;
   std::vector<int4> seg;
   std::vector<int> mrow, mslot0;
   int slots = 0;
@@ -1055,7 +1060,7 @@ void MacauPrior::build_dense(int nrmax, hipStream_t s) {
   for (int r = 0; r < nrow; r++)
     for (int t = 0; t < kDenseRem && rptr[r] + t < rptr[r + 1]; t++) rfix[(size_t)r * kDenseRem + t] = rcols[rptr[r] + t];
   if (rcols.empty()) rcols.push_back(0);
-  dense_bufs.resize(6);
+  dense_bufs.resize(7);
   upload(dense_bufs[5], rfix, s);
   upload(dense_bufs[0], dcol, s);
   upload(dense_bufs[1], rbits, s);
@@ -1076,7 +1081,24 @@ void MacauPrior::build_dense(int nrmax, hipStream_t s) {
   Fd_plan.rem_cols = dense_bufs[3].as<int>();
   Fd_plan.rem_fix = dense_bufs[5].as<int>();
   const int nkc = (nrow + kDenseRowChunk - 1) / kDenseRowChunk;
-  Ftd_plan = spmm_plan(Ft_rowptr, nrmax, Ftd_plan_bufs, s, &cand, nkc);
+  // columns of F without nonzeros, none within kEmptyReach columns either
+  // way: no work unit (the CG forms their KP rows as reg * P)
+  std::vector<int> nxt(nfeat + 1, nfeat);
+  for (int j = nfeat - 1; j >= 0; j--) nxt[j] = Ft_rowptr[j + 1] > Ft_rowptr[j] ? j : nxt[j + 1];
+  std::vector<char> omit(nfeat, 0);
+  {
+    int prev = -(1 << 30);  // the last column with nonzeros before j
+    for (int j = 0; j < nfeat; j++) {
+      if (Ft_rowptr[j + 1] > Ft_rowptr[j]) {
+        prev = j;
+        continue;
+      }
+      omit[j] = (j - prev > kEmptyReach && nxt[j] - j > kEmptyReach) ? 1 : 0;
+    }
+  }
+  upload(dense_bufs[6], nxt, s);
+  Ftd_plan = spmm_plan(Ft_rowptr, nrmax, Ftd_plan_bufs, s, &cand, nkc, &omit);
+  Ftd_plan.next_nonempty = dense_bufs[6].as<int>();
   Ftd_plan.dense_role = 2;
   Ftd_plan.ndp = ndp;
   Ftd_plan.nd = nd;
@@ -1207,6 +1229,10 @@ int blockcg_device(const CsrDev& F0, const CsrDev& Ft0, const double* reg, const
   c.partial = b.partial.as<double>();
   c.nblk = dots_blocks(nfeat);
   c.small = b.small.as<double>();
+  if (dense) {
+    c.nxt = Ft.next_nonempty;
+    c.reg = reg;
+  }
   const double* Bb = B;
   if (nr != ld) {
     launch_rows_copy(B, ld, b.B.as<double>(), nr, r0, 0, nr, nfeat, s);
