@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(kSpmmThreads)
 
 // F P for a binary F with a dense column block, nr = 32 (dense_role 1): the
 // dense columns through FP64 matrix-core products, the other nonzeros by
-// gathers.  A work-group (4 waves) takes 32 rows of F; wave (tile, half)
+// gathers.  A work-group (8 waves) takes 64 rows of F; wave (tile, half)
 // owns rows 16 tile .. + 15 and components 16 half .. + 15 of them.  The
 // work-group first copies the in-rows of the dense columns into LDS as
 // [half][j][16] (a k-step's four rows of 16 doubles: two bank halves, each
@@ -319,16 +319,17 @@ __global__ void __launch_bounds__(kSpmmThreads)
 // products.  All loads that need no earlier load are issued before the
 // convergence flag is tested.  Sums run in the matrix core's order, then
 // the remaining nonzeros in CSR order.
-constexpr int kDenseStage = kDenseMaxCols * 16 / 256;  // 16-byte copies per thread
+constexpr int kDenseFThreads = 512;  // 8 waves: 64 rows of F per work-group
+constexpr int kDenseStage = kDenseMaxCols * 16 / kDenseFThreads;  // 16-byte copies per thread
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kDenseFThreads)
     k_spmm_dense_f(CsrDev A, const double* __restrict__ in, double* __restrict__ out, const double* skip) {
   __shared__ double sP[2 * kDenseMaxCols * 16];
   const double sk = skip ? *skip : 0.0;
   const int ndp = A.ndp;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
   const int half = wv & 1;
-  const int r0 = blockIdx.x * 32 + (wv >> 1) * 16;
+  const int r0 = blockIdx.x * (kDenseFThreads / 8) + (wv >> 1) * 16;
   const int ra = min(r0 + m, A.nfrow - 1);  // rows past the end read the last row and are not stored
   // loads at clamped addresses, unused values dropped afterwards: no branch
   // around a load, so they are all in flight together
@@ -337,7 +338,8 @@ __global__ void __launch_bounds__(256)
   for (int u = 0; u < kDenseMaxCols / 32; u++) wd[u] = A.row_bits[(size_t)ra * A.ndw + min(u, A.ndw - 1)];
   int col[kDenseStage];
 #pragma unroll
-  for (int u = 0; u < kDenseStage; u++) col[u] = A.dcol[min((int)threadIdx.x + 256 * u, ndp * 16 - 1) >> 4];
+  for (int u = 0; u < kDenseStage; u++)
+    col[u] = A.dcol[min((int)threadIdx.x + kDenseFThreads * u, ndp * 16 - 1) >> 4];
   // the lane's output rows 4 v + l / 16
   int rb[4], re[4];
   int4 rc[4][kDenseRem / 4];
@@ -354,7 +356,7 @@ __global__ void __launch_bounds__(256)
   double2 cv[kDenseStage];
 #pragma unroll
   for (int u = 0; u < kDenseStage; u++) {
-    const int e = threadIdx.x + 256 * u;
+    const int e = threadIdx.x + kDenseFThreads * u;
     cv[u] = reinterpret_cast<const double2*>(in + (size_t)col[u] * 32)[e & 15];
   }
   double rv[4][kDenseRem];
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(256)
   for (int u = 0; u < kDenseStage; u++) asm volatile("" ::"v"(cv[u].x), "v"(cv[u].y));
 #pragma unroll
   for (int u = 0; u < kDenseStage; u++) {
-    const int e = threadIdx.x + 256 * u, j = e >> 4, p = e & 15;
+    const int e = threadIdx.x + kDenseFThreads * u, j = e >> 4, p = e & 15;
     if (e < ndp * 16) *reinterpret_cast<double2*>(sP + ((size_t)(p >> 3) * ndp + j) * 16 + 2 * (p & 7)) = cv[u];
   }
 #pragma unroll
@@ -506,7 +508,8 @@ static void launch_spmm_t(const CsrDev& A, const double* in, int nr, double* out
                           const double* add, hipStream_t s, const double* skip) {
   const bool aligned = (((uintptr_t)in | (uintptr_t)out | (uintptr_t)add | (uintptr_t)A.partial) & 15) == 0;
   if (A.dense_role == 1 && nr == 32 && !add && aligned) {
-    hipLaunchKernelGGL(k_spmm_dense_f, dim3((A.nfrow + 31) / 32), dim3(256), 0, s, A, in, out, skip);
+    hipLaunchKernelGGL(k_spmm_dense_f, dim3((A.nfrow + kDenseFThreads / 8 - 1) / (kDenseFThreads / 8)), dim3(kDenseFThreads), 0,
+                       s, A, in, out, skip);
     return;
   }
   if (A.dense_role == 2 && nr == 32 && aligned && A.nd > 0) {
