@@ -281,6 +281,48 @@ def test_model_macau_matches_oracle():
     assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_model_macau_dense_block_matches_oracle(graph):
+    """K = 32 with binary side information that has 40 dense columns, sparse
+    columns among the first 1000 and 2000 featureless ones: the block CG's
+    dense-block products and its K P = lambda P shortcut inside the Gibbs
+    loop (eager, and as replayed chunks of CG iterations)."""
+    announce("model macau dense block graph=%s" % graph)
+    rng = np.random.default_rng(21)
+    nrow, ncol, K, nfeat = 1200, 150, 32, 3000
+    Fd = np.zeros((nrow, nfeat), dtype=bool)
+    dense = rng.choice(1000, 40, replace=False)
+    Fd[:, dense] = rng.random((nrow, 40)) < 0.3
+    Fd[:, :1000] |= rng.random((nrow, 1000)) < 0.004
+    fr, fc = np.nonzero(Fd)
+    W = rng.standard_normal((nfeat, 3)) * 0.3
+    Ut = Fd.astype(float) @ W
+    Vt = rng.standard_normal((ncol, 3))
+    mask = rng.random((nrow, ncol)) < 0.1
+    r, c = np.nonzero(mask)
+    y = np.sum(Ut[r] * Vt[c], 1) + 0.3 * rng.standard_normal(len(r))
+    perm = rng.permutation(len(r))
+    te, tr = perm[:len(r) // 5], perm[len(r) // 5:]
+    m = GpuModel(2, K, 10.0)
+    m.data(np.stack([r[tr], c[tr]], 1), y[tr], [nrow, ncol], np.stack([r[te], c[te]], 1), y[te])
+    m.prior(side=(fr, fc, None, nrow, nfeat), lambda_beta=5.0, tol=1e-6)
+    m.prior()
+    m.setup(4, 6, SEED, PLAIN, graph=graph)
+    modes, mean, nnz = O.matrix_rows(r[tr], c[tr], y[tr], nrow, ncol)
+    ti, tv = O.matrix_test(r[te], c[te], y[te], nrow, ncol)
+    side = [O.csr_pair(fr, fc, np.ones(len(fr)), nrow, nfeat), None]
+    kw = dict(precision=10.0, side=side, lambda_beta=5.0, tol=1e-6, burnin=4, nsamples=6, seed=SEED)
+    m.iterate(1)
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, max_iters=1, want_U=True, **kw)
+    assert row_rel_err(m.latents(0, nrow), ref.U[0]) < TOL_FP32
+    assert row_rel_err(m.latents(1, ncol), ref.U[1]) < TOL_FP32
+    lb = _lib.lib().macau_get_link_lambda(m.h, 0)
+    assert abs(lb - ref.lambda_beta[0]) / ref.lambda_beta[0] < TOL_FP32
+    m.iterate(4)
+    ref = O.run(modes, mean, nnz, [nrow, ncol], K, ti, tv, max_iters=5, **kw)
+    assert abs(m.rmse() - ref.rmse) / ref.rmse < 0.005
+
+
 @pytest.mark.parametrize("solver", ["spectral", "cholesky"])
 @pytest.mark.parametrize("nfeat,binary", [(70, True), (130, False), (33, False), (700, True)])
 def test_model_macau_ftf_matches_oracle(nfeat, binary, solver, monkeypatch):
