@@ -510,6 +510,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--sampler", choices=["batched", "batched-norefine", "plain"], default=None,
                     help="batched-norefine: the batched sampler without the condition probes and refinement")
+    ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="1..32",
+                    help="hardware queues per process for --workload c5 (GPU_MAX_HW_QUEUES)")
     ap.add_argument("--tune", action="store_true",
                     help="time the default launch shapes, run the autotuner, time the tuned shapes")
     args = ap.parse_args()
@@ -526,7 +528,7 @@ def main():
         # process (HIP's default is 4) they serialise least (batch it/s with
         # 4, 8, 12, 16, 32 queues: 24.7, 23.8, 24.8, 27.2, 12.8); read when the
         # library first touches the GPU, so set before that
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
         return bench_c5(args, log)
     t0 = time.perf_counter()
     w = load_workload(args.workload, args.seed)
